@@ -1,0 +1,217 @@
+// pm_grid.hip -- GPU uniform grid: the replacement of CompactNSearch's
+// add_point_set / z_sort / sort_field / update_point_sets (Dataset.cpp:49-76)
+// and of Frame::ComputeAABB (Dataset.cpp:78-92).
+//
+// Pipeline (one stream, no host round trip except the extent read-back):
+//   bounds reduce -> cell id + histogram (atomics) -> exclusive scan ->
+//   scatter into cell-major float4 -> per-cell sort by input index.
+// The per-cell sort makes the stored order deterministic (cell, input index),
+// which is the canonical neighbour order the density sums follow.
+// All passes are HBM-bound streaming passes over n particles or C cells.
+#include "pm_internal.h"
+
+#include <float.h>
+
+namespace pm {
+
+static constexpr int kThreads = 256;
+
+uint32_t bounds_partials(uint64_t n)
+{
+    uint64_t b = (n + kThreads - 1) / kThreads;
+    if (b > 1024) b = 1024;
+    if (b < 1) b = 1;
+    return (uint32_t)b;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bounds_partial(const float *__restrict__ xyz, uint64_t n,
+                                                             float *__restrict__ partial)
+{
+    float mn[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, mx[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads) {
+        for (int a = 0; a < 3; a++) {
+            float v = xyz[3 * i + a];
+            mn[a] = fminf(mn[a], v);
+            mx[a] = fmaxf(mx[a], v);
+        }
+    }
+    __shared__ float red[6][kThreads];
+    for (int a = 0; a < 3; a++) { red[a][threadIdx.x] = mn[a]; red[3 + a][threadIdx.x] = mx[a]; }
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int a = 0; a < 3; a++) {
+                red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + s]);
+                red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_bounds_final(const float *__restrict__ partial, uint32_t np,
+                                                           float *__restrict__ out)
+{
+    __shared__ float red[6][kThreads];
+    float v[6] = { FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX };
+    for (uint32_t i = threadIdx.x; i < np; i += kThreads)
+        for (int a = 0; a < 3; a++) {
+            v[a] = fminf(v[a], partial[6 * i + a]);
+            v[3 + a] = fmaxf(v[3 + a], partial[6 * i + 3 + a]);
+        }
+    for (int a = 0; a < 6; a++) red[a][threadIdx.x] = v[a];
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int a = 0; a < 3; a++) {
+                red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + s]);
+                red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t launch_bounds_reduce(const float *xyz, uint64_t n, float *partial, uint32_t npartial, float *out,
+                                hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bounds_partial, dim3(npartial), dim3(kThreads), 0, st, xyz, n, partial);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kThreads), 0, st, partial, npartial, out);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_cell_count(GridBuildArgs a)
+{
+    uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= a.n) return;
+    int32_t cx = pm_key(a.xyz[3 * i + 0], a.inv_h) - a.ox;
+    int32_t cy = pm_key(a.xyz[3 * i + 1], a.inv_h) - a.oy;
+    int32_t cz = pm_key(a.xyz[3 * i + 2], a.inv_h) - a.oz;
+    uint32_t c = (uint32_t)cx + (uint32_t)a.nx * ((uint32_t)cy + (uint32_t)a.ny * (uint32_t)cz);
+    a.cell_of[i] = c;
+    atomicAdd(&a.counts[c], 1u);
+}
+
+// ---- exclusive scan of u32 (len up to 2^31), 1024 elements per block ----
+static constexpr int kScanPer = 4;
+static constexpr int kScanBlock = kThreads * kScanPer;
+
+size_t scan_tmp_size(uint32_t len)
+{
+    return (size_t)((len + kScanBlock - 1) / kScanBlock) + 1;
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t &total)
+{
+    // Hillis-Steele over 256 thread totals in LDS.
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kThreads; off <<= 1) {
+        uint32_t add = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    total = sh[kThreads - 1];
+    uint32_t incl = sh[threadIdx.x];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_blocks(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                          uint32_t len, uint32_t *__restrict__ block_sums)
+{
+    __shared__ uint32_t sh[kThreads];
+    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    uint32_t v[kScanPer], sum = 0;
+    for (int j = 0; j < kScanPer; j++) {
+        v[j] = base + j < len ? in[base + j] : 0u;
+        sum += v[j];
+    }
+    uint32_t total;
+    uint32_t ex = block_excl_scan(sum, sh, total);
+    for (int j = 0; j < kScanPer; j++) {
+        if (base + j < len) out[base + j] = ex;
+        ex += v[j];
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_sums(uint32_t *__restrict__ sums, uint32_t nb)
+{
+    __shared__ uint32_t sh[kThreads];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += kThreads) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t total;
+        uint32_t ex = block_excl_scan(v, sh, total);
+        if (i < nb) sums[i] = ex + carry;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_add(uint32_t *__restrict__ out, uint32_t len,
+                                                       const uint32_t *__restrict__ sums)
+{
+    uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    uint32_t add = sums[blockIdx.x];
+    for (int j = 0; j < kScanPer; j++)
+        if (base + j < len) out[base + j] += add;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scatter(GridBuildArgs a)
+{
+    uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= a.n) return;
+    uint32_t c = a.cell_of[i];
+    uint32_t slot = a.cell_start[c] + atomicAdd(&a.cursor[c], 1u);
+    a.sorted[slot] = make_float4(a.xyz[3 * i + 0], a.xyz[3 * i + 1], a.xyz[3 * i + 2], __uint_as_float((uint32_t)i));
+}
+
+// Insertion sort of each cell's particles by input index (cells hold O(10-100)).
+__global__ __launch_bounds__(kThreads) void k_cell_sort(float4 *__restrict__ sorted, const uint32_t *__restrict__ cell_start,
+                                                        uint32_t ncells)
+{
+    uint32_t c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= ncells) return;
+    uint32_t b = cell_start[c], e = cell_start[c + 1];
+    for (uint32_t i = b + 1; i < e; i++) {
+        float4 v = sorted[i];
+        uint32_t key = __float_as_uint(v.w);
+        uint32_t j = i;
+        while (j > b && __float_as_uint(sorted[j - 1].w) > key) {
+            sorted[j] = sorted[j - 1];
+            j--;
+        }
+        sorted[j] = v;
+    }
+}
+
+hipError_t launch_grid_build(const GridBuildArgs &a, hipStream_t st)
+{
+    uint32_t len = a.ncells + 1;
+    uint32_t fill_blocks = (uint32_t)((len + kThreads - 1) / kThreads);
+    if (fill_blocks > 65535) fill_blocks = 65535;
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(kThreads), 0, st, a.counts, (uint64_t)len, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(kThreads), 0, st, a.cursor, (uint64_t)a.ncells, 0u);
+    uint32_t pblocks = (uint32_t)((a.n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_cell_count, dim3(pblocks), dim3(kThreads), 0, st, a);
+    uint32_t nb = (len + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kThreads), 0, st, a.counts, a.cell_start, len, a.scan_tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, st, a.scan_tmp, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kThreads), 0, st, a.cell_start, len, a.scan_tmp);
+    hipLaunchKernelGGL(k_scatter, dim3(pblocks), dim3(kThreads), 0, st, a);
+    uint32_t cblocks = (a.ncells + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(k_cell_sort, dim3(cblocks), dim3(kThreads), 0, st, a.sorted, a.cell_start, a.ncells);
+    return hipGetLastError();
+}
+
+}  // namespace pm

@@ -1,0 +1,213 @@
+"""ctypes binding of libpmarch.so (the C ABI declared in include/pmarch.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI.  There is no
+CPU fallback: if the library is missing, or no GPU is present, these wrappers
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import re
+
+import numpy as np
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent.parent          # bachelor-thesis_amd/
+ROOT = PKG_DIR.parent
+LIB_PATH = PKG_DIR / "libpmarch.so"
+HEADER = ROOT / "include" / "pmarch.h"
+
+PM_OK, PM_PENDING = 0, 1
+MODE_CONFIG, MODE_REFERENCE = 0, 1
+NORMAL_CDIFF, NORMAL_GRADW = 0, 1
+SKIP_NONE, SKIP_EMPTY, SKIP_DG_BOX, SKIP_DG_CONS = 0, 1, 2, 3
+
+
+class PmSettings(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32), ("max_steps", C.c_int32), ("step", C.c_float), ("iso", C.c_float),
+        ("bisect_iters", C.c_int32), ("final_fp64", C.c_int32), ("normal_eps", C.c_float),
+        ("normal_mode", C.c_int32), ("skip_mode", C.c_int32), ("compensated", C.c_int32),
+        ("learned", C.c_int32), ("reserved", C.c_int32 * 5),
+    ]
+
+
+class PmStats(C.Structure):
+    _fields_ = [
+        ("rays", C.c_uint64), ("rays_in_aabb", C.c_uint64), ("hits", C.c_uint64), ("samples", C.c_uint64),
+        ("cand", C.c_uint64), ("march_samples", C.c_uint64), ("mlp_evals", C.c_uint64), ("reserved", C.c_uint64),
+        ("kernel_ms", C.c_float), ("total_ms", C.c_float),
+    ]
+
+    def as_dict(self):
+        return {k: (float(getattr(self, k)) if k.endswith("_ms") else int(getattr(self, k)))
+                for k, _ in self._fields_ if k != "reserved"}
+
+
+class PmFrameInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64), ("ncells", C.c_uint64), ("origin", C.c_int32 * 3), ("dims", C.c_int32 * 3),
+        ("h", C.c_float), ("pmin", C.c_float * 3), ("pmax", C.c_float * 3), ("bmin", C.c_float * 3),
+        ("bmax", C.c_float * 3), ("build_ms", C.c_float),
+    ]
+
+
+_lib = None
+
+
+def header_functions() -> list[str]:
+    """Names of the functions include/pmarch.h declares."""
+    txt = HEADER.read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pm_[a-z0-9_]+)\s*\(", txt)))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"libpmarch.so not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(str(LIB_PATH))
+    vp, u32, u64, i32, f32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_float
+    L.pm_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.pm_destroy.argtypes = [vp]
+    L.pm_destroy.restype = None
+    L.pm_last_error.argtypes = [vp]
+    L.pm_last_error.restype = C.c_char_p
+    L.pm_set_frame.argtypes = [vp, i32, vp, u64, f32]
+    L.pm_set_frame_device.argtypes = [vp, i32, vp, u64, f32]
+    L.pm_frame_info_get.argtypes = [vp, i32, C.POINTER(PmFrameInfo)]
+    L.pm_get_grid.argtypes = [vp, i32, vp, vp, vp]
+    L.pm_march_async.argtypes = [vp, i32, C.POINTER(PmSettings), vp, vp, u32, u32, vp, vp, vp, vp]
+    L.pm_query.argtypes = [vp]
+    L.pm_wait.argtypes = [vp]
+    L.pm_download.argtypes = [vp, vp, vp, vp]
+    L.pm_get_stats.argtypes = [vp, C.POINTER(PmStats)]
+    L.pm_neighbor_counts.argtypes = [vp, i32, vp, u64, vp, vp]
+    L.pm_density.argtypes = [vp, i32, vp, u64, C.c_int, vp]
+    L.pm_density64.argtypes = [vp, i32, vp, u64, vp]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def settings_from(d: dict) -> PmSettings:
+    s = PmSettings()
+    for k, v in d.items():
+        setattr(s, k, v)
+    return s
+
+
+class PmError(RuntimeError):
+    pass
+
+
+class Marcher:
+    """Thin owner of a pm_ctx (one GPU)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        p = C.c_void_p()
+        rc = self.L.pm_create(device, C.byref(p))
+        if rc != PM_OK:
+            raise PmError(f"pm_create failed ({rc}): no usable GPU")
+        self.ctx = p
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.pm_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise PmError(f"{what} failed ({rc}): {self.L.pm_last_error(self.ctx).decode()}")
+        return rc
+
+    def set_frame(self, frame: int, xyz: np.ndarray, h: float):
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        self._check(self.L.pm_set_frame(self.ctx, frame, _ptr(xyz), xyz.shape[0], float(h)), "pm_set_frame")
+        return self.frame_info(frame)
+
+    def frame_info(self, frame: int) -> PmFrameInfo:
+        fi = PmFrameInfo()
+        self._check(self.L.pm_frame_info_get(self.ctx, frame, C.byref(fi)), "pm_frame_info_get")
+        return fi
+
+    def grid(self, frame: int):
+        fi = self.frame_info(frame)
+        cs = np.zeros(fi.ncells + 1, np.uint32)
+        srt = np.zeros((fi.n, 4), np.float32)
+        idx = np.zeros(fi.n, np.uint32)
+        self._check(self.L.pm_get_grid(self.ctx, frame, _ptr(cs), _ptr(srt), _ptr(idx)), "pm_get_grid")
+        return fi, cs, srt, idx
+
+    def march_async(self, frame: int, settings: dict, inv_pv, cam, W: int, H: int, depth=None,
+                    outputs: bool = True, want_t: bool = True):
+        s = settings_from(settings)
+        inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
+        cam = np.ascontiguousarray(cam, np.float32).reshape(3)
+        pos = nrm = t = None
+        if outputs:
+            pos = np.zeros((H, W, 4), np.float32)
+            nrm = np.zeros((H, W, 4), np.float32)
+            t = np.zeros((H, W), np.float32) if want_t else None
+        if depth is not None:
+            depth = np.ascontiguousarray(depth, np.float32)
+        self._keep = dict(s=s, inv_pv=inv_pv, cam=cam, depth=depth)
+        self._check(self.L.pm_march_async(self.ctx, frame, C.byref(s), _ptr(inv_pv), _ptr(cam), W, H, _ptr(depth),
+                                          _ptr(pos), _ptr(nrm), _ptr(t)), "pm_march_async")
+        return pos, nrm, t
+
+    def query(self) -> int:
+        return self._check(self.L.pm_query(self.ctx), "pm_query")
+
+    def wait(self):
+        self._check(self.L.pm_wait(self.ctx), "pm_wait")
+
+    def march(self, frame: int, settings: dict, inv_pv, cam, W: int, H: int, depth=None, want_t=True):
+        pos, nrm, t = self.march_async(frame, settings, inv_pv, cam, W, H, depth, True, want_t)
+        self.wait()
+        return pos, nrm, t, self.stats()
+
+    def stats(self) -> dict:
+        st = PmStats()
+        self._check(self.L.pm_get_stats(self.ctx, C.byref(st)), "pm_get_stats")
+        return st.as_dict()
+
+    def download(self, W, H):
+        pos = np.zeros((H, W, 4), np.float32)
+        nrm = np.zeros((H, W, 4), np.float32)
+        t = np.zeros((H, W), np.float32)
+        self._check(self.L.pm_download(self.ctx, _ptr(pos), _ptr(nrm), _ptr(t)), "pm_download")
+        return pos, nrm, t
+
+    def neighbor_counts(self, frame: int, q):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+        cnt = np.zeros(q.shape[0], np.uint32)
+        cand = np.zeros(q.shape[0], np.uint32)
+        self._check(self.L.pm_neighbor_counts(self.ctx, frame, _ptr(q), q.shape[0], _ptr(cnt), _ptr(cand)),
+                    "pm_neighbor_counts")
+        return cnt, cand
+
+    def density(self, frame: int, q, compensated=True):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+        rho = np.zeros(q.shape[0], np.float32)
+        self._check(self.L.pm_density(self.ctx, frame, _ptr(q), q.shape[0], int(compensated), _ptr(rho)), "pm_density")
+        return rho
+
+    def density64(self, frame: int, q):
+        q = np.ascontiguousarray(q, np.float64).reshape(-1, 3)
+        rho = np.zeros(q.shape[0], np.float64)
+        self._check(self.L.pm_density64(self.ctx, frame, _ptr(q), q.shape[0], _ptr(rho)), "pm_density64")
+        return rho

@@ -1,0 +1,146 @@
+/*
+ * pmarch.h -- C ABI of the MI355X particle-fluid ray marcher (libpmarch.so).
+ *
+ * Replaces the reference's CPU marcher behind the same launch/poll protocol:
+ *   class RayMarcher            src/app/AdvancedRenderer/RayMarcher.h:28-77
+ *     Prepare(...)              src/app/AdvancedRenderer/RayMarcher.cpp:76-100
+ *     Start()                   src/app/AdvancedRenderer/RayMarcher.cpp:102-112
+ *     IsDone()                  src/app/AdvancedRenderer/RayMarcher.h:51 -> ThreadPool.cpp:33-36
+ *     Exit()                    src/app/AdvancedRenderer/RayMarcher.cpp:71-74
+ *   struct VisualizationSettings src/app/AdvancedRenderer/RayMarcher.h:12-26
+ *   Frame::Frame / BuildSearch / ComputeAABB / BuildDensityGrid
+ *                               src/app/Dataset.cpp:9-24, 49-164
+ *   Dataset::GetNeighbors       src/app/Dataset.cpp:272-280 (CompactNSearch::find_neighbors)
+ *
+ * Conventions: every call returns an int status (PM_OK = 0, PM_PENDING = 1,
+ * negative = error, message in pm_last_error); no exception crosses the ABI.
+ * Outputs of pm_march_async are valid only after pm_query returns PM_OK or
+ * pm_wait returns (this replaces the reference's racy IsDone counter poll).
+ * One host thread per context.  Images are row-major, index = y * W + x;
+ * positions and normals are RGBA32F (16 B per pixel, as BilateralBuffer::Color,
+ * BilateralBuffer.cpp:18); a miss writes (0,0,0,0) to both.
+ */
+#ifndef PMARCH_H
+#define PMARCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_OK          0
+#define PM_PENDING     1
+#define PM_ERR_ARG    (-1)
+#define PM_ERR_HIP    (-2)
+#define PM_ERR_FRAME  (-3)
+#define PM_ERR_SIZE   (-4)
+#define PM_ERR_NODEV  (-5)
+#define PM_ERR_STATE  (-6)
+
+#define PM_MODE_CONFIG     0  /* AABB entry, fixed step, bisection, gradient normals   */
+#define PM_MODE_REFERENCE  1  /* PerPixel_Isotropic, RayMarcher.cpp:256-344            */
+
+#define PM_NORMAL_CDIFF    0  /* normalize(-grad rho) by central differences           */
+#define PM_NORMAL_GRADW    1  /* normalize(sum gradW), Kernel.cpp:34-52 (with 1/|r|)   */
+
+#define PM_SKIP_NONE       0
+#define PM_SKIP_EMPTY      1  /* configured mode: skip samples whose 27 cells are empty */
+#define PM_SKIP_DG_BOX     2  /* reference mode: DensityGrid, box flag                 */
+#define PM_SKIP_DG_CONS    3  /* reference mode: DensityGrid, conservative flag        */
+
+/* March settings.  The first four fields carry VisualizationSettings'
+ * MaxSteps / StepSize / IsoDensity (RayMarcher.h:16-18); the rest select the
+ * configured march.  Layout is fixed: 16 x 4 bytes. */
+typedef struct pm_settings {
+    int32_t mode;          /* PM_MODE_*                                              */
+    int32_t max_steps;     /* reference: MaxSteps; configured: cap on march samples, 0 = none */
+    float   step;          /* step length in world units (> 0)                        */
+    float   iso;           /* iso density in kernel units                             */
+    int32_t bisect_iters;  /* configured: bisection iterations on the crossing bracket */
+    int32_t final_fp64;    /* configured: last bisection iteration evaluated in FP64  */
+    float   normal_eps;    /* configured: central-difference offset (world units)     */
+    int32_t normal_mode;   /* PM_NORMAL_*                                             */
+    int32_t skip_mode;     /* PM_SKIP_*                                               */
+    int32_t compensated;   /* 1 = Neumaier-compensated FP32 density sums              */
+    int32_t learned;       /* 1 = learned step proposal (needs pm_set_mlp)            */
+    int32_t reserved[5];
+} pm_settings;
+
+/* Per-launch counters, written by the march kernel itself. */
+typedef struct pm_stats {
+    uint64_t rays;          /* primary rays (W*H per view)                   */
+    uint64_t rays_in_aabb;  /* rays that enter the padded particle AABB      */
+    uint64_t hits;          /* rays with an iso crossing                     */
+    uint64_t samples;       /* density evaluations of every kind             */
+    uint64_t cand;          /* sum over samples of particles in the 27 cells */
+    uint64_t march_samples; /* density evaluations of the march phase        */
+    uint64_t mlp_evals;     /* learned step: network evaluations             */
+    uint64_t reserved;
+    float    kernel_ms;     /* march kernel time, HIP events on the context stream */
+    float    total_ms;      /* launch to completion incl. copies             */
+} pm_stats;
+
+/* Frame description after pm_set_frame (the GPU grid of Frame::BuildSearch). */
+typedef struct pm_frame_info {
+    uint64_t n;             /* particles                                     */
+    uint64_t ncells;        /* dense cells                                   */
+    int32_t  origin[3];     /* cell key of dense cell (0,0,0)                */
+    int32_t  dims[3];       /* dense grid dimensions                         */
+    float    h;             /* support = cell width (ParticleRadius)         */
+    float    pmin[3], pmax[3];  /* particle bounds                           */
+    float    bmin[3], bmax[3];  /* padded AABB (Frame::ComputeAABB, Dataset.cpp:78-92) */
+    float    build_ms;      /* grid build time on the GPU                    */
+} pm_frame_info;
+
+typedef struct pm_ctx pm_ctx;
+
+/* RayMarcher::RayMarcher (RayMarcher.cpp:66-69): opens device `device`. */
+int  pm_create(int device, pm_ctx **out);
+/* RayMarcher::Exit (RayMarcher.cpp:71-74). */
+void pm_destroy(pm_ctx *ctx);
+const char *pm_last_error(const pm_ctx *ctx);
+
+/* Frame::Frame (Dataset.cpp:9-24): copies n float3 positions (the caller keeps
+ * ownership) and builds the uniform grid, AABB and skip structures on the GPU. */
+int pm_set_frame(pm_ctx *ctx, int32_t frame, const float *xyz, uint64_t n, float h);
+/* Same, from positions already in device memory (frame streaming). */
+int pm_set_frame_device(pm_ctx *ctx, int32_t frame, const void *d_xyz, uint64_t n, float h);
+int pm_frame_info_get(pm_ctx *ctx, int32_t frame, pm_frame_info *out);
+/* The frame's grid: cell_start[ncells+1], sorted[4n] (x,y,z,0), index[n]
+ * (input index of each stored particle).  Any pointer may be NULL. */
+int pm_get_grid(pm_ctx *ctx, int32_t frame, uint32_t *cell_start, float *sorted, uint32_t *index);
+
+/* RayMarcher::Prepare + Start (RayMarcher.cpp:76-112).  inv_pv is
+ * Camera3D::GetInvProjectionView (column-major 16 floats), cam is
+ * CameraController3D::Position.  depth_in (W*H floats) is required in
+ * PM_MODE_REFERENCE and ignored otherwise.  Output pointers are host memory;
+ * any may be NULL, and with all three NULL the results stay on the device
+ * (pm_download).  Returns immediately. */
+int pm_march_async(pm_ctx *ctx, int32_t frame, const pm_settings *s, const float inv_pv[16],
+                   const float cam[3], uint32_t W, uint32_t H, const float *depth_in,
+                   float *positions_out, float *normals_out, float *t_out);
+/* RayMarcher::IsDone: PM_OK when the march and its copies are complete,
+ * PM_PENDING while running, negative on error. */
+int pm_query(pm_ctx *ctx);
+/* Blocks until the pending march is complete. */
+int pm_wait(pm_ctx *ctx);
+/* Copies the last march's device results to host memory (any pointer NULL). */
+int pm_download(pm_ctx *ctx, float *positions, float *normals, float *t);
+int pm_get_stats(pm_ctx *ctx, pm_stats *out);
+
+/* Dataset::GetNeighbors (Dataset.cpp:272-280) as counts: for each query point,
+ * the number of particles with |q - x_j|^2 < h^2 and of stencil candidates. */
+int pm_neighbor_counts(pm_ctx *ctx, int32_t frame, const float *q, uint64_t nq,
+                       uint32_t *counts, uint32_t *ncand);
+/* Density sum_j W(x_j - q) (RayMarcher.cpp:321-325) at query points, FP32
+ * (compensated or plain) and FP64. */
+int pm_density(pm_ctx *ctx, int32_t frame, const float *q, uint64_t nq, int compensated,
+               float *rho);
+int pm_density64(pm_ctx *ctx, int32_t frame, const double *q, uint64_t nq, double *rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

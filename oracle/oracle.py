@@ -1,0 +1,327 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product path (bachelor-thesis_amd/) never
+imports this module.
+
+Each wrapper names the reference function its C counterpart restates; the C
+source (oracle/pm_oracle.c) carries the file:line citations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+RESTING_LATTICE_SUM = 3.1415061532636885
+"""sum_j W(x_j)/sigma over a perfect lattice of spacing h/2 (27 non-zero terms)."""
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("max_steps", C.c_int32),
+        ("step", C.c_float),
+        ("iso", C.c_float),
+        ("bisect_iters", C.c_int32),
+        ("final_fp64", C.c_int32),
+        ("normal_eps", C.c_float),
+        ("normal_mode", C.c_int32),
+        ("skip_mode", C.c_int32),
+        ("compensated", C.c_int32),
+    ]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("rays", C.c_uint64),
+        ("rays_in_aabb", C.c_uint64),
+        ("hits", C.c_uint64),
+        ("samples", C.c_uint64),
+        ("cand", C.c_uint64),
+        ("march_samples", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class GridStruct(C.Structure):
+    _fields_ = [
+        ("h", C.c_float), ("inv_h", C.c_float),
+        ("pmin", C.c_float * 3), ("pmax", C.c_float * 3),
+        ("bmin", C.c_float * 3), ("bmax", C.c_float * 3),
+        ("origin", C.c_int32 * 3), ("dims", C.c_int32 * 3),
+        ("ncells", C.c_uint64), ("n", C.c_uint64),
+        ("cell_start", C.POINTER(C.c_uint32)),
+        ("sorted", C.POINTER(C.c_float)),
+        ("sorted_index", C.POINTER(C.c_uint32)),
+        ("dg_dims", C.c_int32 * 3),
+        ("dg_flag_box", C.POINTER(C.c_uint8)),
+        ("dg_flag_cons", C.POINTER(C.c_uint8)),
+        ("dg_count_box", C.POINTER(C.c_uint32)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        fp = np.ctypeslib.ndpointer
+        L.or_W_f.restype = C.c_float
+        L.or_W_f.argtypes = [C.c_float, C.c_float]
+        L.or_W_d.restype = C.c_double
+        L.or_W_d.argtypes = [C.c_double, C.c_double]
+        L.or_sigma_f.restype = C.c_float
+        L.or_sigma_f.argtypes = [C.c_float]
+        L.or_gradW_f.argtypes = [fp(np.float32, flags="C"), C.c_float, fp(np.float32, flags="C")]
+        L.or_key.restype = C.c_int32
+        L.or_key.argtypes = [C.c_float, C.c_float]
+        L.or_grid_struct_size.restype = C.c_uint64
+        L.or_grid_build.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float, C.POINTER(GridStruct)]
+        L.or_grid_free.argtypes = [C.POINTER(GridStruct)]
+        L.or_neighbors_brute.restype = C.c_uint32
+        L.or_neighbors_brute.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float,
+                                         fp(np.float32, flags="C"), fp(np.uint32, flags="C"), C.c_uint32]
+        L.or_neighbors_grid.restype = C.c_uint32
+        L.or_neighbors_grid.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"),
+                                        fp(np.uint32, flags="C"), C.c_uint32, C.POINTER(C.c_uint32)]
+        L.or_neighbors_hash.restype = C.c_uint32
+        L.or_neighbors_hash.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float,
+                                        fp(np.float32, flags="C"), fp(np.uint32, flags="C"), C.c_uint32]
+        L.or_density_batch_f.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), C.c_uint64,
+                                         C.c_int, fp(np.float32, flags="C"), C.c_void_p]
+        L.or_density_batch_d.argtypes = [C.POINTER(GridStruct), fp(np.float64, flags="C"), C.c_uint64,
+                                         fp(np.float64, flags="C")]
+        L.or_march.argtypes = [C.POINTER(GridStruct), C.POINTER(Settings), fp(np.float32, flags="C"),
+                               fp(np.float32, flags="C"), C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                               C.c_int, fp(np.float32, flags="C"), fp(np.float32, flags="C"), C.c_void_p,
+                               C.POINTER(Counters)]
+        L.or_march_f64.argtypes = [C.POINTER(GridStruct), C.POINTER(Settings), fp(np.float32, flags="C"),
+                                   fp(np.float32, flags="C"), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                                   fp(np.float64, flags="C"), fp(np.float64, flags="C"), C.c_void_p]
+        L.or_hash_build.restype = C.c_void_p
+        L.or_hash_build.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float]
+        L.or_hash_free.argtypes = [C.c_void_p]
+        L.or_march_hashed.argtypes = [C.c_void_p, C.POINTER(GridStruct), C.POINTER(Settings),
+                                      fp(np.float32, flags="C"), fp(np.float32, flags="C"), C.c_uint32,
+                                      C.c_uint32, C.c_uint32, C.c_int, fp(np.float32, flags="C"),
+                                      fp(np.float32, flags="C"), C.c_void_p, C.POINTER(Counters)]
+        L.or_ray.argtypes = [fp(np.float32, flags="C"), fp(np.float32, flags="C"), C.c_uint32, C.c_uint32,
+                             C.c_uint32, fp(np.float32, flags="C"), fp(np.float32, flags="C")]
+        L.or_slab.argtypes = [fp(np.float32, flags="C")] * 5
+        assert L.or_grid_struct_size() == C.sizeof(GridStruct), "GridStruct layout mismatch"
+        _lib = L
+    return _lib
+
+
+def sigma(h: float) -> float:
+    return float(lib().or_sigma_f(h))
+
+
+def W(r2: float, h: float) -> float:
+    """CubicSplineKernel::W (Kernel.cpp:16-32) on |r|^2, FP32."""
+    return float(lib().or_W_f(r2, h))
+
+
+def W64(r2: float, h: float) -> float:
+    return float(lib().or_W_d(r2, h))
+
+
+def gradW(r, h: float) -> np.ndarray:
+    """CubicSplineKernel::gradW (Kernel.cpp:34-52), FP32, with its extra 1/|r|."""
+    out = np.zeros(3, np.float32)
+    lib().or_gradW_f(np.ascontiguousarray(r, np.float32), h, out)
+    return out
+
+
+def key(x: float, h: float) -> int:
+    return int(lib().or_key(np.float32(x), np.float32(1.0) / np.float32(h)))
+
+
+class Grid:
+    """Dense uniform grid (cell = h) with CompactNSearch-style keys."""
+
+    def __init__(self, xyz: np.ndarray, h: float):
+        self.xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        self.h = float(np.float32(h))
+        self.g = GridStruct()
+        rc = lib().or_grid_build(self.xyz.reshape(-1), self.xyz.shape[0], self.h, C.byref(self.g))
+        if rc != 0:
+            raise RuntimeError(f"or_grid_build failed ({rc})")
+
+    def __del__(self):
+        try:
+            lib().or_grid_free(C.byref(self.g))
+        except Exception:
+            pass
+
+    @property
+    def dims(self):
+        return tuple(self.g.dims)
+
+    @property
+    def origin(self):
+        return tuple(self.g.origin)
+
+    @property
+    def bmin(self):
+        return np.array(self.g.bmin[:], np.float32)
+
+    @property
+    def bmax(self):
+        return np.array(self.g.bmax[:], np.float32)
+
+    def cell_start(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.g.cell_start, shape=(self.g.ncells + 1,)).copy()
+
+    def sorted_xyz(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.g.sorted, shape=(self.g.n, 4)).copy()
+
+    def sorted_index(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.g.sorted_index, shape=(self.g.n,)).copy()
+
+    def densitygrid(self):
+        n = int(np.prod(self.g.dg_dims[:]))
+        return (tuple(self.g.dg_dims),
+                np.ctypeslib.as_array(self.g.dg_count_box, shape=(n,)).copy(),
+                np.ctypeslib.as_array(self.g.dg_flag_box, shape=(n,)).copy(),
+                np.ctypeslib.as_array(self.g.dg_flag_cons, shape=(n,)).copy())
+
+    def neighbors(self, q) -> tuple[np.ndarray, int]:
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.zeros(1 << 16, np.uint32)
+        cand = C.c_uint32(0)
+        n = lib().or_neighbors_grid(C.byref(self.g), q, out, out.size, C.byref(cand))
+        return out[:n].copy(), int(cand.value)
+
+    def density(self, q, compensated=True, with_cand=False):
+        q = np.ascontiguousarray(np.asarray(q, np.float32).reshape(-1, 3))
+        rho = np.zeros(q.shape[0], np.float32)
+        cand = np.zeros(q.shape[0], np.uint32)
+        lib().or_density_batch_f(C.byref(self.g), q.reshape(-1), q.shape[0], int(compensated), rho,
+                                 cand.ctypes.data)
+        return (rho, cand) if with_cand else rho
+
+    def density64(self, q):
+        q = np.ascontiguousarray(np.asarray(q, np.float64).reshape(-1, 3))
+        rho = np.zeros(q.shape[0], np.float64)
+        lib().or_density_batch_d(C.byref(self.g), q.reshape(-1), q.shape[0], rho)
+        return rho
+
+
+def neighbors_brute(xyz, h, q) -> np.ndarray:
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    out = np.zeros(1 << 16, np.uint32)
+    n = lib().or_neighbors_brute(xyz, xyz.size // 3, h, np.ascontiguousarray(q, np.float32), out, out.size)
+    return out[:n].copy()
+
+
+def neighbors_hash(xyz, h, q) -> np.ndarray:
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    out = np.zeros(1 << 16, np.uint32)
+    n = lib().or_neighbors_hash(xyz, xyz.size // 3, h, np.ascontiguousarray(q, np.float32), out, out.size)
+    return out[:n].copy()
+
+
+def ray(inv_pv, cam, W: int, H: int, x: int, y: int):
+    """Ray of pixel (x, y): origin = camera, direction = normalize(unproject(clip, 1) - camera)."""
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().or_ray(np.ascontiguousarray(inv_pv, np.float32).reshape(16), np.ascontiguousarray(cam, np.float32),
+                 W, H, y * W + x, o, d)
+    return o, d
+
+
+def slab(o, d, bmin, bmax):
+    tnf = np.zeros(2, np.float32)
+    ok = lib().or_slab(*(np.ascontiguousarray(v, np.float32) for v in (o, d, bmin, bmax)), tnf)
+    return bool(ok), float(tnf[0]), float(tnf[1])
+
+
+def make_settings(**kw) -> Settings:
+    s = Settings()
+    defaults = dict(mode=0, max_steps=0, step=0.0, iso=0.0, bisect_iters=8, final_fp64=1,
+                    normal_eps=0.0, normal_mode=0, skip_mode=0, compensated=1)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(s, k, v)
+    return s
+
+
+def march(grid: Grid, settings: Settings, inv_pv, cam, W: int, H: int, depth=None, row_step=1,
+          threads=None, want_t=True):
+    """Oracle march, FP32 (configured or reference mode, selected by settings.mode)."""
+    if threads is None:
+        threads = max(1, (os.cpu_count() or 2) - 1)
+    inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
+    cam = np.ascontiguousarray(cam, np.float32).reshape(3)
+    pos = np.zeros((H, W, 4), np.float32)
+    nrm = np.zeros((H, W, 4), np.float32)
+    t = np.full((H, W), -1.0, np.float32) if want_t else None
+    dptr = None
+    if depth is not None:
+        depth = np.ascontiguousarray(depth, np.float32)
+        dptr = depth.ctypes.data
+    cnt = Counters()
+    rc = lib().or_march(C.byref(grid.g), C.byref(settings), inv_pv, cam, W, H, dptr, row_step, threads,
+                        pos.reshape(-1), nrm.reshape(-1), t.ctypes.data if want_t else None, C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError(f"or_march failed ({rc})")
+    return pos, nrm, t, cnt.as_dict()
+
+
+def march64(grid: Grid, settings: Settings, inv_pv, cam, W: int, H: int, row_step=1, threads=None):
+    """FP64 configured march: the ground truth."""
+    if threads is None:
+        threads = max(1, (os.cpu_count() or 2) - 1)
+    inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
+    cam = np.ascontiguousarray(cam, np.float32).reshape(3)
+    pos = np.zeros((H, W, 4), np.float64)
+    nrm = np.zeros((H, W, 4), np.float64)
+    t = np.full((H, W), -1.0, np.float64)
+    rc = lib().or_march_f64(C.byref(grid.g), C.byref(settings), inv_pv, cam, W, H, row_step, threads,
+                            pos.reshape(-1), nrm.reshape(-1), t.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"or_march_f64 failed ({rc})")
+    return pos, nrm, t
+
+
+class HashedSearch:
+    """CompactNSearch-style hashed cell table (27 lookups per query)."""
+
+    def __init__(self, xyz, h):
+        self.xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+        self.h = float(np.float32(h))
+        self.p = lib().or_hash_build(self.xyz, self.xyz.size // 3, self.h)
+
+    def __del__(self):
+        try:
+            lib().or_hash_free(self.p)
+        except Exception:
+            pass
+
+    def march(self, grid: Grid, settings: Settings, inv_pv, cam, W, H, row_step=1, threads=None):
+        if threads is None:
+            threads = max(1, (os.cpu_count() or 2) - 1)
+        inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
+        cam = np.ascontiguousarray(cam, np.float32).reshape(3)
+        pos = np.zeros((H, W, 4), np.float32)
+        nrm = np.zeros((H, W, 4), np.float32)
+        t = np.full((H, W), -1.0, np.float32)
+        cnt = Counters()
+        rc = lib().or_march_hashed(self.p, C.byref(grid.g), C.byref(settings), inv_pv, cam, W, H, row_step,
+                                   threads, pos.reshape(-1), nrm.reshape(-1), t.ctypes.data, C.byref(cnt))
+        if rc != 0:
+            raise RuntimeError(f"or_march_hashed failed ({rc})")
+        return pos, nrm, t, cnt.as_dict()

@@ -1,0 +1,1028 @@
+/*
+ * pm_oracle.c -- CPU oracle for the particle-fluid iso-surface ray march.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker; the product path never links it.
+ *
+ * Build: gcc -O2 -std=c11 -ffp-contract=off -fPIC -shared -pthread (oracle/Makefile).
+ * -ffp-contract=off keeps every a*b+c as two rounded operations, which is what
+ * the HIP kernels do as well, so FP32 results can match bit for bit.
+ *
+ * Parity pins (all in tests/): closed-form kernel values and the perfect-lattice
+ * density sum (A5), brute-force neighbour lists (A4), closed-form ray geometry
+ * (A1/A2), closed-form single-particle roots (A6), radial/planar normals (A7),
+ * and the FP64 march as ground truth.  CompactNSearch's own return order and
+ * find_neighbors_box are not available anywhere here: parity against those two
+ * is unpinned, and the oracle fixes its own canonical order (below).
+ */
+#define _GNU_SOURCE
+#include "pm_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_PI_F 3.14159265358979323846f
+#define OR_PI_D 3.14159265358979323846
+
+/* ------------------------------------------------------------------------- */
+/* SPH cubic spline kernel -- Kernel.cpp:8-14 (constants), :16-32 (W),        */
+/* :34-52 (gradW).  sigma = 8 / (pi h^3), evaluated as ((pi*h)*h)*h.          */
+/* ------------------------------------------------------------------------- */
+
+float or_sigma_f(float h) { return 8.0f / (OR_PI_F * h * h * h); }
+
+static inline float W_f(float r2, float h2, float hinv, float sig)
+{
+    if (r2 >= h2) return 0.0f;                  /* Kernel.cpp:20-21 */
+    float q = sqrtf(r2) * hinv;                 /* :23 */
+    if (q >= 0.5f) {                            /* :25-29 */
+        float qq = 1.0f - q;
+        return sig * (2.0f * qq * qq * qq);
+    }
+    return sig * (6.0f * (q * q * q - q * q) + 1.0f);   /* :31 */
+}
+
+static inline double W_d(double r2, double h2, double hinv, double sig)
+{
+    if (r2 >= h2) return 0.0;
+    double q = sqrt(r2) * hinv;
+    if (q >= 0.5) {
+        double qq = 1.0 - q;
+        return sig * (2.0 * qq * qq * qq);
+    }
+    return sig * (6.0 * (q * q * q - q * q) + 1.0);
+}
+
+float or_W_f(float r2, float h) { return W_f(r2, h * h, 1.0f / h, or_sigma_f(h)); }
+
+double or_W_d(double r2, double h)
+{
+    return W_d(r2, h * h, 1.0 / h, 8.0 / (OR_PI_D * h * h * h));
+}
+
+/* gradW with the reference's extra 1/|r| (Kernel.cpp:43): gradQ = normalize(r)/(|r| h). */
+static inline void gradW_f(const float r[3], float h, float h2, float hinv, float sig, float out[3])
+{
+    float rn = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+    if (rn >= h2) { out[0] = out[1] = out[2] = 0.0f; return; }
+    float rl = sqrtf(rn);
+    float q = rl * hinv;
+    float inv = 1.0f / sqrtf(rn);               /* glm::normalize = v * inversesqrt(dot) */
+    float den = rl * h;
+    float gq[3] = { (r[0] * inv) / den, (r[1] * inv) / den, (r[2] * inv) / den };
+    if (q >= 0.5f) {
+        float qq = 1.0f - q;
+        float f = 6.0f * qq * qq;
+        float ns = -sig;
+        for (int a = 0; a < 3; a++) out[a] = (ns * gq[a]) * f;
+        return;
+    }
+    float f = 6.0f * (3.0f * q * q - 2.0f * q);
+    for (int a = 0; a < 3; a++) out[a] = (sig * gq[a]) * f;
+}
+
+void or_gradW_f(const float r[3], float h, float out[3])
+{
+    gradW_f(r, h, h * h, 1.0f / h, or_sigma_f(h), out);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Cell keys.  CompactNSearch (vendor/CompactNSearch, absent here; call sites */
+/* Dataset.cpp:54-74, 277) keys a point by int(x / r), minus one for x < 0,   */
+/* with r the search radius = ParticleRadius = h.                             */
+/* ------------------------------------------------------------------------- */
+
+int32_t or_key(float x, float inv_h)
+{
+    float v = inv_h * x;
+    return x >= 0.0f ? (int32_t)v : (int32_t)v - 1;
+}
+
+uint64_t or_grid_struct_size(void) { return sizeof(or_grid); }
+
+static inline uint64_t cell_lin(const or_grid *g, int32_t x, int32_t y, int32_t z)
+{
+    return (uint64_t)x + (uint64_t)g->dims[0] * ((uint64_t)y + (uint64_t)g->dims[1] * (uint64_t)z);
+}
+
+/* Dense-cell coordinates of the stencil centre; 0 if the 27-cell stencil leaves
+ * the grid, in which case no particle can lie within h of q. */
+static inline int stencil_centre(const or_grid *g, float x, float y, float z, int32_t c[3])
+{
+    c[0] = or_key(x, g->inv_h) - g->origin[0];
+    c[1] = or_key(y, g->inv_h) - g->origin[1];
+    c[2] = or_key(z, g->inv_h) - g->origin[2];
+    for (int a = 0; a < 3; a++)
+        if (c[a] < 1 || c[a] > g->dims[a] - 2) return 0;
+    return 1;
+}
+
+static void densitygrid_build(or_grid *g);
+
+int or_grid_build(const float *xyz, uint64_t n, float h, or_grid *g)
+{
+    memset(g, 0, sizeof(*g));
+    if (n == 0 || !(h > 0.0f)) return -1;
+    g->h = h;
+    g->inv_h = 1.0f / h;
+    g->n = n;
+    /* Frame::ComputeAABB (Dataset.cpp:78-92): min/max, then pad by ParticleRadius. */
+    for (int a = 0; a < 3; a++) { g->pmin[a] = xyz[a]; g->pmax[a] = xyz[a]; }
+    for (uint64_t i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            float v = xyz[3 * i + a];
+            if (v < g->pmin[a]) g->pmin[a] = v;
+            if (v > g->pmax[a]) g->pmax[a] = v;
+        }
+    float pad = 1.0f * h;
+    for (int a = 0; a < 3; a++) { g->bmin[a] = g->pmin[a] - pad; g->bmax[a] = g->pmax[a] + pad; }
+
+    /* Keys are monotone in x, so the key range follows from the bounds.  Two
+     * empty cells of margin on each side keep every stencil of a point inside
+     * the padded AABB within the grid. */
+    uint64_t nc = 1;
+    for (int a = 0; a < 3; a++) {
+        int32_t kmin = or_key(g->pmin[a], g->inv_h), kmax = or_key(g->pmax[a], g->inv_h);
+        g->origin[a] = kmin - 2;
+        g->dims[a] = kmax - kmin + 5;
+        nc *= (uint64_t)g->dims[a];
+    }
+    if (nc > (1ull << 31)) return -2;
+    g->ncells = nc;
+    g->cell_start = (uint32_t *)calloc(nc + 1, sizeof(uint32_t));
+    g->sorted = (float *)malloc(4 * n * sizeof(float));
+    g->sorted_index = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint32_t *cell = (uint32_t *)malloc(n * sizeof(uint32_t));
+    if (!g->cell_start || !g->sorted || !g->sorted_index || !cell) { free(cell); or_grid_free(g); return -3; }
+
+    /* Stable counting sort: histogram, exclusive scan, scatter in input order. */
+    for (uint64_t i = 0; i < n; i++) {
+        int32_t c[3];
+        for (int a = 0; a < 3; a++) c[a] = or_key(xyz[3 * i + a], g->inv_h) - g->origin[a];
+        cell[i] = (uint32_t)cell_lin(g, c[0], c[1], c[2]);
+        g->cell_start[cell[i] + 1]++;
+    }
+    for (uint64_t c = 0; c < nc; c++) g->cell_start[c + 1] += g->cell_start[c];
+    uint32_t *cursor = (uint32_t *)malloc(nc * sizeof(uint32_t));
+    memcpy(cursor, g->cell_start, nc * sizeof(uint32_t));
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t slot = cursor[cell[i]]++;
+        g->sorted[4 * slot + 0] = xyz[3 * i + 0];
+        g->sorted[4 * slot + 1] = xyz[3 * i + 1];
+        g->sorted[4 * slot + 2] = xyz[3 * i + 2];
+        g->sorted[4 * slot + 3] = 0.0f;
+        g->sorted_index[slot] = (uint32_t)i;
+    }
+    free(cursor);
+    free(cell);
+    densitygrid_build(g);
+    return 0;
+}
+
+void or_grid_free(or_grid *g)
+{
+    free(g->cell_start); free(g->sorted); free(g->sorted_index);
+    free(g->dg_flag_box); free(g->dg_flag_cons); free(g->dg_count_box);
+    memset(g, 0, sizeof(*g));
+}
+
+/* ------------------------------------------------------------------------- */
+/* Neighbour search                                                           */
+/* Canonical order: the 27 cells around key(q) row by row (dz outer, dy, then */
+/* the three x-cells of a row), each cell in ascending input index.           */
+/* The distance test is |q - x_j|^2 < h^2 with ((dx dx + dy dy) + dz dz).      */
+/* ------------------------------------------------------------------------- */
+
+uint32_t or_neighbors_brute(const float *xyz, uint64_t n, float h, const float q[3],
+                            uint32_t *out, uint32_t cap)
+{
+    float h2 = h * h;
+    uint32_t cnt = 0;
+    for (uint64_t j = 0; j < n; j++) {
+        float dx = q[0] - xyz[3 * j], dy = q[1] - xyz[3 * j + 1], dz = q[2] - xyz[3 * j + 2];
+        float r2 = (dx * dx + dy * dy) + dz * dz;
+        if (r2 < h2) { if (cnt < cap) out[cnt] = (uint32_t)j; cnt++; }
+    }
+    return cnt;
+}
+
+uint32_t or_neighbors_grid(const or_grid *g, const float q[3], uint32_t *out, uint32_t cap,
+                           uint32_t *ncand)
+{
+    int32_t c[3];
+    uint32_t cnt = 0, cand = 0;
+    float h2 = g->h * g->h;
+    if (stencil_centre(g, q[0], q[1], q[2], c)) {
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++) {
+                uint64_t base = cell_lin(g, c[0] - 1, c[1] + dy, c[2] + dz);
+                uint32_t b = g->cell_start[base], e = g->cell_start[base + 3];
+                cand += e - b;
+                for (uint32_t s = b; s < e; s++) {
+                    const float *p = g->sorted + 4 * (uint64_t)s;
+                    float dx = q[0] - p[0], dyy = q[1] - p[1], dzz = q[2] - p[2];
+                    float r2 = (dx * dx + dyy * dyy) + dzz * dzz;
+                    if (r2 < h2) { if (cnt < cap) out[cnt] = g->sorted_index[s]; cnt++; }
+                }
+            }
+    }
+    if (ncand) *ncand = cand;
+    return cnt;
+}
+
+/* Hashed cell table: the CompactNSearch-style structure (cell key -> list of
+ * point indices, 27 lookups per query) used by the reference-style baseline. */
+typedef struct { int32_t k[3]; uint32_t start, count; } hash_slot;
+struct or_hash {
+    float h, inv_h;
+    const float *xyz;
+    uint64_t n;
+    uint32_t *idx;        /* point indices grouped by cell, ascending inside a cell */
+    hash_slot *table;     /* open addressing, count == 0 marks an empty slot        */
+    uint64_t mask;
+};
+
+static inline uint64_t hash3(int32_t x, int32_t y, int32_t z)
+{
+    /* the usual large-prime spatial hash */
+    return ((uint64_t)(uint32_t)x * 73856093ull) ^ ((uint64_t)(uint32_t)y * 19349663ull) ^
+           ((uint64_t)(uint32_t)z * 83492791ull);
+}
+
+static const hash_slot *hash_find(const or_hash *hs, int32_t x, int32_t y, int32_t z)
+{
+    uint64_t s = hash3(x, y, z) & hs->mask;
+    for (;;) {
+        const hash_slot *e = &hs->table[s];
+        if (e->count == 0) return NULL;
+        if (e->k[0] == x && e->k[1] == y && e->k[2] == z) return e;
+        s = (s + 1) & hs->mask;
+    }
+}
+
+typedef struct { int32_t k[3]; uint32_t i; } keyed;
+static int keyed_cmp(const void *a, const void *b)
+{
+    const keyed *x = (const keyed *)a, *y = (const keyed *)b;
+    for (int d = 2; d >= 0; d--) {
+        if (x->k[d] != y->k[d]) return x->k[d] < y->k[d] ? -1 : 1;
+    }
+    return x->i < y->i ? -1 : (x->i > y->i);
+}
+
+or_hash *or_hash_build(const float *xyz, uint64_t n, float h)
+{
+    or_hash *hs = (or_hash *)calloc(1, sizeof(or_hash));
+    hs->h = h; hs->inv_h = 1.0f / h; hs->xyz = xyz; hs->n = n;
+    keyed *kv = (keyed *)malloc(n * sizeof(keyed));
+    for (uint64_t i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) kv[i].k[a] = or_key(xyz[3 * i + a], hs->inv_h);
+        kv[i].i = (uint32_t)i;
+    }
+    qsort(kv, n, sizeof(keyed), keyed_cmp);
+    uint64_t cap = 16;
+    while (cap < 2 * n) cap <<= 1;
+    hs->mask = cap - 1;
+    hs->table = (hash_slot *)calloc(cap, sizeof(hash_slot));
+    hs->idx = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint64_t i = 0;
+    while (i < n) {
+        uint64_t j = i;
+        while (j < n && kv[j].k[0] == kv[i].k[0] && kv[j].k[1] == kv[i].k[1] && kv[j].k[2] == kv[i].k[2]) {
+            hs->idx[j] = kv[j].i;
+            j++;
+        }
+        uint64_t s = hash3(kv[i].k[0], kv[i].k[1], kv[i].k[2]) & hs->mask;
+        while (hs->table[s].count) s = (s + 1) & hs->mask;
+        memcpy(hs->table[s].k, kv[i].k, sizeof(kv[i].k));
+        hs->table[s].start = (uint32_t)i;
+        hs->table[s].count = (uint32_t)(j - i);
+        i = j;
+    }
+    free(kv);
+    return hs;
+}
+
+void or_hash_free(or_hash *hs)
+{
+    if (!hs) return;
+    free(hs->idx); free(hs->table); free(hs);
+}
+
+/* Dataset::GetNeighbors (Dataset.cpp:272-280) equivalent on the hashed table. */
+static uint32_t hash_neighbors(const or_hash *hs, const float q[3], uint32_t *out, uint32_t cap,
+                               uint32_t *ncand)
+{
+    int32_t k[3];
+    for (int a = 0; a < 3; a++) k[a] = or_key(q[a], hs->inv_h);
+    float h2 = hs->h * hs->h;
+    uint32_t cnt = 0, cand = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const hash_slot *e = hash_find(hs, k[0] + dx, k[1] + dy, k[2] + dz);
+                if (!e) continue;
+                cand += e->count;
+                for (uint32_t s = e->start; s < e->start + e->count; s++) {
+                    uint32_t j = hs->idx[s];
+                    const float *p = hs->xyz + 3 * (uint64_t)j;
+                    float ddx = q[0] - p[0], ddy = q[1] - p[1], ddz = q[2] - p[2];
+                    float r2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                    if (r2 < h2) { if (cnt < cap) out[cnt] = j; cnt++; }
+                }
+            }
+    if (ncand) *ncand = cand;
+    return cnt;
+}
+
+uint32_t or_neighbors_hash(const float *xyz, uint64_t n, float h, const float q[3],
+                           uint32_t *out, uint32_t cap)
+{
+    or_hash *hs = or_hash_build(xyz, n, h);
+    uint32_t c = hash_neighbors(hs, q, out, cap, NULL);
+    or_hash_free(hs);
+    return c;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Density: rho(x) = sum_j W(x_j - x) (RayMarcher.cpp:315-325), in canonical */
+/* order.  Compensated form: Neumaier's variant of Kahan summation.           */
+/* ------------------------------------------------------------------------- */
+
+typedef struct { float s, c; } neu_f;
+static inline void neu_add(neu_f *a, float w)
+{
+    float t = a->s + w;
+    if (fabsf(a->s) >= fabsf(w)) a->c += (a->s - t) + w;
+    else a->c += (w - t) + a->s;
+    a->s = t;
+}
+
+float or_density_f(const or_grid *g, const float q[3], int compensated, uint32_t *ncand)
+{
+    int32_t c[3];
+    float h2 = g->h * g->h, hinv = 1.0f / g->h, sig = or_sigma_f(g->h);
+    neu_f acc = { 0.0f, 0.0f };
+    float plain = 0.0f;
+    uint32_t cand = 0;
+    if (stencil_centre(g, q[0], q[1], q[2], c)) {
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++) {
+                uint64_t base = cell_lin(g, c[0] - 1, c[1] + dy, c[2] + dz);
+                uint32_t b = g->cell_start[base], e = g->cell_start[base + 3];
+                cand += e - b;
+                for (uint32_t s = b; s < e; s++) {
+                    const float *p = g->sorted + 4 * (uint64_t)s;
+                    float rx = p[0] - q[0], ry = p[1] - q[1], rz = p[2] - q[2];
+                    float r2 = (rx * rx + ry * ry) + rz * rz;
+                    if (r2 < h2) {
+                        float w = W_f(r2, h2, hinv, sig);
+                        if (compensated) neu_add(&acc, w);
+                        else plain += w;
+                    }
+                }
+            }
+    }
+    if (ncand) *ncand = cand;
+    return compensated ? acc.s + acc.c : plain;
+}
+
+double or_density_d(const or_grid *g, const double q[3])
+{
+    double h = (double)g->h;
+    double h2 = h * h, hinv = 1.0 / h, sig = 8.0 / (OR_PI_D * h * h * h);
+    int32_t c[3];
+    double acc = 0.0;
+    /* The stencil is chosen from the FP32-rounded point; the FP32 and FP64
+     * stencils can only differ for points within an ulp of a cell face, where
+     * the difference is a cell at distance >= h and contributes nothing. */
+    if (stencil_centre(g, (float)q[0], (float)q[1], (float)q[2], c)) {
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++) {
+                uint64_t base = cell_lin(g, c[0] - 1, c[1] + dy, c[2] + dz);
+                uint32_t b = g->cell_start[base], e = g->cell_start[base + 3];
+                for (uint32_t s = b; s < e; s++) {
+                    const float *p = g->sorted + 4 * (uint64_t)s;
+                    double rx = (double)p[0] - q[0], ry = (double)p[1] - q[1], rz = (double)p[2] - q[2];
+                    double r2 = (rx * rx + ry * ry) + rz * rz;
+                    if (r2 < h2) acc += W_d(r2, h2, hinv, sig);
+                }
+            }
+    }
+    return acc;
+}
+
+void or_density_batch_f(const or_grid *g, const float *q, uint64_t nq, int compensated,
+                        float *rho, uint32_t *ncand)
+{
+    for (uint64_t i = 0; i < nq; i++)
+        rho[i] = or_density_f(g, q + 3 * i, compensated, ncand ? ncand + i : NULL);
+}
+
+void or_density_batch_d(const or_grid *g, const double *q, uint64_t nq, double *rho)
+{
+    for (uint64_t i = 0; i < nq; i++) rho[i] = or_density_d(g, q + 3 * i);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Reference DensityGrid (Dataset.cpp:94-164, query :26-47).  Cell width h    */
+/* over the padded AABB.  find_neighbors_box is a fork-only CompactNSearch    */
+/* call whose source is absent; the box flag reads it as "some particle with  */
+/* |x_j - centre|_inf < h", i.e. count * W(0) > 1.  The conservative flag     */
+/* widens the box to 1.5h (+ margin) so that a cleared cell provably has      */
+/* zero density everywhere inside it.                                         */
+/* ------------------------------------------------------------------------- */
+
+static uint32_t count_linf(const or_grid *g, const float c[3], float R)
+{
+    int32_t lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = or_key(c[a] - R, g->inv_h) - g->origin[a];
+        hi[a] = or_key(c[a] + R, g->inv_h) - g->origin[a];
+        if (lo[a] < 0) lo[a] = 0;
+        if (hi[a] > g->dims[a] - 1) hi[a] = g->dims[a] - 1;
+        if (lo[a] > hi[a]) return 0;
+    }
+    uint32_t cnt = 0;
+    for (int32_t z = lo[2]; z <= hi[2]; z++)
+        for (int32_t y = lo[1]; y <= hi[1]; y++) {
+            uint32_t b = g->cell_start[cell_lin(g, lo[0], y, z)], e = g->cell_start[cell_lin(g, hi[0], y, z) + 1];
+            for (uint32_t s = b; s < e; s++) {
+                const float *p = g->sorted + 4 * (uint64_t)s;
+                if (fabsf(p[0] - c[0]) < R && fabsf(p[1] - c[1]) < R && fabsf(p[2] - c[2]) < R) cnt++;
+            }
+        }
+    return cnt;
+}
+
+static void densitygrid_build(or_grid *g)
+{
+    float cw = 1.0f * g->h;
+    uint64_t tot = 1;
+    for (int a = 0; a < 3; a++) {
+        float ext = g->bmax[a] - g->bmin[a];
+        g->dg_dims[a] = (int32_t)ceilf(ext / cw);
+        tot *= (uint64_t)g->dg_dims[a];
+    }
+    g->dg_flag_box = (uint8_t *)calloc(tot, 1);
+    g->dg_flag_cons = (uint8_t *)calloc(tot, 1);
+    g->dg_count_box = (uint32_t *)calloc(tot, sizeof(uint32_t));
+    float sig = or_sigma_f(g->h);
+    for (int32_t z = 0; z < g->dg_dims[2]; z++)
+        for (int32_t y = 0; y < g->dg_dims[1]; y++)
+            for (int32_t x = 0; x < g->dg_dims[0]; x++) {
+                float c[3] = { g->bmin[0] + ((float)x + 0.5f) * cw, g->bmin[1] + ((float)y + 0.5f) * cw,
+                               g->bmin[2] + ((float)z + 0.5f) * cw };
+                uint64_t id = (uint64_t)x + (uint64_t)g->dg_dims[0] * ((uint64_t)y + (uint64_t)g->dg_dims[1] * (uint64_t)z);
+                uint32_t nb = count_linf(g, c, g->h);
+                g->dg_count_box[id] = nb;
+                g->dg_flag_box[id] = ((float)nb * sig > 1.0f);
+                g->dg_flag_cons[id] = count_linf(g, c, 1.5f * g->h * 1.001f) > 0;
+            }
+}
+
+/* QueryDensityGrid (Dataset.cpp:26-47): returns the node id or -1. */
+static int64_t dg_query(const or_grid *g, const float p[3], float nmin[3], float nmax[3])
+{
+    float inv = 1.0f / g->h;
+    int32_t k[3];
+    for (int a = 0; a < 3; a++) {
+        k[a] = (int32_t)floorf((p[a] - g->bmin[a]) * inv);
+        if (!(k[a] >= 0 && k[a] < g->dg_dims[a])) return -1;
+    }
+    for (int a = 0; a < 3; a++) {
+        nmin[a] = g->bmin[a] + (float)k[a] * g->h;      /* :132 */
+        nmax[a] = nmin[a] + g->h;                       /* :133 */
+    }
+    return (int64_t)k[0] + (int64_t)g->dg_dims[0] * ((int64_t)k[1] + (int64_t)g->dg_dims[1] * (int64_t)k[2]);
+}
+
+static inline float gmin(float x, float y) { return (y < x) ? y : x; }
+static inline float gmax(float x, float y) { return (x < y) ? y : x; }
+
+/* intersectAABB (RayMarcher.cpp:51-62): t is in units of dir (= the step). */
+static void intersect_aabb(const float o[3], const float d[3], const float bmin[3],
+                           const float bmax[3], float out[3])
+{
+    float t1[3], t2[3];
+    for (int a = 0; a < 3; a++) {
+        float tmn = (bmin[a] - o[a]) / d[a], tmx = (bmax[a] - o[a]) / d[a];
+        t1[a] = gmin(tmn, tmx);
+        t2[a] = gmax(tmn, tmx);
+    }
+    float tnear = gmax(gmax(t1[0], t1[1]), t1[2]);
+    float tfar = gmin(gmin(t2[0], t2[1]), t2[2]);
+    float t = (tnear < tfar) ? tfar : tnear;
+    if (t >= 0.0f) for (int a = 0; a < 3; a++) out[a] = o[a] + d[a] * t;
+    else for (int a = 0; a < 3; a++) out[a] = o[a];
+}
+
+/* ------------------------------------------------------------------------- */
+/* Ray generation (RayMarcher.cpp:268-274) and the AABB slab test.           */
+/* mat4 * vec4 as glm evaluates it: (m0 x + m1 y) + (m2 z + m3 w).           */
+/* ------------------------------------------------------------------------- */
+
+static void unproject(const float M[16], float cx, float cy, float cz, float p[3])
+{
+    float w[4];
+    for (int r = 0; r < 4; r++) {
+        float a0 = M[0 * 4 + r] * cx, a1 = M[1 * 4 + r] * cy;
+        float a2 = M[2 * 4 + r] * cz, a3 = M[3 * 4 + r] * 1.0f;
+        w[r] = (a0 + a1) + (a2 + a3);
+    }
+    p[0] = w[0] / w[3]; p[1] = w[1] / w[3]; p[2] = w[2] / w[3];
+}
+
+static void pixel_clip(uint32_t idx, uint32_t W, uint32_t H, float *cx, float *cy)
+{
+    float tw = 2.0f / (float)W, th = 2.0f / (float)H;   /* RayMarcher.cpp:88-89 */
+    *cx = (float)(idx % W) * tw - 1.0f;
+    *cy = (float)(idx / W) * th - 1.0f;
+}
+
+static void normalize3(const float v[3], float out[3])
+{
+    float dd = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    float inv = 1.0f / sqrtf(dd);
+    out[0] = v[0] * inv; out[1] = v[1] * inv; out[2] = v[2] * inv;
+}
+
+/* Slab test against [bmin,bmax] for t >= 0.  Returns 0 on a miss. */
+static int slab(const float o[3], const float d[3], const float bmin[3], const float bmax[3],
+                float *tn, float *tf)
+{
+    float tnear = 0.0f, tfar = INFINITY;
+    for (int a = 0; a < 3; a++) {
+        if (d[a] != 0.0f) {
+            float inv = 1.0f / d[a];
+            float t0 = (bmin[a] - o[a]) * inv, t1 = (bmax[a] - o[a]) * inv;
+            float lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+            if (lo > tnear) tnear = lo;
+            if (hi < tfar) tfar = hi;
+        } else if (o[a] < bmin[a] || o[a] > bmax[a]) {
+            return 0;
+        }
+    }
+    *tn = tnear; *tf = tfar;
+    return tnear <= tfar;
+}
+
+void or_ray(const float invpv[16], const float cam[3], uint32_t W, uint32_t H, uint32_t idx,
+            float o[3], float d[3])
+{
+    float cx, cy, wp[3], dun[3];
+    pixel_clip(idx, W, H, &cx, &cy);
+    unproject(invpv, cx, cy, 1.0f, wp);
+    for (int a = 0; a < 3; a++) { o[a] = cam[a]; dun[a] = wp[a] - cam[a]; }
+    normalize3(dun, d);
+}
+
+int or_slab(const float o[3], const float d[3], const float bmin[3], const float bmax[3], float tnf[2])
+{
+    return slab(o, d, bmin, bmax, &tnf[0], &tnf[1]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Density providers: dense grid or hashed table, same canonical order.       */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    const or_grid *g;
+    const or_hash *hs;   /* non-NULL: reference-style hashed query */
+    uint32_t *nbuf;      /* thread-local neighbour buffer (hashed)  */
+    float h2, hinv, sig;
+    int compensated;
+} dens_ctx;
+
+#define OR_MAX_NEIGHBORS 8192   /* RayMarcher.cpp:14 */
+
+static float dens_f(dens_ctx *dc, const float q[3], or_counters *cnt)
+{
+    uint32_t cand = 0;
+    float rho;
+    if (dc->hs) {
+        /* RayMarcher.cpp:309-325: neighbour list, then sum of W over it. */
+        uint32_t nn = hash_neighbors(dc->hs, q, dc->nbuf, OR_MAX_NEIGHBORS, &cand);
+        if (nn > OR_MAX_NEIGHBORS) nn = OR_MAX_NEIGHBORS;
+        neu_f acc = { 0.0f, 0.0f };
+        float plain = 0.0f;
+        for (uint32_t i = 0; i < nn; i++) {
+            const float *p = dc->hs->xyz + 3 * (uint64_t)dc->nbuf[i];
+            float rx = p[0] - q[0], ry = p[1] - q[1], rz = p[2] - q[2];
+            float r2 = (rx * rx + ry * ry) + rz * rz;
+            float w = W_f(r2, dc->h2, dc->hinv, dc->sig);
+            if (dc->compensated) neu_add(&acc, w); else plain += w;
+        }
+        rho = dc->compensated ? acc.s + acc.c : plain;
+    } else {
+        rho = or_density_f(dc->g, q, dc->compensated, &cand);
+    }
+    cnt->samples++;
+    cnt->cand += cand;
+    return rho;
+}
+
+static void sum_gradW(dens_ctx *dc, const float q[3], float out[3], or_counters *cnt)
+{
+    /* RayMarcher.cpp:331-338: normal += gradW(x_j - x) over the neighbours. */
+    const or_grid *g = dc->g;
+    int32_t c[3];
+    out[0] = out[1] = out[2] = 0.0f;
+    uint32_t cand = 0;
+    if (stencil_centre(g, q[0], q[1], q[2], c)) {
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++) {
+                uint64_t base = cell_lin(g, c[0] - 1, c[1] + dy, c[2] + dz);
+                uint32_t b = g->cell_start[base], e = g->cell_start[base + 3];
+                cand += e - b;
+                for (uint32_t s = b; s < e; s++) {
+                    const float *p = g->sorted + 4 * (uint64_t)s;
+                    float r[3] = { p[0] - q[0], p[1] - q[1], p[2] - q[2] };
+                    float r2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+                    if (r2 < dc->h2) {
+                        float gw[3];
+                        gradW_f(r, g->h, dc->h2, dc->hinv, dc->sig, gw);
+                        out[0] += gw[0]; out[1] += gw[1]; out[2] += gw[2];
+                    }
+                }
+            }
+    }
+    (void)cnt; (void)cand;
+}
+
+/* ------------------------------------------------------------------------- */
+/* One pixel of the configured march: AABB entry, fixed step, first sample    */
+/* with rho >= iso, bisection (last iteration in FP64), normals.              */
+/* ------------------------------------------------------------------------- */
+
+static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], const float cam[3],
+                         uint32_t W, uint32_t H, uint32_t idx, float *pos, float *nrm, float *tout,
+                         or_counters *cnt)
+{
+    const or_grid *g = dc->g;
+    float *P = pos + 4 * (uint64_t)idx, *N = nrm + 4 * (uint64_t)idx;
+    P[0] = P[1] = P[2] = P[3] = 0.0f;
+    N[0] = N[1] = N[2] = N[3] = 0.0f;
+    if (tout) tout[idx] = -1.0f;
+    cnt->rays++;
+
+    float cx, cy, wp[3], dun[3], d[3], o[3] = { cam[0], cam[1], cam[2] };
+    pixel_clip(idx, W, H, &cx, &cy);
+    unproject(M, cx, cy, 1.0f, wp);
+    for (int a = 0; a < 3; a++) dun[a] = wp[a] - cam[a];
+    normalize3(dun, d);
+
+    float tn, tf;
+    if (!slab(o, d, g->bmin, g->bmax, &tn, &tf)) return;
+    cnt->rays_in_aabb++;
+
+    float step = s->step;
+    int64_t k = 0, hitk = -1;
+    for (;; k++) {
+        if (s->max_steps > 0 && k >= s->max_steps) break;
+        float t = tn + (float)k * step;
+        if (t > tf) break;
+        float x[3] = { o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2] };
+        float rho = dens_f(dc, x, cnt);
+        cnt->march_samples++;
+        if (rho >= s->iso) { hitk = k; break; }
+    }
+    if (hitk < 0) return;
+    cnt->hits++;
+
+    float xs[3], th;
+    if (hitk == 0) {
+        th = tn;
+        for (int a = 0; a < 3; a++) xs[a] = o[a] + th * d[a];
+    } else {
+        float lo = tn + (float)(hitk - 1) * step, hi = tn + (float)hitk * step;
+        int n32 = s->final_fp64 && s->bisect_iters > 0 ? s->bisect_iters - 1 : s->bisect_iters;
+        for (int it = 0; it < n32; it++) {
+            float tm = (lo + hi) * 0.5f;
+            float x[3] = { o[0] + tm * d[0], o[1] + tm * d[1], o[2] + tm * d[2] };
+            float rho = dens_f(dc, x, cnt);
+            if (rho >= s->iso) hi = tm; else lo = tm;
+        }
+        if (s->final_fp64 && s->bisect_iters > 0) {
+            double dlo = lo, dhi = hi;
+            double tm = (dlo + dhi) * 0.5;
+            double x[3] = { (double)o[0] + tm * (double)d[0], (double)o[1] + tm * (double)d[1],
+                            (double)o[2] + tm * (double)d[2] };
+            double rho = or_density_d(g, x);
+            cnt->samples++;
+            if (rho >= (double)s->iso) dhi = tm; else dlo = tm;
+            double tt = (dlo + dhi) * 0.5;
+            for (int a = 0; a < 3; a++) xs[a] = (float)((double)o[a] + tt * (double)d[a]);
+            th = (float)tt;
+        } else {
+            th = (lo + hi) * 0.5f;
+            for (int a = 0; a < 3; a++) xs[a] = o[a] + th * d[a];
+        }
+    }
+
+    float n[3] = { 0.0f, 0.0f, 0.0f };
+    if (s->normal_mode == 0) {
+        float gv[3];
+        for (int a = 0; a < 3; a++) {
+            float xp[3] = { xs[0], xs[1], xs[2] }, xm[3] = { xs[0], xs[1], xs[2] };
+            xp[a] = xs[a] + s->normal_eps;
+            xm[a] = xs[a] - s->normal_eps;
+            float rp = dens_f(dc, xp, cnt);
+            float rm = dens_f(dc, xm, cnt);
+            gv[a] = rp - rm;
+        }
+        float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
+        if (dd > 0.0f) {
+            float inv = 1.0f / sqrtf(dd);
+            for (int a = 0; a < 3; a++) n[a] = -gv[a] * inv;
+        }
+    } else {
+        float gs[3];
+        sum_gradW(dc, xs, gs, cnt);
+        normalize3(gs, n);
+    }
+    P[0] = xs[0]; P[1] = xs[1]; P[2] = xs[2]; P[3] = 1.0f;
+    N[0] = n[0]; N[1] = n[1]; N[2] = n[2]; N[3] = 1.0f;
+    if (tout) tout[idx] = th;
+}
+
+/* One pixel of the reference march, PerPixel_Isotropic (RayMarcher.cpp:256-344). */
+static void pixel_reference(dens_ctx *dc, const or_settings *s, const float M[16], const float cam[3],
+                            uint32_t W, uint32_t H, const float *depth, uint32_t idx, float *pos,
+                            float *nrm, float *tout, or_counters *cnt)
+{
+    const or_grid *g = dc->g;
+    float *P = pos + 4 * (uint64_t)idx, *N = nrm + 4 * (uint64_t)idx;
+    float z = depth[idx];
+    P[0] = P[1] = P[2] = P[3] = 0.0f;
+    N[0] = N[1] = N[2] = N[3] = 0.0f;
+    if (tout) tout[idx] = -1.0f;
+    cnt->rays++;
+    if (z == 1.0f) return;                                  /* :264 */
+    cnt->rays_in_aabb++;
+
+    float cx, cy, p[3], dun[3], dn[3], st[3];
+    pixel_clip(idx, W, H, &cx, &cy);
+    unproject(M, cx, cy, z, p);                             /* :268-273 */
+    for (int a = 0; a < 3; a++) dun[a] = p[a] - cam[a];
+    normalize3(dun, dn);
+    for (int a = 0; a < 3; a++) st[a] = s->step * dn[a];    /* :274 */
+
+    for (int i = 0; i < s->max_steps; i++) {
+        for (int a = 0; a < 3; a++) p[a] += st[a];          /* :279 */
+        if (s->skip_mode == 2 || s->skip_mode == 3) {       /* :284-304 */
+            const uint8_t *flag = s->skip_mode == 2 ? g->dg_flag_box : g->dg_flag_cons;
+            float nmin[3], nmax[3];
+            int64_t id;
+            while ((id = dg_query(g, p, nmin, nmax)) >= 0 && !flag[id]) {
+                float q[3];
+                intersect_aabb(p, st, nmin, nmax, q);
+                for (int a = 0; a < 3; a++) p[a] = q[a] + st[a];
+            }
+        }
+        float rho = dens_f(dc, p, cnt);
+        cnt->march_samples++;
+        if (rho >= s->iso) {                                /* :327 */
+            float n[3] = { 0.0f, 0.0f, 0.0f };
+            if (s->normal_mode == 1) {
+                float gs[3];
+                sum_gradW(dc, p, gs, cnt);
+                normalize3(gs, n);                          /* :338 */
+            } else {
+                float gv[3];
+                for (int a = 0; a < 3; a++) {
+                    float xp[3] = { p[0], p[1], p[2] }, xm[3] = { p[0], p[1], p[2] };
+                    xp[a] = p[a] + s->normal_eps;
+                    xm[a] = p[a] - s->normal_eps;
+                    gv[a] = dens_f(dc, xp, cnt) - dens_f(dc, xm, cnt);
+                }
+                float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
+                if (dd > 0.0f) {
+                    float inv = 1.0f / sqrtf(dd);
+                    for (int a = 0; a < 3; a++) n[a] = -gv[a] * inv;
+                }
+            }
+            cnt->hits++;
+            P[0] = p[0]; P[1] = p[1]; P[2] = p[2]; P[3] = 1.0f;
+            N[0] = n[0]; N[1] = n[1]; N[2] = n[2]; N[3] = 1.0f;
+            if (tout) {
+                float dv[3] = { p[0] - cam[0], p[1] - cam[1], p[2] - cam[2] };
+                tout[idx] = sqrtf((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2]);
+            }
+            return;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Work scheduler: the reference's pool hands out one pixel per atomic        */
+/* fetch_add (ThreadPool.cpp:38-55).  Same pattern, with every pixel covered  */
+/* and completion defined by joining the workers.                             */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    const or_grid *g;
+    const or_hash *hs;
+    const or_settings *s;
+    const float *M, *cam, *depth;
+    uint32_t W, H, row_step, nrows;
+    float *pos, *nrm, *tout;
+    _Atomic uint64_t next;
+    uint64_t total;
+    pthread_mutex_t lock;
+    or_counters cnt;
+} march_job;
+
+static void *march_worker(void *arg)
+{
+    march_job *j = (march_job *)arg;
+    dens_ctx dc;
+    memset(&dc, 0, sizeof(dc));
+    dc.g = j->g;
+    dc.hs = j->hs;
+    dc.h2 = j->g->h * j->g->h;
+    dc.hinv = 1.0f / j->g->h;
+    dc.sig = or_sigma_f(j->g->h);
+    dc.compensated = j->s->compensated;
+    dc.nbuf = j->hs ? (uint32_t *)malloc(OR_MAX_NEIGHBORS * sizeof(uint32_t)) : NULL;
+    or_counters c;
+    memset(&c, 0, sizeof(c));
+    uint64_t i;
+    while ((i = atomic_fetch_add(&j->next, 1)) < j->total) {
+        uint32_t row = (uint32_t)(i / j->W) * j->row_step, col = (uint32_t)(i % j->W);
+        uint32_t idx = row * j->W + col;
+        if (j->s->mode == OR_MODE_REFERENCE)
+            pixel_reference(&dc, j->s, j->M, j->cam, j->W, j->H, j->depth, idx, j->pos, j->nrm, j->tout, &c);
+        else
+            pixel_config(&dc, j->s, j->M, j->cam, j->W, j->H, idx, j->pos, j->nrm, j->tout, &c);
+    }
+    free(dc.nbuf);
+    pthread_mutex_lock(&j->lock);
+    j->cnt.rays += c.rays; j->cnt.rays_in_aabb += c.rays_in_aabb; j->cnt.hits += c.hits;
+    j->cnt.samples += c.samples; j->cnt.cand += c.cand; j->cnt.march_samples += c.march_samples;
+    pthread_mutex_unlock(&j->lock);
+    return NULL;
+}
+
+static int run_march(const or_grid *g, const or_hash *hs, const or_settings *s, const float M[16],
+                     const float cam[3], uint32_t W, uint32_t H, const float *depth, uint32_t row_step,
+                     int nthreads, float *pos, float *nrm, float *tout, or_counters *cnt)
+{
+    if (!g || !s || !M || !cam || !pos || !nrm || W == 0 || H == 0) return -1;
+    if (s->mode == OR_MODE_REFERENCE && !depth) return -1;
+    if (!(s->step > 0.0f)) return -1;
+    if (row_step == 0) row_step = 1;
+    march_job j;
+    memset(&j, 0, sizeof(j));
+    j.g = g; j.hs = hs; j.s = s; j.M = M; j.cam = cam; j.depth = depth;
+    j.W = W; j.H = H; j.row_step = row_step;
+    j.nrows = (H + row_step - 1) / row_step;
+    j.pos = pos; j.nrm = nrm; j.tout = tout;
+    j.total = (uint64_t)W * j.nrows;
+    atomic_init(&j.next, 0);
+    pthread_mutex_init(&j.lock, NULL);
+    if (nthreads <= 1) {
+        march_worker(&j);
+    } else {
+        pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+        for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, march_worker, &j);
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+        free(th);
+    }
+    pthread_mutex_destroy(&j.lock);
+    if (cnt) *cnt = j.cnt;
+    return 0;
+}
+
+int or_march(const or_grid *g, const or_settings *s, const float invpv[16], const float cam[3],
+             uint32_t W, uint32_t H, const float *depth, uint32_t row_step, int nthreads,
+             float *pos, float *nrm, float *tout, or_counters *cnt)
+{
+    return run_march(g, NULL, s, invpv, cam, W, H, depth, row_step, nthreads, pos, nrm, tout, cnt);
+}
+
+int or_march_hashed(const or_hash *hs, const or_grid *g, const or_settings *s,
+                    const float invpv[16], const float cam[3], uint32_t W, uint32_t H,
+                    uint32_t row_step, int nthreads, float *pos, float *nrm, float *tout,
+                    or_counters *cnt)
+{
+    if (!hs || s->mode != OR_MODE_CONFIG || s->normal_mode != 0) return -1;
+    return run_march(g, hs, s, invpv, cam, W, H, NULL, row_step, nthreads, pos, nrm, tout, cnt);
+}
+
+/* ------------------------------------------------------------------------- */
+/* FP64 configured march: the ground truth.  Same ray and sample definition   */
+/* as the FP32 march, every operation in double, all bisection iterations in */
+/* double, central-difference normals in double.                              */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    const or_grid *g;
+    const or_settings *s;
+    const float *M, *cam;
+    uint32_t W, H, row_step;
+    double *pos, *nrm, *tout;
+    _Atomic uint64_t next;
+    uint64_t total;
+} march64_job;
+
+static void pixel_f64(const march64_job *j, uint32_t idx)
+{
+    const or_grid *g = j->g;
+    const or_settings *s = j->s;
+    double *P = j->pos + 4 * (uint64_t)idx, *N = j->nrm + 4 * (uint64_t)idx;
+    for (int a = 0; a < 4; a++) P[a] = N[a] = 0.0;
+    if (j->tout) j->tout[idx] = -1.0;
+    double cx = (double)(idx % j->W) * (2.0 / (double)j->W) - 1.0;
+    double cy = (double)(idx / j->W) * (2.0 / (double)j->H) - 1.0;
+    double w[4];
+    for (int r = 0; r < 4; r++)
+        w[r] = ((double)j->M[r] * cx + (double)j->M[4 + r] * cy) +
+               ((double)j->M[8 + r] * 1.0 + (double)j->M[12 + r]);
+    double o[3] = { j->cam[0], j->cam[1], j->cam[2] }, d[3];
+    double dd = 0.0;
+    for (int a = 0; a < 3; a++) { d[a] = w[a] / w[3] - o[a]; dd += d[a] * d[a]; }
+    double inv = 1.0 / sqrt(dd);
+    for (int a = 0; a < 3; a++) d[a] *= inv;
+    double tn = 0.0, tf = INFINITY;
+    for (int a = 0; a < 3; a++) {
+        if (d[a] != 0.0) {
+            double t0 = ((double)g->bmin[a] - o[a]) / d[a], t1 = ((double)g->bmax[a] - o[a]) / d[a];
+            double lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+            if (lo > tn) tn = lo;
+            if (hi < tf) tf = hi;
+        } else if (o[a] < g->bmin[a] || o[a] > g->bmax[a]) return;
+    }
+    if (tn > tf) return;
+    double step = s->step, iso = s->iso;
+    int64_t k, hitk = -1;
+    for (k = 0;; k++) {
+        if (s->max_steps > 0 && k >= s->max_steps) break;
+        double t = tn + (double)k * step;
+        if (t > tf) break;
+        double x[3] = { o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2] };
+        if (or_density_d(g, x) >= iso) { hitk = k; break; }
+    }
+    if (hitk < 0) return;
+    double th;
+    if (hitk == 0) th = tn;
+    else {
+        double lo = tn + (double)(hitk - 1) * step, hi = tn + (double)hitk * step;
+        for (int it = 0; it < s->bisect_iters; it++) {
+            double tm = (lo + hi) * 0.5;
+            double x[3] = { o[0] + tm * d[0], o[1] + tm * d[1], o[2] + tm * d[2] };
+            if (or_density_d(g, x) >= iso) hi = tm; else lo = tm;
+        }
+        th = (lo + hi) * 0.5;
+    }
+    double xs[3] = { o[0] + th * d[0], o[1] + th * d[1], o[2] + th * d[2] };
+    double gv[3];
+    for (int a = 0; a < 3; a++) {
+        double xp[3] = { xs[0], xs[1], xs[2] }, xm[3] = { xs[0], xs[1], xs[2] };
+        xp[a] += s->normal_eps;
+        xm[a] -= s->normal_eps;
+        gv[a] = or_density_d(g, xp) - or_density_d(g, xm);
+    }
+    double gg = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
+    double n[3] = { 0, 0, 0 };
+    if (gg > 0) for (int a = 0; a < 3; a++) n[a] = -gv[a] / sqrt(gg);
+    for (int a = 0; a < 3; a++) { P[a] = xs[a]; N[a] = n[a]; }
+    P[3] = N[3] = 1.0;
+    if (j->tout) j->tout[idx] = th;
+}
+
+static void *march64_worker(void *arg)
+{
+    march64_job *j = (march64_job *)arg;
+    uint64_t i;
+    while ((i = atomic_fetch_add(&j->next, 1)) < j->total) {
+        uint32_t row = (uint32_t)(i / j->W) * j->row_step, col = (uint32_t)(i % j->W);
+        pixel_f64(j, row * j->W + col);
+    }
+    return NULL;
+}
+
+int or_march_f64(const or_grid *g, const or_settings *s, const float invpv[16], const float cam[3],
+                 uint32_t W, uint32_t H, uint32_t row_step, int nthreads,
+                 double *pos, double *nrm, double *tout)
+{
+    if (!g || !s || s->mode != OR_MODE_CONFIG || !(s->step > 0.0f) || W == 0 || H == 0) return -1;
+    if (row_step == 0) row_step = 1;
+    march64_job j;
+    memset(&j, 0, sizeof(j));
+    j.g = g; j.s = s; j.M = invpv; j.cam = cam; j.W = W; j.H = H; j.row_step = row_step;
+    j.pos = pos; j.nrm = nrm; j.tout = tout;
+    j.total = (uint64_t)W * ((H + row_step - 1) / row_step);
+    atomic_init(&j.next, 0);
+    if (nthreads <= 1) march64_worker(&j);
+    else {
+        pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+        for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, march64_worker, &j);
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+        free(th);
+    }
+    return 0;
+}

@@ -1,0 +1,126 @@
+/*
+ * pm_oracle.h -- CPU oracle for the particle-fluid iso-surface ray march.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * never by the product path (the HIP library under bachelor-thesis_amd/).
+ *
+ * It restates, in plain C compiled with -ffp-contract=off, the reference's CPU
+ * marcher (src/app/AdvancedRenderer/RayMarcher.cpp:51-62, 256-344), its SPH
+ * kernel (src/app/Kernel.cpp:8-52), its frame structures (src/app/Dataset.cpp:
+ * 26-47, 78-164, 272-280) and its work scheduler (src/app/ThreadPool.cpp:5-55),
+ * plus the configured march (AABB entry, fixed step, bisection, central-
+ * difference normals) that the GPU path implements.  Every float operation is
+ * written out in the order the GPU kernels use, so FP32 results can be compared
+ * bit for bit; the FP64 march is the ground truth errors are stated against.
+ */
+#ifndef PM_ORACLE_H
+#define PM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field order and meaning as pm_settings in include/pmarch.h. */
+typedef struct or_settings {
+    int32_t mode;          /* 0 = configured march, 1 = reference march          */
+    int32_t max_steps;     /* reference mode: samples that count (MaxSteps)       */
+    float   step;          /* march step in world units                           */
+    float   iso;           /* iso density, absolute kernel units                  */
+    int32_t bisect_iters;  /* configured mode: bisection iterations              */
+    int32_t final_fp64;    /* configured mode: last bisection iteration in FP64   */
+    float   normal_eps;    /* configured mode: central-difference offset          */
+    int32_t normal_mode;   /* 0 = central difference of rho, 1 = sum of gradW     */
+    int32_t skip_mode;     /* 0 none; 1 empty-stencil skip (configured mode);
+                              2 DensityGrid, box flag; 3 DensityGrid, conservative
+                              flag (reference mode)                               */
+    int32_t compensated;   /* 1 = Neumaier-compensated FP32 density sums          */
+} or_settings;
+
+enum { OR_MODE_CONFIG = 0, OR_MODE_REFERENCE = 1 };
+
+typedef struct or_counters {
+    uint64_t rays;          /* primary rays processed                      */
+    uint64_t rays_in_aabb;  /* rays that intersect the padded AABB         */
+    uint64_t hits;          /* rays that found an iso crossing             */
+    uint64_t samples;       /* density evaluations of every kind           */
+    uint64_t cand;          /* sum over samples of stencil candidates      */
+    uint64_t march_samples; /* density evaluations of the march phase      */
+} or_counters;
+
+/* Dense uniform grid, cell width h, cell keys as CompactNSearch computes them
+ * (int(x/h), minus one for negative x), particles stored cell-major (x fastest)
+ * and, inside a cell, by ascending input index. */
+typedef struct or_grid {
+    float    h, inv_h;
+    float    pmin[3], pmax[3];      /* particle bounds                       */
+    float    bmin[3], bmax[3];      /* bounds padded by h (Frame::ComputeAABB) */
+    int32_t  origin[3];             /* key of dense cell (0,0,0)             */
+    int32_t  dims[3];
+    uint64_t ncells;
+    uint64_t n;
+    uint32_t *cell_start;           /* ncells + 1, exclusive scan of counts  */
+    float    *sorted;               /* 4*n: x, y, z, unused                  */
+    uint32_t *sorted_index;         /* input index of each stored particle   */
+    /* reference DensityGrid over the padded AABB (Dataset.cpp:94-164)       */
+    int32_t  dg_dims[3];
+    uint8_t  *dg_flag_box;          /* any particle with |x_j-c|_inf < h      */
+    uint8_t  *dg_flag_cons;         /* any particle with |x_j-c|_inf < 1.5h   */
+    uint32_t *dg_count_box;
+} or_grid;
+
+/* ---- SPH cubic spline kernel (Kernel.cpp:8-52) ---- */
+float  or_W_f(float r2, float h);
+double or_W_d(double r2, double h);
+void   or_gradW_f(const float r[3], float h, float out[3]);
+float  or_sigma_f(float h);
+
+/* ---- frame structures ---- */
+int32_t or_key(float x, float inv_h);
+int  or_grid_build(const float *xyz, uint64_t n, float h, or_grid *g);
+void or_grid_free(or_grid *g);
+uint64_t or_grid_struct_size(void);
+
+/* ---- neighbour search ---- */
+uint32_t or_neighbors_brute(const float *xyz, uint64_t n, float h, const float q[3],
+                            uint32_t *out, uint32_t cap);
+uint32_t or_neighbors_grid(const or_grid *g, const float q[3], uint32_t *out, uint32_t cap,
+                           uint32_t *ncand);
+uint32_t or_neighbors_hash(const float *xyz, uint64_t n, float h, const float q[3],
+                           uint32_t *out, uint32_t cap);
+
+/* ---- density ---- */
+float  or_density_f(const or_grid *g, const float q[3], int compensated, uint32_t *ncand);
+double or_density_d(const or_grid *g, const double q[3]);
+void   or_density_batch_f(const or_grid *g, const float *q, uint64_t nq, int compensated,
+                          float *rho, uint32_t *ncand);
+void   or_density_batch_d(const or_grid *g, const double *q, uint64_t nq, double *rho);
+
+/* ---- ray generation (RayMarcher.cpp:268-274) and AABB slab test ---- */
+void or_ray(const float invpv[16], const float cam[3], uint32_t W, uint32_t H, uint32_t idx,
+            float o[3], float d[3]);
+int  or_slab(const float o[3], const float d[3], const float bmin[3], const float bmax[3], float tnf[2]);
+
+/* ---- march ---- */
+int or_march(const or_grid *g, const or_settings *s, const float invpv[16], const float cam[3],
+             uint32_t W, uint32_t H, const float *depth, uint32_t row_step, int nthreads,
+             float *pos, float *nrm, float *tout, or_counters *cnt);
+int or_march_f64(const or_grid *g, const or_settings *s, const float invpv[16], const float cam[3],
+                 uint32_t W, uint32_t H, uint32_t row_step, int nthreads,
+                 double *pos, double *nrm, double *tout);
+
+/* ---- reference-style CPU baseline: per-sample hashed 27-cell query ---- */
+typedef struct or_hash or_hash;
+or_hash *or_hash_build(const float *xyz, uint64_t n, float h);
+void     or_hash_free(or_hash *hs);
+int or_march_hashed(const or_hash *hs, const or_grid *g, const or_settings *s,
+                    const float invpv[16], const float cam[3], uint32_t W, uint32_t H,
+                    uint32_t row_step, int nthreads, float *pos, float *nrm, float *tout,
+                    or_counters *cnt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,107 @@
+"""GPU parity: grid build, neighbour counts, density and the configured march,
+through the C ABI, against the CPU oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+
+import oracle as O
+from parity import assert_within, compare, extent
+from pmarch import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid_equal(m, frame, xyz, h):
+    fi, cs, srt, idx = m.grid(frame)
+    G = O.Grid(xyz, h)
+    assert tuple(fi.dims) == G.dims and tuple(fi.origin) == G.origin
+    assert np.array_equal(np.array(fi.bmin[:], np.float32), G.bmin)
+    assert np.array_equal(np.array(fi.bmax[:], np.float32), G.bmax)
+    assert np.array_equal(cs, G.cell_start()), "cell_start differs"
+    assert np.array_equal(idx, G.sorted_index()), "stored particle order differs"
+    assert np.array_equal(srt, G.sorted_xyz()), "stored coordinates differ"
+    return G
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_grid_lattice_cube(marcher, seed):
+    sc = scenes.lattice_cube(seed)
+    print(f"grid lattice_cube seed={seed} n={sc.n}", flush=True)
+    marcher.set_frame(0, sc.xyz, sc.h)
+    _grid_equal(marcher, 0, sc.xyz, sc.h)
+
+
+def test_grid_dam_break(marcher):
+    sc = scenes.dam_break(0)
+    print(f"grid dam_break n={sc.n}", flush=True)
+    marcher.set_frame(1, sc.xyz, sc.h)
+    _grid_equal(marcher, 1, sc.xyz, sc.h)
+
+
+def test_grid_negative_coordinates(marcher):
+    rng = np.random.default_rng(5)
+    xyz = rng.uniform(-0.35, 0.2, (5000, 3)).astype(np.float32)
+    xyz[:7] = [[-0.1, 0, 0], [0, -0.1, 0], [0, 0, -0.1], [-0.2, -0.2, -0.2], [0.1, 0.1, 0.1], [0, 0, 0], [-0.0, 0.05, -0.3]]
+    marcher.set_frame(2, xyz, 0.1)
+    _grid_equal(marcher, 2, xyz, 0.1)
+
+
+def test_neighbor_counts(marcher):
+    sc = scenes.lattice_cube(1)
+    marcher.set_frame(0, sc.xyz, sc.h)
+    G = O.Grid(sc.xyz, sc.h)
+    rng = np.random.default_rng(3)
+    q = rng.uniform(G.bmin - 0.05, G.bmax + 0.05, (4000, 3)).astype(np.float32)
+    cnt, cand = marcher.neighbor_counts(0, q)
+    for i in range(0, 4000, 13):
+        nb, nc = G.neighbors(q[i])
+        assert cnt[i] == len(nb) and cand[i] == nc, i
+    # a brute-force pin on a subset
+    for i in range(0, 4000, 397):
+        assert cnt[i] == len(O.neighbors_brute(sc.xyz, sc.h, q[i]))
+
+
+@pytest.mark.parametrize("compensated", [1, 0])
+def test_density_bitexact(marcher, compensated):
+    sc = scenes.dam_break(2, nx=24, ny=40, nz=24)
+    marcher.set_frame(3, sc.xyz, sc.h)
+    G = O.Grid(sc.xyz, sc.h)
+    rng = np.random.default_rng(11)
+    q = rng.uniform(G.bmin, G.bmax, (20000, 3)).astype(np.float32)
+    rg = marcher.density(3, q, compensated=bool(compensated))
+    ro = G.density(q, compensated=bool(compensated))
+    assert np.array_equal(rg, ro), f"max diff {np.abs(rg - ro).max()}"
+    q64 = q.astype(np.float64) + rng.uniform(-1e-4, 1e-4, q.shape)
+    assert np.array_equal(marcher.density64(3, q64), G.density64(q64))
+
+
+@pytest.mark.parametrize("seed,cam", [(0, 0), (1, 1), (2, 0)])
+def test_march_lattice_cube(marcher, seed, cam):
+    sc = scenes.lattice_cube(seed)
+    c = sc.cameras[cam]
+    print(f"march lattice_cube seed={seed} cam={cam} {c.width}x{c.height}", flush=True)
+    marcher.set_frame(0, sc.xyz, sc.h)
+    st = sc.settings()
+    pos, nrm, t, stats = marcher.march(0, st, c.inv_pv, c.eye, c.width, c.height)
+    G = O.Grid(sc.xyz, sc.h)
+    opos, onrm, ot, ocnt = O.march(G, O.make_settings(**st), c.inv_pv, c.eye, c.width, c.height)
+    m = compare(pos, nrm, t, opos, onrm, ot)
+    print(m, flush=True)
+    assert_within(m, extent(sc.xyz))
+    assert m["hits_b"] > 50000
+    # same canonical order and no contraction on either side: identical images
+    assert np.array_equal(pos, opos) and np.array_equal(nrm, onrm) and np.array_equal(t, ot)
+    for k in ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples"):
+        assert stats[k] == ocnt[k], (k, stats[k], ocnt[k])
+
+
+def test_march_vs_fp64(marcher):
+    sc = scenes.lattice_cube(0)
+    c = sc.cameras[0]
+    marcher.set_frame(0, sc.xyz, sc.h)
+    st = sc.settings()
+    pos, nrm, t, _ = marcher.march(0, st, c.inv_pv, c.eye, c.width, c.height)
+    G = O.Grid(sc.xyz, sc.h)
+    p64, n64, t64 = O.march64(G, O.make_settings(**st), c.inv_pv, c.eye, c.width, c.height)
+    m = compare(pos, nrm, t, p64, n64, t64)
+    print("vs FP64:", m, flush=True)
+    assert_within(m, extent(sc.xyz))

@@ -1,0 +1,97 @@
+"""Pins of the oracle's uniform grid and neighbour search (Dataset.cpp:49-76, 272-280;
+the CompactNSearch point query) against brute force and numpy -- no GPU."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def np_keys(xyz, h):
+    """int(x / h), minus one for negative x, in float32 (x * (1/h))."""
+    inv = np.float32(1.0) / np.float32(h)
+    v = (inv * xyz.astype(np.float32)).astype(np.float32)
+    k = np.trunc(v).astype(np.int64)
+    return np.where(xyz < 0, k - 1, k)
+
+
+def test_keys_negative_and_zero():
+    h = 0.1
+    assert O.key(0.0, h) == 0
+    assert O.key(-0.0, h) == 0
+    assert O.key(0.15, h) == 1
+    assert O.key(-0.05, h) == -1
+    assert O.key(-0.1, h) == -2      # int(-1.0000000149) - 1
+    assert O.key(-0.25, h) == -3
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-2, 2, 2000).astype(np.float32)
+    assert all(O.key(float(v), h) == k for v, k in zip(x, np_keys(x, h)))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_counting_sort_layout(seed):
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(-0.3, 0.45, (6000, 3)).astype(np.float32)
+    h = 0.07
+    G = O.Grid(xyz, h)
+    k = np_keys(xyz, h)
+    org = np.array(G.origin)
+    dims = np.array(G.dims)
+    assert np.array_equal(org, k.min(0) - 2) and np.array_equal(dims, k.max(0) - k.min(0) + 5)
+    c = k - org
+    lin = c[:, 0] + dims[0] * (c[:, 1] + dims[1] * c[:, 2])
+    counts = np.bincount(lin, minlength=int(np.prod(dims)))
+    cs = G.cell_start()
+    assert np.array_equal(np.diff(cs), counts)
+    order = np.lexsort((np.arange(len(xyz)), lin))      # by cell, then input index
+    assert np.array_equal(G.sorted_index(), order)
+    assert np.array_equal(G.sorted_xyz()[:, :3], xyz[order])
+    # padded AABB = bounds -+ h (Frame::ComputeAABB)
+    assert np.array_equal(G.bmin, (xyz.min(0) - np.float32(h)).astype(np.float32))
+    assert np.array_equal(G.bmax, (xyz.max(0) + np.float32(h)).astype(np.float32))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_neighbors_grid_hash_brute(seed):
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(-0.2, 0.3, (3000, 3)).astype(np.float32)
+    h = 0.06
+    G = O.Grid(xyz, h)
+    q = rng.uniform(-0.3, 0.4, (300, 3)).astype(np.float32)
+    q[:20] = xyz[:20]                       # queries exactly on particles
+    for p in q:
+        nb_g, cand = G.neighbors(p)
+        nb_b = O.neighbors_brute(xyz, h, p)
+        nb_h = O.neighbors_hash(xyz, h, p)
+        assert np.array_equal(np.sort(nb_g), nb_b)          # same set as brute force
+        assert np.array_equal(nb_g, nb_h)                   # same sequence as the hashed table
+        # candidates = particles in the 27 cells around key(p)
+        kp = np_keys(p[None], h)[0]
+        kk = np_keys(xyz, h)
+        assert cand == int(np.all(np.abs(kk - kp) <= 1, axis=1).sum())
+
+
+def test_neighbors_strict_support():
+    h = 0.125
+    xyz = np.array([[0, 0, 0], [0.125, 0, 0], [0.0625, 0, 0]], np.float32)
+    G = O.Grid(xyz, h)
+    nb, _ = G.neighbors(np.zeros(3, np.float32))
+    assert sorted(nb.tolist()) == [0, 2]            # |r| = h is excluded
+
+
+def test_densitygrid_conservative_flag_clears_only_empty_space():
+    rng = np.random.default_rng(3)
+    xyz = np.concatenate([rng.uniform(0, 0.3, (2000, 3)), rng.uniform(0.7, 0.8, (300, 3))]).astype(np.float32)
+    h = 0.05
+    G = O.Grid(xyz, h)
+    dims, count_box, flag_box, flag_cons = G.densitygrid()
+    ext = G.bmax - G.bmin
+    assert dims == tuple(int(np.ceil(np.float32(e) / np.float32(h))) for e in ext)
+    assert flag_box.sum() <= flag_cons.sum()
+    # sample points inside cleared cells: density exactly zero
+    cleared = np.flatnonzero(flag_cons == 0)
+    pick = rng.choice(cleared, 300, replace=False)
+    z, rem = np.divmod(pick, dims[0] * dims[1])
+    y, x = np.divmod(rem, dims[0])
+    cell = np.stack([x, y, z], -1)
+    pts = G.bmin + (cell + rng.uniform(0, 1, cell.shape)) * h
+    assert np.all(G.density64(pts) == 0.0)
