@@ -12,7 +12,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -42,10 +44,15 @@ struct Frame {
     pm_frame_info info{};
     GridDev g{};
     DevBuf sorted, cell_start, cell_of, counts, cursor, scan_tmp, xyz, bounds_partial, bounds;
+    DevBuf empty27, dg_box, dg_cons, dg_count, coarse;
+    int32_t dg_dims[3] = { 0, 0, 0 };
+    CoarseDev cg{};
     void release()
     {
+        coarse.release();
         sorted.release(); cell_start.release(); cell_of.release(); counts.release(); cursor.release();
         scan_tmp.release(); xyz.release(); bounds_partial.release(); bounds.release();
+        empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release();
     }
 };
 
@@ -58,7 +65,7 @@ struct pm_ctx {
     std::map<int32_t, Frame *> frames;
     std::string err;
     // outputs of the last march
-    DevBuf d_pos, d_nrm, d_t, d_depth, d_counters, d_q, d_out;
+    DevBuf d_pos, d_nrm, d_t, d_depth, d_counters, d_q, d_out, d_band;
     uint64_t npx = 0;
     bool have_t = false;
     // pinned staging for asynchronous hand-off
@@ -69,7 +76,24 @@ struct pm_ctx {
     bool pending = false;
     bool finished = true;
     pm_stats stats{};
+    // learned step proposal: FP32 master weights, Adam moments, BF16 copies
+    DevBuf mlp_p, mlp_m, mlp_v, mlp_W1, mlp_W2, mlp_W2T, mlp_partial, mlp_out, mlp_x, mlp_idx;
+    bool mlp_ready = false;
+    int mlp_step = 0;
+    // training set written by the trace logger
+    DevBuf ds_feat, ds_label, ds_hit, ds_count, ds_meta;
+    uint64_t ds_cap = 0;
 };
+
+static MlpDev mlp_dev(pm_ctx *c)
+{
+    MlpDev n{};
+    n.W1 = (const uint16_t *)c->mlp_W1.p;
+    n.W2 = (const uint16_t *)c->mlp_W2.p;
+    n.W2T = (const uint16_t *)c->mlp_W2T.p;
+    n.p = (const float *)c->mlp_p.p;
+    return n;
+}
 
 static int fail(pm_ctx *c, int code, const char *fmt, ...)
 {
@@ -90,6 +114,23 @@ static int fail(pm_ctx *c, int code, const char *fmt, ...)
         if (_e != hipSuccess) return fail((c), PM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
     } while (0)
 
+// Every transfer goes through the context stream and waits for it: the stream
+// is non-blocking, so null-stream copies or fills would not be ordered with the
+// kernels queued on it.
+static hipError_t copy_sync(pm_ctx *c, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+static hipError_t zero_sync(pm_ctx *c, void *dst, size_t bytes)
+{
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, c->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
 extern "C" {
 
 int pm_create(int device, pm_ctx **out)
@@ -105,6 +146,7 @@ int pm_create(int device, pm_ctx **out)
         hipEventCreate(&c->ev_launch) != hipSuccess || hipEventCreate(&c->ev_k0) != hipSuccess ||
         hipEventCreate(&c->ev_k1) != hipSuccess || hipEventCreate(&c->ev_done) != hipSuccess ||
         c->d_counters.ensure(sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
+        c->d_band.ensure(sizeof(uint32_t) * 64) != hipSuccess ||
         hipHostMalloc((void **)&c->h_counters, sizeof(unsigned long long) * C_COUNT, hipHostMallocDefault) != hipSuccess) {
         pm_destroy(c);
         return PM_ERR_HIP;
@@ -120,7 +162,10 @@ void pm_destroy(pm_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto &kv : c->frames) { kv.second->release(); delete kv.second; }
     c->d_pos.release(); c->d_nrm.release(); c->d_t.release(); c->d_depth.release();
-    c->d_counters.release(); c->d_q.release(); c->d_out.release();
+    c->d_counters.release(); c->d_q.release(); c->d_out.release(); c->d_band.release();
+    c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release(); c->mlp_W1.release(); c->mlp_W2.release();
+    c->mlp_W2T.release(); c->mlp_partial.release(); c->mlp_out.release(); c->mlp_x.release(); c->mlp_idx.release();
+    c->ds_feat.release(); c->ds_label.release(); c->ds_hit.release(); c->ds_count.release(); c->ds_meta.release();
     if (c->h_stage) hipHostFree(c->h_stage);
     if (c->h_counters) hipHostFree(c->h_counters);
     if (c->ev_launch) hipEventDestroy(c->ev_launch);
@@ -215,6 +260,31 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     g.hinv_d = 1.0 / g.h_d;
     g.sig_d = 8.0 / (PM_PI_D * g.h_d * g.h_d * g.h_d);
     for (int a2 = 0; a2 < 3; a2++) { g.bmin[a2] = I.bmin[a2]; g.bmax[a2] = I.bmax[a2]; }
+
+    // skip structures: empty 27-neighbourhoods and the reference DensityGrid
+    HIPCHK(c, f->empty27.ensure(nc));
+    HIPCHK(c, launch_empty27((const uint32_t *)f->cell_start.p, I.dims[0], I.dims[1], I.dims[2], (uint8_t *)f->empty27.p, c->stream));
+    g.empty27 = (const uint8_t *)f->empty27.p;
+    uint64_t dgn = 1;
+    for (int a2 = 0; a2 < 3; a2++) {
+        float cw = 1.0f * h;                            // Dataset.cpp:96
+        f->dg_dims[a2] = (int32_t)std::ceil((I.bmax[a2] - I.bmin[a2]) / cw);   // :102-104
+        dgn *= (uint64_t)f->dg_dims[a2];
+    }
+    if (dgn >= (1ull << 31)) return fail(c, PM_ERR_SIZE, "pm_set_frame: DensityGrid too large");
+    HIPCHK(c, f->dg_box.ensure(dgn));
+    HIPCHK(c, f->dg_cons.ensure(dgn));
+    HIPCHK(c, f->dg_count.ensure(4 * dgn));
+    HIPCHK(c, launch_densitygrid(g, f->dg_dims[0], f->dg_dims[1], f->dg_dims[2], (uint8_t *)f->dg_box.p, (uint8_t *)f->dg_cons.p,
+                                 (uint32_t *)f->dg_count.p, c->stream));
+    g.dg_nx = f->dg_dims[0]; g.dg_ny = f->dg_dims[1]; g.dg_nz = f->dg_dims[2];
+    g.dg_flag = nullptr;
+    // coarse occupancy (4x4x4 dense cells) for the step-proposal features
+    f->cg.nx = (I.dims[0] + 3) / 4; f->cg.ny = (I.dims[1] + 3) / 4; f->cg.nz = (I.dims[2] + 3) / 4;
+    HIPCHK(c, f->coarse.ensure(4ull * f->cg.nx * f->cg.ny * f->cg.nz));
+    f->cg.count = (const uint32_t *)f->coarse.p;
+    HIPCHK(c, launch_coarse_counts(g, f->cg, (uint32_t *)f->coarse.p, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return PM_OK;
 }
 
@@ -289,8 +359,14 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
     if (!std::isfinite(s->iso)) return fail(c, PM_ERR_ARG, "iso must be finite");
     if (s->bisect_iters < 0 || s->bisect_iters > 40) return fail(c, PM_ERR_ARG, "bisect_iters out of [0,40]");
     if (s->normal_mode != PM_NORMAL_CDIFF && s->normal_mode != PM_NORMAL_GRADW) return fail(c, PM_ERR_ARG, "unknown normal_mode");
-    if (s->skip_mode != PM_SKIP_NONE) return fail(c, PM_ERR_ARG, "skip_mode %d is not available in this build", s->skip_mode);
-    if (s->learned) return fail(c, PM_ERR_ARG, "learned step: no network loaded");
+    if (s->skip_mode < PM_SKIP_NONE || s->skip_mode > PM_SKIP_DG_CONS) return fail(c, PM_ERR_ARG, "unknown skip_mode %d", s->skip_mode);
+    if (s->mode == PM_MODE_CONFIG && (s->skip_mode == PM_SKIP_DG_BOX || s->skip_mode == PM_SKIP_DG_CONS))
+        return fail(c, PM_ERR_ARG, "the DensityGrid skip belongs to the reference march");
+    if (s->mode == PM_MODE_REFERENCE && s->skip_mode == PM_SKIP_EMPTY)
+        return fail(c, PM_ERR_ARG, "the empty-stencil skip belongs to the configured march");
+    if (s->learned && !c->mlp_ready) return fail(c, PM_ERR_STATE, "learned step: no network loaded (pm_mlp_set)");
+    if (s->learned && s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "the learned step belongs to the configured march");
+    if (s->learned && !(s->learned_alpha >= 0.0f && s->learned_alpha <= 4.0f)) return fail(c, PM_ERR_ARG, "learned_alpha out of [0,4]");
     if (s->mode == PM_MODE_REFERENCE && (s->max_steps < 0 || s->max_steps > (1 << 24)))
         return fail(c, PM_ERR_ARG, "max_steps out of range");
     if (s->mode == PM_MODE_CONFIG) {
@@ -328,6 +404,7 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
 
     MarchLaunch L{};
     L.g = f->g;
+    L.g.dg_flag = (const uint8_t *)(s->skip_mode == PM_SKIP_DG_BOX ? f->dg_box.p : f->dg_cons.p);
     L.s = *s;
     memcpy(L.M, inv_pv, sizeof(L.M));
     memcpy(L.cam, cam, sizeof(L.cam));
@@ -336,6 +413,14 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     L.nrm = (float4 *)c->d_nrm.p;
     L.t = (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
+    L.band_ctr = (uint32_t *)c->d_band.p;
+    L.cg = f->cg;
+    if (s->learned) {
+        L.net = mlp_dev(c);
+        L.mlp_alpha = s->learned_alpha > 0.0f ? s->learned_alpha : 0.7f;
+        L.mlp_min_skip = s->learned_min_skip > 0 ? s->learned_min_skip : 4;
+        L.mlp_max_skip = s->learned_max_skip > 0 ? s->learned_max_skip : 24;
+    }
 
     HIPCHK(c, hipEventRecord(c->ev_launch, c->stream));
     if (s->mode == PM_MODE_REFERENCE) {
@@ -421,9 +506,9 @@ int pm_download(pm_ctx *c, float *positions, float *normals, float *t)
     if (c->pending) return fail(c, PM_ERR_STATE, "pm_download: march in flight; call pm_wait first");
     if (c->npx == 0) return fail(c, PM_ERR_STATE, "pm_download: nothing marched yet");
     HIPCHK(c, hipSetDevice(c->device));
-    if (positions) HIPCHK(c, hipMemcpy(positions, c->d_pos.p, 16 * c->npx, hipMemcpyDeviceToHost));
-    if (normals) HIPCHK(c, hipMemcpy(normals, c->d_nrm.p, 16 * c->npx, hipMemcpyDeviceToHost));
-    if (t) HIPCHK(c, hipMemcpy(t, c->d_t.p, 4 * c->npx, hipMemcpyDeviceToHost));
+    if (positions) HIPCHK(c, copy_sync(c, positions, c->d_pos.p, 16 * c->npx, hipMemcpyDeviceToHost));
+    if (normals) HIPCHK(c, copy_sync(c, normals, c->d_nrm.p, 16 * c->npx, hipMemcpyDeviceToHost));
+    if (t) HIPCHK(c, copy_sync(c, t, c->d_t.p, 4 * c->npx, hipMemcpyDeviceToHost));
     return PM_OK;
 }
 
@@ -490,6 +575,289 @@ int pm_density64(pm_ctx *c, int32_t frame, const double *q, uint64_t nq, double 
     HIPCHK(c, launch_density64(f->g, (const double *)c->d_q.p, (uint32_t)nq, (double *)c->d_out.p, c->stream));
     HIPCHK(c, hipMemcpyAsync(rho, c->d_out.p, 8 * nq, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
+// ---------------------------------------------------------------- learned --
+
+int pm_mlp_set(pm_ctx *c, const float *params)
+{
+    if (!c || !params) return PM_ERR_ARG;
+    if (c->pending) return fail(c, PM_ERR_STATE, "a march is in flight");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t nb = sizeof(float) * PM_MLP_NPARAM;
+    HIPCHK(c, c->mlp_p.ensure(nb));
+    HIPCHK(c, c->mlp_m.ensure(nb));
+    HIPCHK(c, c->mlp_v.ensure(nb));
+    HIPCHK(c, c->mlp_W1.ensure(2 * PM_MLP_H * PM_MLP_K1));
+    HIPCHK(c, c->mlp_W2.ensure(2 * PM_MLP_H * PM_MLP_H));
+    HIPCHK(c, c->mlp_W2T.ensure(2 * PM_MLP_H * PM_MLP_H));
+    std::vector<float> p(params, params + PM_MLP_NPARAM);
+    for (int i = 0; i < PM_MLP_H; i++)
+        for (int j = PM_MLP_IN; j < PM_MLP_K1; j++) p[PM_MLP_OFF_W1 + i * PM_MLP_K1 + j] = 0.0f;
+    for (float v : p)
+        if (!std::isfinite(v)) return fail(c, PM_ERR_ARG, "pm_mlp_set: non-finite parameter");
+    HIPCHK(c, copy_sync(c, c->mlp_p.p, p.data(), nb, hipMemcpyHostToDevice));
+    HIPCHK(c, zero_sync(c, c->mlp_m.p, nb));
+    HIPCHK(c, zero_sync(c, c->mlp_v.p, nb));
+    HIPCHK(c, launch_mlp_sync((const float *)c->mlp_p.p, (uint16_t *)c->mlp_W1.p, (uint16_t *)c->mlp_W2.p,
+                              (uint16_t *)c->mlp_W2T.p, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->mlp_ready = true;
+    c->mlp_step = 0;
+    return PM_OK;
+}
+
+int pm_mlp_get(pm_ctx *c, float *params)
+{
+    if (!c || !params) return PM_ERR_ARG;
+    if (!c->mlp_ready) return fail(c, PM_ERR_STATE, "no network loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, copy_sync(c, params, c->mlp_p.p, sizeof(float) * PM_MLP_NPARAM, hipMemcpyDeviceToHost));
+    return PM_OK;
+}
+
+int pm_mlp_forward(pm_ctx *c, const float *x, uint64_t n, float *y)
+{
+    if (!c || !x || !y || n == 0 || n >= (1ull << 31)) return PM_ERR_ARG;
+    if (!c->mlp_ready) return fail(c, PM_ERR_STATE, "no network loaded");
+    if (c->pending) return fail(c, PM_ERR_STATE, "a march is in flight");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->mlp_x.ensure(sizeof(float) * PM_MLP_IN * n));
+    HIPCHK(c, c->mlp_out.ensure(sizeof(float) * n));
+    HIPCHK(c, copy_sync(c, c->mlp_x.p, x, sizeof(float) * PM_MLP_IN * n, hipMemcpyHostToDevice));
+    HIPCHK(c, launch_mlp_forward(mlp_dev(c), (const float *)c->mlp_x.p, (uint32_t)n, (float *)c->mlp_out.p, c->stream));
+    HIPCHK(c, hipMemcpyAsync(y, c->mlp_out.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
+// One forward/backward over device rows `feat`/`label` (ids from idx, or 0..n-1),
+// partial gradients reduced in wave order; optionally one Adam step.
+static int mlp_pass(pm_ctx *c, const float *feat, const float *label, const uint32_t *idx, uint32_t n, float w_over,
+                    bool apply, float lr, float *loss_dev_out, float *grad_dev_out)
+{
+    uint32_t nw = n / 64;
+    HIPCHK(c, c->mlp_partial.ensure(sizeof(float) * (size_t)nw * (PM_MLP_NPARAM + 1)));
+    TrainArgs ta{};
+    ta.net = mlp_dev(c);
+    ta.feat = feat;
+    ta.label = label;
+    ta.idx = idx;
+    ta.inv_batch = 1.0f / (float)n;
+    ta.w_over = w_over;
+    ta.partial = (float *)c->mlp_partial.p;
+    HIPCHK(c, launch_mlp_train(ta, nw, c->stream));
+    AdamArgs aa{};
+    aa.partial = (const float *)c->mlp_partial.p;
+    aa.nparts = nw;
+    aa.p = (float *)c->mlp_p.p;
+    aa.m = (float *)c->mlp_m.p;
+    aa.v = (float *)c->mlp_v.p;
+    aa.apply = apply ? 1 : 0;
+    aa.grad_out = grad_dev_out;
+    aa.loss_out = loss_dev_out;
+    if (apply) {
+        c->mlp_step++;
+        aa.lr = lr;
+        aa.b1 = 0.9f;
+        aa.b2 = 0.999f;
+        aa.eps = 1e-8f;
+        aa.bc1 = 1.0f - std::pow(0.9f, (float)c->mlp_step);
+        aa.bc2 = 1.0f - std::pow(0.999f, (float)c->mlp_step);
+    }
+    HIPCHK(c, launch_mlp_adam(aa, c->stream));
+    if (apply)
+        HIPCHK(c, launch_mlp_sync((const float *)c->mlp_p.p, (uint16_t *)c->mlp_W1.p, (uint16_t *)c->mlp_W2.p,
+                                  (uint16_t *)c->mlp_W2T.p, c->stream));
+    return PM_OK;
+}
+
+int pm_mlp_grad(pm_ctx *c, const float *x, const float *target, uint64_t n, float w_over, float *loss, float *grad)
+{
+    if (!c || !x || !target || !loss || !grad) return PM_ERR_ARG;
+    if (n == 0 || n % 64 != 0 || n > (1u << 24)) return fail(c, PM_ERR_ARG, "pm_mlp_grad: n must be a positive multiple of 64");
+    if (!c->mlp_ready) return fail(c, PM_ERR_STATE, "no network loaded");
+    if (c->pending) return fail(c, PM_ERR_STATE, "a march is in flight");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->mlp_x.ensure(sizeof(float) * (PM_MLP_IN + 1) * n));
+    HIPCHK(c, c->mlp_out.ensure(sizeof(float) * (PM_MLP_NPARAM + 1)));
+    float *dx = (float *)c->mlp_x.p, *dt = dx + PM_MLP_IN * n;
+    HIPCHK(c, copy_sync(c, dx, x, sizeof(float) * PM_MLP_IN * n, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_sync(c, dt, target, sizeof(float) * n, hipMemcpyHostToDevice));
+    float *dg = (float *)c->mlp_out.p;
+    int rc = mlp_pass(c, dx, dt, nullptr, (uint32_t)n, w_over, false, 0.f, dg + PM_MLP_NPARAM, dg);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(grad, dg, sizeof(float) * PM_MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(loss, dg + PM_MLP_NPARAM, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
+int pm_dataset_clear(pm_ctx *c, uint64_t capacity)
+{
+    if (!c || capacity == 0 || capacity >= (1ull << 31)) return PM_ERR_ARG;
+    if (c->pending) return fail(c, PM_ERR_STATE, "a march is in flight");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, c->ds_feat.ensure(sizeof(float) * PM_MLP_IN * capacity));
+    HIPCHK(c, c->ds_label.ensure(sizeof(float) * capacity));
+    HIPCHK(c, c->ds_hit.ensure(capacity));
+    HIPCHK(c, c->ds_meta.ensure(8 * capacity));
+    HIPCHK(c, c->ds_count.ensure(sizeof(uint32_t)));
+    HIPCHK(c, zero_sync(c, c->ds_count.p, sizeof(uint32_t)));
+    c->ds_cap = capacity;
+    return PM_OK;
+}
+
+static int ds_size(pm_ctx *c, uint64_t *n)
+{
+    uint32_t v = 0;
+    HIPCHK(c, copy_sync(c, &v, c->ds_count.p, sizeof(v), hipMemcpyDeviceToHost));
+    if (v > c->ds_cap) {                     // the logger overshoots its cap by the rows it dropped
+        v = (uint32_t)c->ds_cap;
+    }
+    *n = v;
+    return PM_OK;
+}
+
+int pm_trace_append(pm_ctx *c, int32_t frame, const pm_settings *s, const float inv_pv[16], const float cam[3], uint32_t W,
+                    uint32_t H, uint32_t ray_stride, uint32_t k_stride, uint32_t seed, uint64_t max_new_rows,
+                    uint64_t *rows_total)
+{
+    if (!c || !s || !inv_pv || !cam || ray_stride == 0 || k_stride == 0) return PM_ERR_ARG;
+    if (c->ds_cap == 0) return fail(c, PM_ERR_STATE, "pm_trace_append: call pm_dataset_clear first");
+    if (s->mode != PM_MODE_CONFIG || s->learned) return fail(c, PM_ERR_ARG, "pm_trace_append: traces come from the plain configured march");
+    int rc = pm_march_async(c, frame, s, inv_pv, cam, W, H, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    rc = pm_wait(c);
+    if (rc) return rc;
+    uint64_t before = 0;
+    rc = ds_size(c, &before);
+    if (rc) return rc;
+    uint64_t cap = c->ds_cap;
+    if (max_new_rows > 0 && before + max_new_rows < cap) cap = before + max_new_rows;
+    Frame *f = get_frame(c, frame);
+    TraceArgs a{};
+    a.g = f->g;
+    a.cg = f->cg;
+    memcpy(a.M, inv_pv, sizeof(a.M));
+    memcpy(a.cam, cam, sizeof(a.cam));
+    a.W = W; a.H = H;
+    a.step = s->step;
+    a.iso = s->iso;
+    a.t_hit = (const float *)c->d_t.p;
+    a.ray_stride = ray_stride;
+    a.k_stride = k_stride;
+    a.seed = seed;
+    a.count = (uint32_t *)c->ds_count.p;
+    a.cap = (uint32_t)cap;
+    a.feat = (float *)c->ds_feat.p;
+    a.label = (float *)c->ds_label.p;
+    a.is_hit = (uint8_t *)c->ds_hit.p;
+    a.meta = (uint32_t *)c->ds_meta.p;
+    HIPCHK(c, launch_trace(a, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // rows past the cap were counted but not written: bring the counter back to the cap
+    uint32_t v = 0;
+    HIPCHK(c, copy_sync(c, &v, c->ds_count.p, sizeof(v), hipMemcpyDeviceToHost));
+    if (v > cap) {
+        v = (uint32_t)cap;
+        HIPCHK(c, copy_sync(c, c->ds_count.p, &v, sizeof(v), hipMemcpyHostToDevice));
+    }
+    if (rows_total) *rows_total = v;
+    return PM_OK;
+}
+
+int pm_dataset_get(pm_ctx *c, uint64_t offset, uint64_t n, float *feat, float *label, uint8_t *is_hit, uint32_t *meta)
+{
+    if (!c) return PM_ERR_ARG;
+    uint64_t tot = 0;
+    int rc = ds_size(c, &tot);
+    if (rc) return rc;
+    if (offset + n > tot) return fail(c, PM_ERR_ARG, "pm_dataset_get: range beyond the %llu rows", (unsigned long long)tot);
+    if (feat) HIPCHK(c, copy_sync(c, feat, (float *)c->ds_feat.p + PM_MLP_IN * offset, sizeof(float) * PM_MLP_IN * n, hipMemcpyDeviceToHost));
+    if (label) HIPCHK(c, copy_sync(c, label, (float *)c->ds_label.p + offset, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (is_hit) HIPCHK(c, copy_sync(c, is_hit, (uint8_t *)c->ds_hit.p + offset, n, hipMemcpyDeviceToHost));
+    if (meta) HIPCHK(c, copy_sync(c, meta, (uint32_t *)c->ds_meta.p + 2 * offset, 8 * n, hipMemcpyDeviceToHost));
+    return PM_OK;
+}
+
+int pm_mlp_train(pm_ctx *c, int epochs, uint32_t batch, float lr, float w_over, uint64_t seed, float *loss_per_epoch)
+{
+    if (!c || epochs <= 0 || batch == 0 || batch % 64 != 0 || !(lr > 0.0f)) return PM_ERR_ARG;
+    if (!c->mlp_ready) return fail(c, PM_ERR_STATE, "no network loaded (pm_mlp_set)");
+    if (c->pending) return fail(c, PM_ERR_STATE, "a march is in flight");
+    HIPCHK(c, hipSetDevice(c->device));
+    uint64_t n = 0;
+    int rc = ds_size(c, &n);
+    if (rc) return rc;
+    if (n < batch) return fail(c, PM_ERR_STATE, "pm_mlp_train: %llu rows, fewer than one batch", (unsigned long long)n);
+    uint64_t nbatches = n / batch;
+    HIPCHK(c, c->mlp_idx.ensure(sizeof(uint32_t) * nbatches * batch));
+    HIPCHK(c, c->mlp_out.ensure(sizeof(float) * (nbatches + PM_MLP_NPARAM + 1)));
+    std::vector<uint32_t> perm(n);
+    std::vector<float> losses(nbatches);
+    std::mt19937_64 rng(seed);
+    for (int e = 0; e < epochs; e++) {
+        for (uint64_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
+        std::shuffle(perm.begin(), perm.end(), rng);
+        HIPCHK(c, copy_sync(c, c->mlp_idx.p, perm.data(), sizeof(uint32_t) * nbatches * batch, hipMemcpyHostToDevice));
+        for (uint64_t b = 0; b < nbatches; b++) {
+            rc = mlp_pass(c, (const float *)c->ds_feat.p, (const float *)c->ds_label.p,
+                          (const uint32_t *)c->mlp_idx.p + b * batch, batch, w_over, true, lr, (float *)c->mlp_out.p + b,
+                          nullptr);
+            if (rc) return rc;
+        }
+        HIPCHK(c, copy_sync(c, losses.data(), c->mlp_out.p, sizeof(float) * nbatches, hipMemcpyDeviceToHost));
+        double sum = 0.0;
+        for (float v : losses) sum += v;
+        if (loss_per_epoch) loss_per_epoch[e] = (float)(sum / (double)nbatches);
+    }
+    return PM_OK;
+}
+
+int pm_get_skip_grids(pm_ctx *c, int32_t frame, uint8_t *empty27, int32_t dg_dims[3], uint8_t *flag_box,
+                      uint8_t *flag_cons, uint32_t *count_box)
+{
+    if (!c) return PM_ERR_ARG;
+    Frame *f = get_frame(c, frame);
+    if (!f) return fail(c, PM_ERR_FRAME, "frame %d not set", frame);
+    HIPCHK(c, hipSetDevice(c->device));
+    uint64_t dgn = (uint64_t)f->dg_dims[0] * f->dg_dims[1] * f->dg_dims[2];
+    if (dg_dims) for (int a = 0; a < 3; a++) dg_dims[a] = f->dg_dims[a];
+    if (empty27) HIPCHK(c, copy_sync(c, empty27, f->empty27.p, f->info.ncells, hipMemcpyDeviceToHost));
+    if (flag_box) HIPCHK(c, copy_sync(c, flag_box, f->dg_box.p, dgn, hipMemcpyDeviceToHost));
+    if (flag_cons) HIPCHK(c, copy_sync(c, flag_cons, f->dg_cons.p, dgn, hipMemcpyDeviceToHost));
+    if (count_box) HIPCHK(c, copy_sync(c, count_box, f->dg_count.p, 4 * dgn, hipMemcpyDeviceToHost));
+    return PM_OK;
+}
+
+int pm_stream_copy_gbps(pm_ctx *c, uint64_t bytes, int iters, float *gbps)
+{
+    if (!c || !gbps || bytes < 16 || iters <= 0) return PM_ERR_ARG;
+    if (c->pending) return fail(c, PM_ERR_STATE, "a march is in flight");
+    HIPCHK(c, hipSetDevice(c->device));
+    bytes &= ~(uint64_t)15;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) { hipFree(a); return fail(c, PM_ERR_HIP, "hipMalloc failed"); }
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipMemsetAsync(a, 0, bytes, c->stream);
+    launch_stream_copy(a, b, bytes, c->stream);   // warm-up
+    hipEventRecord(e0, c->stream);
+    for (int i = 0; i < iters; i++) launch_stream_copy(a, b, bytes, c->stream);
+    hipEventRecord(e1, c->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return fail(c, PM_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+    *gbps = (float)(2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9);
     return PM_OK;
 }
 

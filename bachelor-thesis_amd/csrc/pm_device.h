@@ -26,6 +26,10 @@ struct GridDev {
     float h, inv_h, h2, sig;  // support, 1/h, h*h, 8/(pi h^3)
     double h_d, h2_d, hinv_d, sig_d;
     float bmin[3], bmax[3];   // padded AABB
+    const uint8_t *empty27;   // per cell: 1 if the cell and its 26 neighbours are empty
+    // reference DensityGrid over the padded AABB (Dataset.cpp:94-164)
+    const uint8_t *dg_flag;   // flag per node for the selected skip rule
+    int32_t dg_nx, dg_ny, dg_nz;
 };
 
 // CompactNSearch cell key: int(x / h), minus one for negative x.
@@ -104,6 +108,78 @@ __device__ __forceinline__ float pm_density32(const GridDev &g, float qx, float 
         }
     }
     return compensated ? s + c : s;
+}
+
+// Same sum as pm_density32, with the in-support candidates compacted through a
+// per-lane LDS buffer before W is evaluated.  A wave walks its rows together;
+// the distance test is cheap and the W evaluation (sqrt, branch, compensated
+// add) then runs over each lane's own short list, so lanes whose candidates are
+// mostly out of support no longer hold the wave in the expensive branch.  The
+// buffer is drained in order, so the summation order -- and the result -- is
+// exactly that of pm_density32.
+//   buf: this lane's slot 0; slot j is buf[j * 64] (lane-interleaved: no bank
+//   conflicts whatever slot each lane is at).
+template <int CAP>
+__device__ __forceinline__ void pm_drain(const GridDev &g, const float *buf, int n, float &s, float &c)
+{
+    for (int j = 0; j < n; j++) {
+        float w = pm_W_in(buf[j * 64], g.inv_h, g.sig);
+        pm_neu_add(s, c, w);
+    }
+}
+
+// Distance test of candidate p against q; appends |p - q|^2 to the lane's
+// buffer when it is inside the support.
+__device__ __forceinline__ void pm_test_append(const GridDev &g, float4 p, float qx, float qy, float qz, float *buf,
+                                               int &n)
+{
+    float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+    float r2 = (rx * rx + ry * ry) + rz * rz;
+    if (r2 < g.h2) { buf[n * 64] = r2; n++; }
+}
+
+// UNROLL candidates are loaded before any is tested, so each lane keeps UNROLL
+// independent gathers in flight instead of one dependent load per iteration.
+template <int CAP, int UNROLL>
+__device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, float qy, float qz, float *buf,
+                                                  uint32_t &cand)
+{
+    static_assert(UNROLL >= 1 && UNROLL <= CAP, "unroll");
+    int32_t cx, cy, cz;
+    float s = 0.0f, c = 0.0f;
+    bool in = pm_stencil(g, qx, qy, qz, cx, cy, cz);
+    int n = 0;
+    if (in) {
+        for (int dz = -1; dz <= 1; dz++) {
+            for (int dy = -1; dy <= 1; dy++) {
+                uint32_t base = (uint32_t)(cx - 1) + (uint32_t)g.nx * ((uint32_t)(cy + dy) + (uint32_t)g.ny * (uint32_t)(cz + dz));
+                uint32_t b = g.cell_start[base], e = g.cell_start[base + 3];
+                cand += e - b;
+                if (__ballot(n + (int)(e - b) > CAP) != 0) {   // some lane might overflow:
+                    pm_drain<CAP>(g, buf, n, s, c);           // all drain together
+                    n = 0;
+                }
+                if ((int)(e - b) <= CAP) {
+                    uint32_t i = b;
+                    for (; i + UNROLL <= e; i += UNROLL) {
+                        float4 p[UNROLL];
+#pragma unroll
+                        for (int u = 0; u < UNROLL; u++) p[u] = g.sorted[i + u];
+#pragma unroll
+                        for (int u = 0; u < UNROLL; u++) pm_test_append(g, p[u], qx, qy, qz, buf, n);
+                    }
+                    for (; i < e; i++) pm_test_append(g, g.sorted[i], qx, qy, qz, buf, n);
+                } else {                                // a row longer than the buffer
+                    for (uint32_t i = b; i < e; i++) {
+                        pm_test_append(g, g.sorted[i], qx, qy, qz, buf, n);
+                        if (n == CAP) { pm_drain<CAP>(g, buf, n, s, c); n = 0; }
+                    }
+                }
+            }
+        }
+    }
+    pm_drain<CAP>(g, buf, n, s, c);
+    return s + c;
 }
 
 // FP64 density; the stencil is chosen from the FP32-rounded point.

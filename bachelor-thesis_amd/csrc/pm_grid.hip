@@ -214,4 +214,105 @@ hipError_t launch_grid_build(const GridBuildArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
+// empty27[c] = 1 when cells c-1..c+1 on every axis hold no particle (border
+// cells count missing neighbours as empty).
+__global__ __launch_bounds__(kThreads) void k_empty27(const uint32_t *__restrict__ cs, int32_t nx, int32_t ny, int32_t nz,
+                                                      uint8_t *__restrict__ out)
+{
+    uint32_t c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= (uint32_t)nx * ny * nz) return;
+    int32_t x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+    int32_t x0 = x > 0 ? x - 1 : 0, x1 = x < nx - 1 ? x + 1 : nx - 1;
+    uint32_t cnt = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            int32_t yy = y + dy, zz = z + dz;
+            if (yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
+            uint32_t row = (uint32_t)nx * ((uint32_t)yy + (uint32_t)ny * (uint32_t)zz);
+            cnt += cs[row + x1 + 1] - cs[row + x0];
+        }
+    out[c] = cnt == 0;
+}
+
+hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz, uint8_t *out, hipStream_t st)
+{
+    uint32_t n = (uint32_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_empty27, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, cs, nx, ny, nz, out);
+    return hipGetLastError();
+}
+
+// Reference DensityGrid node flags (Dataset.cpp:94-164).  Node (x,y,z) covers
+// [bmin + (x,y,z) h, +h); its centre is bmin + ((x,y,z) + 0.5) h.  Two flags:
+//   box  : some particle with |x_j - centre|_inf < h and count * W(0) > 1
+//          (one reading of find_neighbors_box, whose source is absent);
+//   cons : some particle with |x_j - centre|_inf < 1.5 h (1.001 margin), i.e.
+//          a cleared node has zero density everywhere inside it.
+__global__ __launch_bounds__(kThreads) void k_densitygrid(GridDev g, int32_t dnx, int32_t dny, int32_t dnz,
+                                                          uint8_t *__restrict__ flag_box, uint8_t *__restrict__ flag_cons,
+                                                          uint32_t *__restrict__ count_box)
+{
+    uint32_t id = blockIdx.x * kThreads + threadIdx.x;
+    if (id >= (uint32_t)dnx * dny * dnz) return;
+    int32_t k[3] = { (int32_t)(id % dnx), (int32_t)((id / dnx) % dny), (int32_t)(id / (dnx * dny)) };
+    float c[3];
+    const float cw = 1.0f * g.h;
+    for (int a = 0; a < 3; a++) c[a] = g.bmin[a] + ((float)k[a] + 0.5f) * cw;
+    const int32_t org[3] = { g.ox, g.oy, g.oz }, dim[3] = { g.nx, g.ny, g.nz };
+    uint32_t res[2] = { 0u, 0u };
+    const float R[2] = { g.h, 1.5f * g.h * 1.001f };
+    for (int w = 0; w < 2; w++) {
+        int32_t lo[3], hi[3];
+        bool empty = false;
+        for (int a = 0; a < 3; a++) {
+            lo[a] = pm_key(c[a] - R[w], g.inv_h) - org[a];
+            hi[a] = pm_key(c[a] + R[w], g.inv_h) - org[a];
+            if (lo[a] < 0) lo[a] = 0;
+            if (hi[a] > dim[a] - 1) hi[a] = dim[a] - 1;
+            if (lo[a] > hi[a]) empty = true;
+        }
+        if (empty) continue;
+        uint32_t cnt = 0;
+        for (int32_t z = lo[2]; z <= hi[2]; z++)
+            for (int32_t y = lo[1]; y <= hi[1]; y++) {
+                uint32_t row = (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+                uint32_t b = g.cell_start[row + lo[0]], e = g.cell_start[row + hi[0] + 1];
+                for (uint32_t i = b; i < e; i++) {
+                    float4 p = g.sorted[i];
+                    if (fabsf(p.x - c[0]) < R[w] && fabsf(p.y - c[1]) < R[w] && fabsf(p.z - c[2]) < R[w]) cnt++;
+                }
+            }
+        res[w] = cnt;
+    }
+    count_box[id] = res[0];
+    flag_box[id] = (float)res[0] * g.sig > 1.0f;
+    flag_cons[id] = res[1] > 0;
+}
+
+hipError_t launch_densitygrid(const GridDev &g, int32_t dnx, int32_t dny, int32_t dnz, uint8_t *flag_box, uint8_t *flag_cons,
+                              uint32_t *count_box, hipStream_t st)
+{
+    uint32_t n = (uint32_t)dnx * dny * dnz;
+    hipLaunchKernelGGL(k_densitygrid, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, g, dnx, dny, dnz, flag_box,
+                       flag_cons, count_box);
+    return hipGetLastError();
+}
+
+// Streaming copy (read n float4, write n float4): the measured-bandwidth
+// reference point for the roofline and for scaling HBM byte counters.
+__global__ __launch_bounds__(kThreads) void k_stream_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                          uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads)
+        dst[i] = src[i];
+}
+
+hipError_t launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st)
+{
+    uint64_t n = bytes / 16;
+    uint64_t blocks = (n + kThreads - 1) / kThreads;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_stream_copy, dim3((uint32_t)blocks), dim3(kThreads), 0, st, (const float4 *)src, (float4 *)dst, n);
+    return hipGetLastError();
+}
+
 }  // namespace pm

@@ -5,12 +5,19 @@
 #include <stdint.h>
 
 #include "pm_device.h"
+#include "pm_mlp.h"
 #include "pmarch.h"
 
 namespace pm {
 
 // Per-launch counters (u64 each), accumulated by the march kernel.
 enum Counter { C_RAYS = 0, C_IN_AABB, C_HITS, C_SAMPLES, C_CAND, C_MARCH_SAMPLES, C_MLP, C_COUNT };
+
+// Coarse occupancy grid of the step-proposal features: 4x4x4 dense cells.
+struct CoarseDev {
+    const uint32_t *count;
+    int32_t nx, ny, nz;
+};
 
 struct MarchLaunch {
     GridDev g;
@@ -23,7 +30,42 @@ struct MarchLaunch {
     float4 *nrm;
     float *t;             // may be null
     unsigned long long *counters;
+    uint32_t *band_ctr;        // 8 per-XCD tile counters (persistent schedule)
+    uint32_t persist_blocks;   // workgroups of the persistent schedule, 0 = default
+    // learned step proposal
+    CoarseDev cg;
+    MlpDev net;
+    float mlp_alpha;           // fraction of the predicted distance that is skipped
+    int32_t mlp_min_skip;      // below this many steps the ray continues step by step
+    int32_t mlp_max_skip;      // longest jump, in steps
 };
+
+// Batched launch: many (view, frame) jobs in one persistent kernel.
+struct BatchJob {
+    GridDev g;
+    CoarseDev cg;
+    float M[16];
+    float cam[3];
+    uint32_t W, H, tiles_x;
+    float4 *pos;
+    float4 *nrm;
+    float *t;
+};
+// A run of consecutive 8x8 tiles of one job; waves take chunks, so the job's
+// parameters stay uniform across a wave.
+struct BatchChunk {
+    uint32_t job, tile0, ntiles, pad;
+};
+struct BatchLaunch {
+    MarchLaunch base;          // settings, counters and network; g/M/cam/W/H/outputs come from the job
+    const BatchJob *jobs;
+    const BatchChunk *chunks;
+    uint32_t nchunks;
+    uint32_t *chunk_ctr;
+};
+hipError_t launch_march_batch(const BatchLaunch &B, uint32_t blocks, hipStream_t st);
+hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
+hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st);
 
 // Grid build: bounds of n float3 points (device) into out[6] (device).
 hipError_t launch_bounds_reduce(const float *xyz, uint64_t n, float *partial, uint32_t npartial,
@@ -52,6 +94,53 @@ hipError_t launch_density(const GridDev &g, const float *q, uint32_t nq, int com
 hipError_t launch_density64(const GridDev &g, const double *q, uint32_t nq, double *rho, hipStream_t st);
 hipError_t launch_neighbor_counts(const GridDev &g, const float *q, uint32_t nq, uint32_t *counts,
                                   uint32_t *ncand, hipStream_t st);
+
+hipError_t launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
+hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz, uint8_t *out, hipStream_t st);
+hipError_t launch_densitygrid(const GridDev &g, int32_t dnx, int32_t dny, int32_t dnz, uint8_t *flag_box, uint8_t *flag_cons,
+                              uint32_t *count_box, hipStream_t st);
+
+hipError_t launch_coarse_counts(const GridDev &g, const CoarseDev &cg, uint32_t *out, hipStream_t st);
+
+struct TraceArgs {
+    GridDev g;
+    CoarseDev cg;
+    float M[16];
+    float cam[3];
+    uint32_t W, H;
+    float step, iso;
+    const float *t_hit;       // plain-march t per pixel (-1 = miss)
+    uint32_t ray_stride, k_stride, seed;
+    uint32_t *count;          // rows written so far (device)
+    uint32_t cap;
+    float *feat;              // [cap][24]
+    float *label;             // [cap]
+    uint8_t *is_hit;          // [cap]
+    uint32_t *meta;           // [cap][2]: pixel index, sample index k
+};
+hipError_t launch_trace(const TraceArgs &a, hipStream_t st);
+
+struct TrainArgs {
+    MlpDev net;
+    const float *feat;
+    const float *label;
+    const uint32_t *idx;      // batch sample ids (NULL: 0..B-1)
+    float inv_batch, w_over;
+    float *partial;           // [nwaves][NPARAM + 1]
+};
+struct AdamArgs {
+    const float *partial;
+    uint32_t nparts;
+    float *p, *m, *v;
+    float lr, b1, b2, eps, bc1, bc2;
+    int apply;
+    float *grad_out;          // optional summed gradient
+    float *loss_out;
+};
+hipError_t launch_mlp_train(const TrainArgs &a, uint32_t nwaves, hipStream_t st);
+hipError_t launch_mlp_adam(const AdamArgs &a, hipStream_t st);
+hipError_t launch_mlp_sync(const float *p, uint16_t *W1, uint16_t *W2, uint16_t *W2T, hipStream_t st);
+hipError_t launch_mlp_forward(const MlpDev &net, const float *feat, uint32_t n, float *y, hipStream_t st);
 
 // Host restatement of the CompactNSearch key, for the grid extent.
 inline int32_t host_key(float x, float inv_h)

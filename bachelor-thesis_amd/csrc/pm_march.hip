@@ -13,11 +13,18 @@
 // evaluation per active lane, whatever its phase, so the expensive 27-cell
 // candidate loop stays converged across the wave while rays hit at different
 // steps.
+#include "pm_features.h"
 #include "pm_internal.h"
 
 namespace pm {
 
-enum Phase : int { PH_MARCH = 0, PH_BISECT = 1, PH_BISECT64 = 2, PH_NORMAL = 3, PH_GRADW = 4, PH_DONE = 5 };
+// PH_PROPOSE: waiting for a step proposal (taken by the whole wave at once);
+// PH_LAND: density at the proposed sample; PH_BSEARCH: binary search for the
+// first sample at or above iso between the last sample below iso and the landing.
+enum Phase : int {
+    PH_MARCH = 0, PH_BISECT = 1, PH_BISECT64 = 2, PH_NORMAL = 3, PH_GRADW = 4, PH_DONE = 5,
+    PH_PROPOSE = 6, PH_LAND = 7, PH_BSEARCH = 8
+};
 
 // glm mat4 * vec4 (type_mat4x4.inl): (m0 x + m1 y) + (m2 z + m3 w), column-major M.
 __device__ __forceinline__ void pm_unproject(const float *M, float cx, float cy, float cz, float &px, float &py,
@@ -140,169 +147,484 @@ __device__ __forceinline__ bool pm_tile(uint32_t W, uint32_t H, uint32_t &px, ui
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_march_config(MarchLaunch L)
+// Per-ray state of the configured march, carried across loop iterations.
+struct RayState {
+    float d0, d1, d2;     // direction (origin = camera)
+    float tn, tf;         // AABB entry / exit
+    float lo, hi, th;     // bisection bracket, final t
+    float xs0, xs1, xs2;  // hit point
+    float gv0, gv1, gv2, rplus;
+    int k, it, ncd, phase;
+    uint32_t idx;
+    int klo, khi;         // learned step: last sample below iso / landing sample
+    int lrn;              // 1 while the ray still takes proposals
+};
+
+struct Counts {
+    uint32_t rays = 0, in_aabb = 0, hits = 0, samples = 0, march = 0, mlp = 0;
+    unsigned long long cand = 0;
+};
+
+__device__ __forceinline__ void write_miss(const MarchLaunch &L, uint32_t idx)
 {
+    L.pos[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    L.nrm[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (L.t) L.t[idx] = -1.0f;
+}
+
+// Ray of pixel (px, py) (RayMarcher.cpp:268-274, with clip z = 1) and its AABB
+// interval.  Leaves phase = PH_MARCH, or writes the miss and PH_DONE.
+__device__ __forceinline__ void ray_setup(const MarchLaunch &L, RayState &r, uint32_t px, uint32_t py, Counts &cn)
+{
+    cn.rays++;
+    r.idx = py * L.W + px;
+    float cx = (float)px * (2.0f / (float)L.W) - 1.0f;
+    float cy = (float)py * (2.0f / (float)L.H) - 1.0f;
+    float wx, wy, wz;
+    pm_unproject(L.M, cx, cy, 1.0f, wx, wy, wz);
+    float d[3];
+    pm_normalize(wx - L.cam[0], wy - L.cam[1], wz - L.cam[2], d[0], d[1], d[2]);
+    r.d0 = d[0]; r.d1 = d[1]; r.d2 = d[2];
+    float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
+    if (pm_slab(o, d, L.g.bmin, L.g.bmax, r.tn, r.tf)) {
+        r.phase = PH_MARCH;
+        r.k = 0;
+        r.lrn = L.s.learned;
+        if (r.lrn) {            // first proposal at the entry; klo = -1 stands for "before the entry"
+            r.klo = -1;
+            r.phase = PH_PROPOSE;
+        }
+        cn.in_aabb++;
+    } else {
+        write_miss(L, r.idx);
+        r.phase = PH_DONE;
+    }
+}
+
+// Empty-space skip: when the sample's cell has an empty 27-neighbourhood,
+// every sample the ray places in that cell has zero density, so the march
+// resumes at the last sample before the ray leaves the cell box
+// [key h, (key+1) h].  The first-crossing sample -- and the image -- are those
+// of the march without the skip.  Returns the next sample index.
+__device__ __forceinline__ float pm_cell_exit(int32_t key, float h, float o, float d, float texit)
+{
+    if (d == 0.0f) return texit;
+    float lo = (float)key * h, hi = (float)(key + 1) * h;
+    float ta = ((d > 0.0f ? hi : lo) - o) / d;
+    return ta < texit ? ta : texit;
+}
+
+__device__ __forceinline__ int pm_skip_empty(const GridDev &g, float o0, float o1, float o2, const RayState &r, float qx,
+                                             float qy, float qz, float step)
+{
+    int32_t cx, cy, cz;
+    if (!pm_stencil(g, qx, qy, qz, cx, cy, cz)) return r.k;
+    uint32_t lin = (uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz);
+    if (!g.empty27[lin]) return r.k;
+    float texit = INFINITY;
+    texit = pm_cell_exit(cx + g.ox, g.h, o0, r.d0, texit);
+    texit = pm_cell_exit(cy + g.oy, g.h, o1, r.d1, texit);
+    texit = pm_cell_exit(cz + g.oz, g.h, o2, r.d2, texit);
+    float kf = floorf((texit - r.tn) / step);
+    return kf > (float)r.k ? (int)kf : r.k + 1;
+}
+
+__device__ __forceinline__ void to_normals(const pm_settings &s, RayState &r)
+{
+    r.phase = s.normal_mode == PM_NORMAL_GRADW ? PH_GRADW : PH_NORMAL;
+    r.ncd = 0;
+}
+
+__device__ __forceinline__ void set_hit_point(const MarchLaunch &L, RayState &r, float t)
+{
+    r.th = t;
+    r.xs0 = L.cam[0] + t * r.d0; r.xs1 = L.cam[1] + t * r.d1; r.xs2 = L.cam[2] + t * r.d2;
+}
+
+// The first sample k at or above iso is known: bracket [t_{k-1}, t_k] and
+// start the bisection (or take t_0 when the ray starts inside the surface).
+__device__ __forceinline__ void start_bracket(const MarchLaunch &L, RayState &r, int n32)
+{
+    const pm_settings &s = L.s;
+    if (r.k == 0) {
+        set_hit_point(L, r, r.tn);
+        to_normals(s, r);
+        return;
+    }
+    r.lo = r.tn + (float)(r.k - 1) * s.step;
+    r.hi = r.tn + (float)r.k * s.step;
+    r.it = 0;
+    if (n32 > 0) r.phase = PH_BISECT;
+    else if (s.final_fp64 && s.bisect_iters > 0) r.phase = PH_BISECT64;
+    else { set_hit_point(L, r, (r.lo + r.hi) * 0.5f); to_normals(s, r); }
+}
+
+// One density evaluation of this ray, whatever its phase, and the transition
+// it implies.  Dens(q, cand) is the FP32 density.
+template <class Dens>
+__device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens dens, Counts &cn)
+{
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
+    const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
+    const int n32 = (s.final_fp64 && s.bisect_iters > 0) ? s.bisect_iters - 1 : s.bisect_iters;
+
+    if (r.phase == PH_BISECT64) {
+        double tmd = ((double)r.lo + (double)r.hi) * 0.5;
+        double x = (double)o0 + tmd * (double)r.d0;
+        double y = (double)o1 + tmd * (double)r.d1;
+        double z = (double)o2 + tmd * (double)r.d2;
+        double rho = pm_density64(g, x, y, z);
+        cn.samples++;
+        double dlo = r.lo, dhi = r.hi;
+        if (rho >= (double)s.iso) dhi = tmd; else dlo = tmd;
+        double tt = (dlo + dhi) * 0.5;
+        r.xs0 = (float)((double)o0 + tt * (double)r.d0);
+        r.xs1 = (float)((double)o1 + tt * (double)r.d1);
+        r.xs2 = (float)((double)o2 + tt * (double)r.d2);
+        r.th = (float)tt;
+        to_normals(s, r);
+        return;
+    }
+    if (r.phase == PH_GRADW) {
+        float sx, sy, sz, nx, ny, nz;
+        uint32_t cand = 0;
+        pm_sum_gradw(g, r.xs0, r.xs1, r.xs2, sx, sy, sz, cand);
+        pm_normalize(sx, sy, sz, nx, ny, nz);
+        L.pos[r.idx] = make_float4(r.xs0, r.xs1, r.xs2, 1.0f);
+        L.nrm[r.idx] = make_float4(nx, ny, nz, 1.0f);
+        if (L.t) L.t[r.idx] = r.th;
+        cn.hits++;
+        r.phase = PH_DONE;
+        return;
+    }
+
+    float qx, qy, qz;
+    if (r.phase == PH_LAND || r.phase == PH_BSEARCH) {
+        int kq = r.phase == PH_LAND ? r.khi : (r.klo + r.khi) / 2;
+        float t = r.tn + (float)kq * s.step;
+        qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
+    } else if (r.phase == PH_MARCH) {
+        for (;;) {
+            float t = r.tn + (float)r.k * s.step;
+            if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) {
+                write_miss(L, r.idx);
+                r.phase = PH_DONE;
+                return;
+            }
+            qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
+            if (s.skip_mode != PM_SKIP_EMPTY) break;
+            int kn = pm_skip_empty(g, o0, o1, o2, r, qx, qy, qz, s.step);
+            if (kn == r.k) break;
+            r.k = kn;
+        }
+    } else if (r.phase == PH_BISECT) {
+        float tm = (r.lo + r.hi) * 0.5f;
+        qx = o0 + tm * r.d0; qy = o1 + tm * r.d1; qz = o2 + tm * r.d2;
+    } else {  // PH_NORMAL: +x, -x, +y, -y, +z, -z
+        qx = r.xs0; qy = r.xs1; qz = r.xs2;
+        int a = r.ncd >> 1;
+        float e = s.normal_eps;
+        if (a == 0) qx = (r.ncd & 1) ? r.xs0 - e : r.xs0 + e;
+        else if (a == 1) qy = (r.ncd & 1) ? r.xs1 - e : r.xs1 + e;
+        else qz = (r.ncd & 1) ? r.xs2 - e : r.xs2 + e;
+    }
+
+    uint32_t cand = 0;
+    float rho = dens(qx, qy, qz, cand);
+    cn.samples++;
+    cn.cand += cand;
+
+    if (r.phase == PH_MARCH) {
+        cn.march++;
+        if (rho >= s.iso) {
+            start_bracket(L, r, n32);
+        } else if (r.lrn) {
+            r.klo = r.k;
+            r.phase = PH_PROPOSE;
+        } else {
+            r.k++;
+        }
+    } else if (r.phase == PH_LAND) {
+        cn.march++;
+        if (rho >= s.iso) {
+            if (r.khi - r.klo == 1) { r.k = r.khi; start_bracket(L, r, n32); }
+            else r.phase = PH_BSEARCH;
+        } else {
+            r.klo = r.k = r.khi;
+            r.phase = PH_PROPOSE;
+        }
+    } else if (r.phase == PH_BSEARCH) {
+        cn.march++;
+        int mid = (r.klo + r.khi) / 2;
+        if (rho >= s.iso) r.khi = mid; else r.klo = mid;
+        if (r.khi - r.klo == 1) { r.k = r.khi; start_bracket(L, r, n32); }
+    } else if (r.phase == PH_BISECT) {
+        float tm = (r.lo + r.hi) * 0.5f;
+        if (rho >= s.iso) r.hi = tm; else r.lo = tm;
+        r.it++;
+        if (r.it >= n32) {
+            if (s.final_fp64 && s.bisect_iters > 0) r.phase = PH_BISECT64;
+            else { set_hit_point(L, r, (r.lo + r.hi) * 0.5f); to_normals(s, r); }
+        }
+    } else {  // PH_NORMAL
+        if ((r.ncd & 1) == 0) {
+            r.rplus = rho;
+        } else {
+            float gg = r.rplus - rho;
+            int a = r.ncd >> 1;
+            if (a == 0) r.gv0 = gg; else if (a == 1) r.gv1 = gg; else r.gv2 = gg;
+        }
+        r.ncd++;
+        if (r.ncd == 6) {
+            float dd = (r.gv0 * r.gv0 + r.gv1 * r.gv1) + r.gv2 * r.gv2;
+            float nx = 0.f, ny = 0.f, nz = 0.f;
+            if (dd > 0.0f) {
+                float inv = 1.0f / sqrtf(dd);
+                nx = -r.gv0 * inv; ny = -r.gv1 * inv; nz = -r.gv2 * inv;
+            }
+            L.pos[r.idx] = make_float4(r.xs0, r.xs1, r.xs2, 1.0f);
+            L.nrm[r.idx] = make_float4(nx, ny, nz, 1.0f);
+            if (L.t) L.t[r.idx] = r.th;
+            cn.hits++;
+            r.phase = PH_DONE;
+        }
+    }
+}
+
+struct DensPlain {
+    const GridDev *g;
+    bool comp;
+    __device__ float operator()(float x, float y, float z, uint32_t &cand) const
+    {
+        return pm_density32(*g, x, y, z, comp, cand);
+    }
+};
+
+template <int CAP, int UNROLL>
+struct DensLds {
+    const GridDev *g;
+    float *buf;
+    __device__ float operator()(float x, float y, float z, uint32_t &cand) const
+    {
+        return pm_density32_lds<CAP, UNROLL>(*g, x, y, z, buf, cand);
+    }
+};
+
+static constexpr int kLdsCap = 32;
+
+// Schedule "tile": one 8x8 tile per wave, one ray per lane, all lanes in
+// lockstep until the last ray of the tile is done.
+template <bool LDS, int UNROLL>
+__global__ __launch_bounds__(256) void k_march_tile(MarchLaunch L)
+{
+    __shared__ float lds_r2[LDS ? 4 * kLdsCap * 64 : 1];
     uint32_t px, py;
     if (!pm_tile(L.W, L.H, px, py)) return;   // wave-uniform
-    const GridDev &g = L.g;
-    const pm_settings &s = L.s;
-    const bool active = px < L.W && py < L.H;
-    const uint32_t idx = py * L.W + px;
-    const bool comp = s.compensated != 0;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Counts cn;
+    RayState r{};
+    r.phase = PH_DONE;
+    if (px < L.W && py < L.H) ray_setup(L, r, px, py, cn);
+    if (LDS && L.s.compensated) {
+        DensLds<kLdsCap, UNROLL> dens{ &L.g, lds_r2 + wave * kLdsCap * 64 + lane };
+        while (r.phase != PH_DONE) ray_step(L, r, dens, cn);
+    } else {
+        DensPlain dens{ &L.g, L.s.compensated != 0 };
+        while (r.phase != PH_DONE) ray_step(L, r, dens, cn);
+    }
+    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+}
 
-    uint32_t c_rays = 0, c_in = 0, c_hits = 0, c_samples = 0, c_ms = 0;
-    unsigned long long c_cand = 0;
-
-    float o[3] = { L.cam[0], L.cam[1], L.cam[2] }, d[3] = { 0.f, 0.f, 0.f };
-    float tn = 0.0f, tf = 0.0f;
-    int phase = PH_DONE;
-    if (active) {
-        c_rays = 1;
-        float cx = (float)px * (2.0f / (float)L.W) - 1.0f;     // RayMarcher.cpp:268-270
-        float cy = (float)py * (2.0f / (float)L.H) - 1.0f;
-        float wx, wy, wz;
-        pm_unproject(L.M, cx, cy, 1.0f, wx, wy, wz);
-        pm_normalize(wx - o[0], wy - o[1], wz - o[2], d[0], d[1], d[2]);
-        if (pm_slab(o, d, g.bmin, g.bmax, tn, tf)) {
-            phase = PH_MARCH;
-            c_in = 1;
+// Schedule 0: persistent waves.  The image is cut into 8x8 tiles and the tiles
+// into 8 contiguous bands, one per XCD (workgroups are dispatched round-robin
+// over the XCDs, so blockIdx % 8 names the XCD); a wave takes tiles from its
+// band's counter and, once the band is drained, from the other bands.  A lane
+// whose ray is finished takes the next ray of the wave's current tile before
+// the next density evaluation, so all 64 lanes stay busy until the image runs
+// out.  Rays that miss the AABB are retired during refill.  Every wave exits
+// once no tile is left and its lanes are idle.
+// A step proposal for every lane in PH_PROPOSE: features at the lane's current
+// sample (one stencil pass for rho and grad rho), one MLP evaluation for the
+// whole wave (BF16 MFMA), and the jump to the proposed sample on the same
+// sample lattice t_k = t_entry + k step.  Called by all 64 lanes together.
+// Kept out of line: the feature pass and the network need registers the
+// density loop should not be charged for.
+__device__ __noinline__ void wave_propose(const MarchLaunch &L, RayState &r, Counts &cn, uint16_t *X, uint16_t *H,
+                                          float *Y)
+{
+    const int l = threadIdx.x & 63;
+    const bool want = r.phase == PH_PROPOSE;
+    float f[PM_MLP_IN];
+#pragma unroll
+    for (int j = 0; j < PM_MLP_IN; j++) f[j] = 0.0f;
+    if (want) {
+        float t = r.tn + (float)r.k * L.s.step;
+        float q0 = L.cam[0] + t * r.d0, q1 = L.cam[1] + t * r.d1, q2 = L.cam[2] + t * r.d2;
+        float rho, gx, gy, gz;
+        uint32_t cand = 0;
+        density_grad_feat(L.g, q0, q1, q2, rho, gx, gy, gz, cand);
+        cn.samples++;
+        cn.cand += cand;
+        const float d[3] = { r.d0, r.d1, r.d2 };
+        mlp_features(L.g, L.cg, L.cam, d, t, r.tn, r.tf, rho, sqrtf((gx * gx + gy * gy) + gz * gz), (float)cand,
+                     L.s.iso, f);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < PM_MLP_K1; j++) X[l * PM_MLP_K1 + j] = j < PM_MLP_IN ? f2bf(f[j]) : (uint16_t)0;
+    wave_lds_sync();
+    float y = mlp_wave_forward_halves(L.net, X, H, Y);
+    wave_lds_sync();
+    if (want) {
+        cn.mlp++;
+        float dist = L.g.h * (exp2f(fminf(y, 8.0f)) - 1.0f);
+        float sk = fminf(floorf(L.mlp_alpha * dist / L.s.step), (float)L.mlp_max_skip);
+        int kmax = (int)floorf((r.tf - r.tn) / L.s.step);
+        if (kmax > 0 && r.tn + (float)kmax * L.s.step > r.tf) kmax--;
+        if (!(sk >= (float)L.mlp_min_skip) || r.k + L.mlp_min_skip > kmax) {
+            r.lrn = 0;                       // continue step by step from the first unevaluated sample
+            r.k = r.klo + 1;
+            r.phase = PH_MARCH;
         } else {
-            L.pos[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-            L.nrm[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (L.t) L.t[idx] = -1.0f;
+            int kt = r.k + (int)fminf(sk, 1.0e6f);
+            r.khi = kt > kmax ? kmax : kt;
+            r.phase = PH_LAND;
         }
     }
+}
 
-    const int n32 = (s.final_fp64 && s.bisect_iters > 0) ? s.bisect_iters - 1 : s.bisect_iters;
-    int k = 0, it = 0, ncd = 0;
-    float lo = 0.f, hi = 0.f, th = 0.f;
-    double tmd = 0.0;
-    float xs0 = 0.f, xs1 = 0.f, xs2 = 0.f;
-    float gv0 = 0.f, gv1 = 0.f, gv2 = 0.f, rplus = 0.f;
+// Schedule 0: persistent waves.  The image is cut into 8x8 tiles and the tiles
+// into 8 contiguous bands, one per XCD (workgroups are dispatched round-robin
+// over the XCDs, so blockIdx % 8 names the XCD); a wave takes tiles from its
+// band's counter and, once the band is drained, from the other bands.  A lane
+// whose ray is finished takes the next ray of the wave's current tile before
+// the next density evaluation, so all 64 lanes stay busy until the image runs
+// out.  Rays that miss the AABB are retired during refill.  Every wave exits
+// once no tile is left and its lanes are idle.
+// LEARNED adds the step proposals; its LDS region per wave holds either the
+// density compaction buffer or the network's activations (never both at once).
+template <int UNROLL, bool LEARNED>
+__global__ __launch_bounds__(256) void k_march_persist(MarchLaunch L, uint32_t *__restrict__ band_ctr)
+{
+    // floats per wave: the compaction buffer, or X[64][32] + H[32][64] bf16 + Y[64]
+    constexpr int kRegion = LEARNED ? 2112 : kLdsCap * 64;
+    __shared__ float lds[4 * kRegion];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float *region = lds + wave * kRegion;
+    float *buf = region + lane;
+    const uint32_t tiles_x = (L.W + 7) / 8, tiles_y = (L.H + 7) / 8, T = tiles_x * tiles_y;
+    const uint32_t xcd = blockIdx.x & 7u;
+    const bool use_lds = L.s.compensated != 0;
 
-    while (phase != PH_DONE) {
-        float qx = 0.f, qy = 0.f, qz = 0.f;
-        if (phase == PH_MARCH) {
-            float t = tn + (float)k * s.step;
-            if (t > tf || (s.max_steps > 0 && k >= s.max_steps)) {
-                L.pos[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-                L.nrm[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (L.t) L.t[idx] = -1.0f;
-                phase = PH_DONE;
-                continue;
-            }
-            qx = o[0] + t * d[0]; qy = o[1] + t * d[1]; qz = o[2] + t * d[2];
-        } else if (phase == PH_BISECT) {
-            float tm = (lo + hi) * 0.5f;
-            qx = o[0] + tm * d[0]; qy = o[1] + tm * d[1]; qz = o[2] + tm * d[2];
-        } else if (phase == PH_NORMAL) {
-            qx = xs0; qy = xs1; qz = xs2;
-            int a = ncd >> 1;
-            float e = s.normal_eps;
-            if (a == 0) qx = (ncd & 1) ? xs0 - e : xs0 + e;
-            else if (a == 1) qy = (ncd & 1) ? xs1 - e : xs1 + e;
-            else qz = (ncd & 1) ? xs2 - e : xs2 + e;
-        }
+    Counts cn;
+    RayState r{};
+    r.phase = PH_DONE;
+    uint32_t tile = 0, steal = 0;
+    int qpos = 64;
+    bool exhausted = false;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
 
-        if (phase == PH_BISECT64) {
-            tmd = ((double)lo + (double)hi) * 0.5;
-            double x = (double)o[0] + tmd * (double)d[0];
-            double y = (double)o[1] + tmd * (double)d[1];
-            double z = (double)o[2] + tmd * (double)d[2];
-            double rho = pm_density64(g, x, y, z);
-            c_samples++;
-            double dlo = lo, dhi = hi;
-            if (rho >= (double)s.iso) dhi = tmd; else dlo = tmd;
-            double tt = (dlo + dhi) * 0.5;
-            xs0 = (float)((double)o[0] + tt * (double)d[0]);
-            xs1 = (float)((double)o[1] + tt * (double)d[1]);
-            xs2 = (float)((double)o[2] + tt * (double)d[2]);
-            th = (float)tt;
-            phase = s.normal_mode == PM_NORMAL_GRADW ? PH_GRADW : PH_NORMAL;
-            ncd = 0;
-            continue;
-        }
-        if (phase == PH_GRADW) {
-            float sx, sy, sz;
-            uint32_t cand = 0;
-            pm_sum_gradw(g, xs0, xs1, xs2, sx, sy, sz, cand);
-            float nx, ny, nz;
-            pm_normalize(sx, sy, sz, nx, ny, nz);
-            L.pos[idx] = make_float4(xs0, xs1, xs2, 1.0f);
-            L.nrm[idx] = make_float4(nx, ny, nz, 1.0f);
-            if (L.t) L.t[idx] = th;
-            c_hits++;
-            phase = PH_DONE;
-            continue;
-        }
-
-        uint32_t cand = 0;
-        float rho = pm_density32(g, qx, qy, qz, comp, cand);
-        c_samples++;
-        c_cand += cand;
-
-        if (phase == PH_MARCH) {
-            c_ms++;
-            if (rho >= s.iso) {
-                if (k == 0) {
-                    th = tn;
-                    xs0 = o[0] + th * d[0]; xs1 = o[1] + th * d[1]; xs2 = o[2] + th * d[2];
-                    phase = s.normal_mode == PM_NORMAL_GRADW ? PH_GRADW : PH_NORMAL;
-                    ncd = 0;
-                } else {
-                    lo = tn + (float)(k - 1) * s.step;
-                    hi = tn + (float)k * s.step;
-                    it = 0;
-                    if (n32 > 0) phase = PH_BISECT;
-                    else if (s.final_fp64 && s.bisect_iters > 0) phase = PH_BISECT64;
-                    else {
-                        th = (lo + hi) * 0.5f;
-                        xs0 = o[0] + th * d[0]; xs1 = o[1] + th * d[1]; xs2 = o[2] + th * d[2];
-                        phase = s.normal_mode == PM_NORMAL_GRADW ? PH_GRADW : PH_NORMAL;
-                        ncd = 0;
+    while (true) {
+        // refill idle lanes from the wave's tile queue
+        while (!exhausted) {
+            unsigned long long needy = __ballot(r.phase == PH_DONE);
+            if (needy == 0) break;
+            if (qpos >= 64) {
+                uint32_t t = 0xffffffffu;
+                if (lane == 0) {
+                    while (steal < 8) {
+                        uint32_t b = (xcd + steal) & 7u;
+                        uint32_t lo = (uint32_t)(((uint64_t)T * b) >> 3), hi = (uint32_t)(((uint64_t)T * (b + 1)) >> 3);
+                        uint32_t v = atomicAdd(&band_ctr[b], 1u);
+                        if (v < hi - lo) { t = lo + v; break; }
+                        steal++;
                     }
                 }
-            } else {
-                k++;
+                t = __shfl(t, 0, 64);
+                steal = __shfl(steal, 0, 64);
+                if (t == 0xffffffffu) { exhausted = true; break; }
+                tile = t;
+                qpos = 0;
             }
-        } else if (phase == PH_BISECT) {
-            float tm = (lo + hi) * 0.5f;
-            if (rho >= s.iso) hi = tm; else lo = tm;
-            it++;
-            if (it >= n32) {
-                if (s.final_fp64 && s.bisect_iters > 0) phase = PH_BISECT64;
-                else {
-                    th = (lo + hi) * 0.5f;
-                    xs0 = o[0] + th * d[0]; xs1 = o[1] + th * d[1]; xs2 = o[2] + th * d[2];
-                    phase = s.normal_mode == PM_NORMAL_GRADW ? PH_GRADW : PH_NORMAL;
-                    ncd = 0;
-                }
+            int rank = __popcll(needy & lt_mask);
+            int take = __popcll(needy);
+            if (r.phase == PH_DONE && qpos + rank < 64) {
+                uint32_t j = (uint32_t)(qpos + rank);
+                uint32_t px = (tile % tiles_x) * 8 + (j & 7u), py = (tile / tiles_x) * 8 + (j >> 3);
+                if (px < L.W && py < L.H) ray_setup(L, r, px, py, cn);
             }
-        } else {  // PH_NORMAL
-            if ((ncd & 1) == 0) {
-                rplus = rho;
-            } else {
-                float gg = rplus - rho;
-                int a = ncd >> 1;
-                if (a == 0) gv0 = gg; else if (a == 1) gv1 = gg; else gv2 = gg;
-            }
-            ncd++;
-            if (ncd == 6) {
-                float dd = (gv0 * gv0 + gv1 * gv1) + gv2 * gv2;
-                float nx = 0.f, ny = 0.f, nz = 0.f;
-                if (dd > 0.0f) {
-                    float inv = 1.0f / sqrtf(dd);
-                    nx = -gv0 * inv; ny = -gv1 * inv; nz = -gv2 * inv;
-                }
-                L.pos[idx] = make_float4(xs0, xs1, xs2, 1.0f);
-                L.nrm[idx] = make_float4(nx, ny, nz, 1.0f);
-                if (L.t) L.t[idx] = th;
-                c_hits++;
-                phase = PH_DONE;
-            }
+            qpos = qpos + take < 64 ? qpos + take : 64;
+        }
+        if (__ballot(r.phase != PH_DONE) == 0) break;
+        if (LEARNED) {
+            // Proposals are batched: the network runs once at least half of the busy
+            // lanes wait for one (or all of them do); meanwhile waiting lanes sit out.
+            const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
+            const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
+            if (nwant > 0 && (2 * nwant >= nbusy || exhausted))
+                wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+        }
+        if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
+            if (use_lds) ray_step(L, r, DensLds<kLdsCap, UNROLL>{ &L.g, buf }, cn);
+            else ray_step(L, r, DensPlain{ &L.g, false }, cn);
         }
     }
-    flush_counters(L.counters, c_rays, c_in, c_hits, c_samples, c_cand, c_ms);
+    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+    if (LEARNED) {
+        unsigned long long m = wave_sum(cn.mlp);
+        if (lane == 0) atomicAdd(&L.counters[C_MLP], m);
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// QueryDensityGrid (Dataset.cpp:26-47) followed by the Flag test: true when p
+// lies in a DensityGrid node whose flag is clear (the node may be skipped), with
+// the node's box in nmin/nmax.
+__device__ __forceinline__ bool pm_dg_query(const GridDev &g, float x, float y, float z, float *nmin, float *nmax)
+{
+    const float p[3] = { x, y, z };
+    const int32_t dims[3] = { g.dg_nx, g.dg_ny, g.dg_nz };
+    const float inv = 1.0f / g.h;
+    int32_t k[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        k[a] = (int32_t)floorf((p[a] - g.bmin[a]) * inv);
+        if (!(k[a] >= 0 && k[a] < dims[a])) return false;
+    }
+    uint32_t id = (uint32_t)k[0] + (uint32_t)dims[0] * ((uint32_t)k[1] + (uint32_t)dims[1] * (uint32_t)k[2]);
+    if (g.dg_flag[id]) return false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nmin[a] = g.bmin[a] + (float)k[a] * g.h;       // Dataset.cpp:132-133
+        nmax[a] = nmin[a] + g.h;
+    }
+    return true;
+}
+
+__device__ __forceinline__ float pm_gmin(float x, float y) { return (y < x) ? y : x; }
+__device__ __forceinline__ float pm_gmax(float x, float y) { return (x < y) ? y : x; }
+
+// intersectAABB (RayMarcher.cpp:51-62): t in units of dir (the step vector).
+__device__ __forceinline__ void pm_intersect_aabb(const float *o, const float *d, const float *bmin, const float *bmax,
+                                                  float *out)
+{
+    float t1[3], t2[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float tmn = (bmin[a] - o[a]) / d[a], tmx = (bmax[a] - o[a]) / d[a];
+        t1[a] = pm_gmin(tmn, tmx);
+        t2[a] = pm_gmax(tmn, tmx);
+    }
+    float tnear = pm_gmax(pm_gmax(t1[0], t1[1]), t1[2]);
+    float tfar = pm_gmin(pm_gmin(t2[0], t2[1]), t2[2]);
+    float t = (tnear < tfar) ? tfar : tnear;
+#pragma unroll
+    for (int a = 0; a < 3; a++) out[a] = t >= 0.0f ? o[a] + d[a] * t : o[a];
 }
 
 // PerPixel_Isotropic (RayMarcher.cpp:256-344): start on the depth-buffer
@@ -334,6 +656,14 @@ __global__ __launch_bounds__(256) void k_march_reference(MarchLaunch L)
             float st0 = s.step * d0, st1 = s.step * d1, st2 = s.step * d2;
             for (int i = 0; i < s.max_steps; i++) {
                 p0 += st0; p1 += st1; p2 += st2;
+                if (s.skip_mode == PM_SKIP_DG_BOX || s.skip_mode == PM_SKIP_DG_CONS) {   // RayMarcher.cpp:284-304
+                    float nmin[3], nmax[3];
+                    while (pm_dg_query(g, p0, p1, p2, nmin, nmax)) {
+                        float pin[3] = { p0, p1, p2 }, stv[3] = { st0, st1, st2 }, q[3];
+                        pm_intersect_aabb(pin, stv, nmin, nmax, q);
+                        p0 = q[0] + st0; p1 = q[1] + st1; p2 = q[2] + st2;
+                    }
+                }
                 uint32_t cand = 0;
                 float rho = pm_density32(g, p0, p1, p2, comp, cand);
                 c_samples++; c_ms++; c_cand += cand;
@@ -380,14 +710,142 @@ __global__ __launch_bounds__(256) void k_march_reference(MarchLaunch L)
     flush_counters(L.counters, c_rays, c_in, c_hits, c_samples, c_cand, c_ms);
 }
 
+// Batched schedule: persistent waves take chunks of tiles (each chunk belongs to
+// one (view, frame) job) from one counter; inside a chunk lanes are refilled
+// ray by ray as in k_march_persist; a wave moves to its next chunk -- and
+// loads that job's grid and camera -- once all its lanes are idle.
+template <bool LEARNED>
+__global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
+{
+    constexpr int kRegion = LEARNED ? 2112 : kLdsCap * 64;
+    __shared__ float lds[4 * kRegion];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float *region = lds + wave * kRegion;
+    float *buf = region + lane;
+    const bool use_lds = B.base.s.compensated != 0;
+
+    MarchLaunch L = B.base;
+    Counts cn;
+    RayState r{};
+    r.phase = PH_DONE;
+    uint32_t tile = 0, tile_i = 0, ntiles = 0, tile0 = 0;
+    int qpos = 64;
+    bool exhausted = false;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+
+    while (true) {
+        while (!exhausted) {
+            unsigned long long needy = __ballot(r.phase == PH_DONE);
+            if (needy == 0) break;
+            if (qpos >= 64) {
+                if (tile_i < ntiles) {
+                    tile = tile0 + tile_i++;
+                    qpos = 0;
+                } else {
+                    if (__ballot(r.phase != PH_DONE) != 0) break;    // drain before changing job
+                    uint32_t cidx = 0;
+                    if (lane == 0) cidx = atomicAdd(B.chunk_ctr, 1u);
+                    cidx = __shfl(cidx, 0, 64);
+                    if (cidx >= B.nchunks) { exhausted = true; break; }
+                    const BatchChunk ch = B.chunks[cidx];
+                    const BatchJob &J = B.jobs[ch.job];
+                    L.g = J.g;
+                    L.cg = J.cg;
+#pragma unroll
+                    for (int i = 0; i < 16; i++) L.M[i] = J.M[i];
+                    L.cam[0] = J.cam[0]; L.cam[1] = J.cam[1]; L.cam[2] = J.cam[2];
+                    L.W = J.W; L.H = J.H;
+                    L.pos = J.pos; L.nrm = J.nrm; L.t = J.t;
+                    tile0 = ch.tile0;
+                    ntiles = ch.ntiles;
+                    tile_i = 0;
+                    continue;
+                }
+            }
+            int rank = __popcll(needy & lt_mask);
+            int take = __popcll(needy);
+            if (r.phase == PH_DONE && qpos + rank < 64) {
+                uint32_t j = (uint32_t)(qpos + rank);
+                const uint32_t tiles_x = (L.W + 7) / 8;
+                uint32_t px = (tile % tiles_x) * 8 + (j & 7u), py = (tile / tiles_x) * 8 + (j >> 3);
+                if (px < L.W && py < L.H) ray_setup(L, r, px, py, cn);
+            }
+            qpos = qpos + take < 64 ? qpos + take : 64;
+        }
+        if (__ballot(r.phase != PH_DONE) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (LEARNED) {
+            const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
+            const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
+            if (nwant > 0 && (2 * nwant >= nbusy || exhausted || (qpos >= 64 && tile_i >= ntiles)))
+                wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+        }
+        if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
+            if (use_lds) ray_step(L, r, DensLds<kLdsCap, 4>{ &L.g, buf }, cn);
+            else ray_step(L, r, DensPlain{ &L.g, false }, cn);
+        }
+    }
+    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+    if (LEARNED) {
+        unsigned long long m = wave_sum(cn.mlp);
+        if (lane == 0) atomicAdd(&L.counters[C_MLP], m);
+    }
+}
+
+hipError_t launch_march_batch(const BatchLaunch &B, uint32_t blocks, hipStream_t st)
+{
+    if (B.base.s.learned) hipLaunchKernelGGL((k_march_batch<true>), dim3(blocks), dim3(256), 0, st, B);
+    else hipLaunchKernelGGL((k_march_batch<false>), dim3(blocks), dim3(256), 0, st, B);
+    return hipGetLastError();
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, p, (int)n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
 {
     uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
-    uint32_t blocks = (tiles + 3) / 4;
-    if (L.s.mode == PM_MODE_REFERENCE)
-        hipLaunchKernelGGL(k_march_reference, dim3(blocks), dim3(256), 0, st, L);
-    else
-        hipLaunchKernelGGL(k_march_config, dim3(blocks), dim3(256), 0, st, L);
+    dim3 tb((tiles + 3) / 4), blk(256);
+    if (L.s.mode == PM_MODE_REFERENCE) {
+        hipLaunchKernelGGL(k_march_reference, tb, blk, 0, st, L);
+        return hipGetLastError();
+    }
+    switch (L.s.learned ? 0 : L.s.variant) {
+    case 1: hipLaunchKernelGGL((k_march_tile<false, 1>), tb, blk, 0, st, L); return hipGetLastError();
+    case 4: hipLaunchKernelGGL((k_march_tile<true, 4>), tb, blk, 0, st, L); return hipGetLastError();
+    case 5: hipLaunchKernelGGL((k_march_tile<true, 8>), tb, blk, 0, st, L); return hipGetLastError();
+    default: break;
+    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L.band_ctr, 8);
+    // enough resident waves to fill the chip; extra waves find no tile and exit
+    uint32_t blocks = L.persist_blocks ? L.persist_blocks : 1024;
+    uint32_t need = (tiles + 3) / 4;
+    if (blocks > need) blocks = (need + 7) & ~7u;
+    if (L.s.learned) {
+        hipLaunchKernelGGL((k_march_persist<4, true>), dim3(blocks), blk, 0, st, L, L.band_ctr);
+        return hipGetLastError();
+    }
+    switch (L.s.variant) {
+    case 2: hipLaunchKernelGGL((k_march_persist<1, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
+    case 3: hipLaunchKernelGGL((k_march_persist<8, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
+    default: hipLaunchKernelGGL((k_march_persist<4, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
+    }
     return hipGetLastError();
 }
 

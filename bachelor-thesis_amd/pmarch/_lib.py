@@ -21,6 +21,7 @@ PM_OK, PM_PENDING = 0, 1
 MODE_CONFIG, MODE_REFERENCE = 0, 1
 NORMAL_CDIFF, NORMAL_GRADW = 0, 1
 SKIP_NONE, SKIP_EMPTY, SKIP_DG_BOX, SKIP_DG_CONS = 0, 1, 2, 3
+MLP_INPUTS, MLP_PARAMS = 24, 6337
 
 
 class PmSettings(C.Structure):
@@ -28,7 +29,8 @@ class PmSettings(C.Structure):
         ("mode", C.c_int32), ("max_steps", C.c_int32), ("step", C.c_float), ("iso", C.c_float),
         ("bisect_iters", C.c_int32), ("final_fp64", C.c_int32), ("normal_eps", C.c_float),
         ("normal_mode", C.c_int32), ("skip_mode", C.c_int32), ("compensated", C.c_int32),
-        ("learned", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("learned", C.c_int32), ("variant", C.c_int32), ("learned_alpha", C.c_float),
+        ("learned_min_skip", C.c_int32), ("learned_max_skip", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -87,6 +89,16 @@ def load():
     L.pm_neighbor_counts.argtypes = [vp, i32, vp, u64, vp, vp]
     L.pm_density.argtypes = [vp, i32, vp, u64, C.c_int, vp]
     L.pm_density64.argtypes = [vp, i32, vp, u64, vp]
+    L.pm_stream_copy_gbps.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
+    L.pm_get_skip_grids.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.pm_mlp_set.argtypes = [vp, vp]
+    L.pm_mlp_get.argtypes = [vp, vp]
+    L.pm_mlp_forward.argtypes = [vp, vp, u64, vp]
+    L.pm_mlp_grad.argtypes = [vp, vp, vp, u64, f32, vp, vp]
+    L.pm_dataset_clear.argtypes = [vp, u64]
+    L.pm_trace_append.argtypes = [vp, i32, C.POINTER(PmSettings), vp, vp, u32, u32, u32, u32, u32, u64, C.POINTER(u64)]
+    L.pm_dataset_get.argtypes = [vp, u64, u64, vp, vp, vp, vp]
+    L.pm_mlp_train.argtypes = [vp, C.c_int, u32, f32, f32, u64, vp]
     _lib = L
     return L
 
@@ -191,6 +203,77 @@ class Marcher:
         t = np.zeros((H, W), np.float32)
         self._check(self.L.pm_download(self.ctx, _ptr(pos), _ptr(nrm), _ptr(t)), "pm_download")
         return pos, nrm, t
+
+    def skip_grids(self, frame: int):
+        fi = self.frame_info(frame)
+        dims = np.zeros(3, np.int32)
+        self._check(self.L.pm_get_skip_grids(self.ctx, frame, None, _ptr(dims), None, None, None), "pm_get_skip_grids")
+        n = int(np.prod(dims))
+        e27 = np.zeros(fi.ncells, np.uint8)
+        fb = np.zeros(n, np.uint8)
+        fc = np.zeros(n, np.uint8)
+        cb = np.zeros(n, np.uint32)
+        self._check(self.L.pm_get_skip_grids(self.ctx, frame, _ptr(e27), _ptr(dims), _ptr(fb), _ptr(fc), _ptr(cb)),
+                    "pm_get_skip_grids")
+        return e27, tuple(int(v) for v in dims), cb, fb, fc
+
+    # ---- learned step proposal ----
+    def mlp_set(self, params):
+        p = np.ascontiguousarray(params, np.float32).reshape(-1)
+        assert p.size == MLP_PARAMS
+        self._check(self.L.pm_mlp_set(self.ctx, _ptr(p)), "pm_mlp_set")
+
+    def mlp_get(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        self._check(self.L.pm_mlp_get(self.ctx, _ptr(p)), "pm_mlp_get")
+        return p
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        y = np.zeros(x.shape[0], np.float32)
+        self._check(self.L.pm_mlp_forward(self.ctx, _ptr(x), x.shape[0], _ptr(y)), "pm_mlp_forward")
+        return y
+
+    def mlp_grad(self, x, t, w_over=4.0):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.L.pm_mlp_grad(self.ctx, _ptr(x), _ptr(t), x.shape[0], float(w_over), _ptr(loss), _ptr(g)),
+                    "pm_mlp_grad")
+        return float(loss[0]), g
+
+    def dataset_clear(self, capacity: int):
+        self._check(self.L.pm_dataset_clear(self.ctx, int(capacity)), "pm_dataset_clear")
+
+    def trace_append(self, frame, settings, inv_pv, cam, W, H, ray_stride=16, k_stride=3, seed=0, max_new_rows=0) -> int:
+        s = settings_from(settings)
+        inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
+        cam = np.ascontiguousarray(cam, np.float32).reshape(3)
+        tot = C.c_uint64(0)
+        self._check(self.L.pm_trace_append(self.ctx, frame, C.byref(s), _ptr(inv_pv), _ptr(cam), W, H, ray_stride,
+                                           k_stride, seed, int(max_new_rows), C.byref(tot)), "pm_trace_append")
+        return int(tot.value)
+
+    def dataset_get(self, offset, n):
+        f = np.zeros((n, MLP_INPUTS), np.float32)
+        y = np.zeros(n, np.float32)
+        hit = np.zeros(n, np.uint8)
+        meta = np.zeros((n, 2), np.uint32)
+        self._check(self.L.pm_dataset_get(self.ctx, offset, n, _ptr(f), _ptr(y), _ptr(hit), _ptr(meta)),
+                    "pm_dataset_get")
+        return f, y, hit, meta
+
+    def mlp_train(self, epochs, batch=4096, lr=2e-3, w_over=4.0, seed=0):
+        losses = np.zeros(epochs, np.float32)
+        self._check(self.L.pm_mlp_train(self.ctx, epochs, batch, float(lr), float(w_over), seed, _ptr(losses)),
+                    "pm_mlp_train")
+        return losses
+
+    def stream_copy_gbps(self, nbytes: int, iters: int = 20) -> float:
+        g = C.c_float(0.0)
+        self._check(self.L.pm_stream_copy_gbps(self.ctx, nbytes, iters, C.byref(g)), "pm_stream_copy_gbps")
+        return float(g.value)
 
     def neighbor_counts(self, frame: int, q):
         q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)

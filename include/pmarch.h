@@ -65,7 +65,12 @@ typedef struct pm_settings {
     int32_t skip_mode;     /* PM_SKIP_*                                               */
     int32_t compensated;   /* 1 = Neumaier-compensated FP32 density sums              */
     int32_t learned;       /* 1 = learned step proposal (needs pm_set_mlp)            */
-    int32_t reserved[5];
+    int32_t variant;       /* kernel schedule: 0 = persistent waves with ray refill,
+                              1 = one 8x8 tile per wave (same results, for A/B)       */
+    float   learned_alpha; /* learned: fraction of the predicted distance skipped (0 = 0.7) */
+    int32_t learned_min_skip; /* learned: shorter proposals end the proposals (0 = 4)  */
+    int32_t learned_max_skip; /* learned: longest jump in steps (0 = 24)              */
+    int32_t reserved[1];
 } pm_settings;
 
 /* Per-launch counters, written by the march kernel itself. */
@@ -111,6 +116,13 @@ int pm_frame_info_get(pm_ctx *ctx, int32_t frame, pm_frame_info *out);
 /* The frame's grid: cell_start[ncells+1], sorted[4n] (x,y,z,0), index[n]
  * (input index of each stored particle).  Any pointer may be NULL. */
 int pm_get_grid(pm_ctx *ctx, int32_t frame, uint32_t *cell_start, float *sorted, uint32_t *index);
+/* The frame's skip structures: empty27[ncells] (1 where a cell and its 26
+ * neighbours are empty) and the reference DensityGrid (Frame::BuildDensityGrid,
+ * Dataset.cpp:94-164) over the padded AABB: dims, box flag, conservative flag
+ * and per-node box counts.  Any pointer may be NULL; call with NULL arrays first
+ * to read dg_dims. */
+int pm_get_skip_grids(pm_ctx *ctx, int32_t frame, uint8_t *empty27, int32_t dg_dims[3], uint8_t *flag_box,
+                      uint8_t *flag_cons, uint32_t *count_box);
 
 /* RayMarcher::Prepare + Start (RayMarcher.cpp:76-112).  inv_pv is
  * Camera3D::GetInvProjectionView (column-major 16 floats), cam is
@@ -139,6 +151,46 @@ int pm_neighbor_counts(pm_ctx *ctx, int32_t frame, const float *q, uint64_t nq,
 int pm_density(pm_ctx *ctx, int32_t frame, const float *q, uint64_t nq, int compensated,
                float *rho);
 int pm_density64(pm_ctx *ctx, int32_t frame, const double *q, uint64_t nq, double *rho);
+
+/* ---- learned step proposal (no counterpart in the reference; it replaces the
+ * role of the DensityGrid skip, Dataset.cpp:94-164, RayMarcher.cpp:284-304) ----
+ * A 24 -> 64 -> 64 -> 1 MLP, BF16 weights with FP32 accumulation on MFMA,
+ * predicts log2(1 + d/h) for d the distance along the ray to the first iso
+ * crossing.  The configured march with learned = 1 jumps ahead on the same
+ * sample lattice, binary-searches back to the first sample at or above iso and
+ * then bisects exactly as the plain march does.
+ * Parameters: PM_MLP_NPARAM floats, W1[64][32] (inputs 24..31 zero), b1[64],
+ * W2[64][64], b2[64], w3[64], b3. */
+#define PM_MLP_INPUTS 24
+#define PM_MLP_PARAMS 6337
+int pm_mlp_set(pm_ctx *ctx, const float *params);
+int pm_mlp_get(pm_ctx *ctx, float *params);
+/* Batch inference of n rows of 24 features (host arrays). */
+int pm_mlp_forward(pm_ctx *ctx, const float *x, uint64_t n, float *y);
+/* Loss sum_i w_i (y_i - t_i)^2 / n (w_i = w_over when y_i > t_i) and its
+ * gradient for the loaded network, n a multiple of 64 (host arrays). */
+int pm_mlp_grad(pm_ctx *ctx, const float *x, const float *target, uint64_t n, float w_over, float *loss,
+                float *grad);
+/* Device training set: clear (capacity rows), append traces of the plain
+ * configured march of one view (rays with hash % ray_stride == 0, every
+ * k_stride-th sample before the crossing, at most max_new_rows rows, 0 = up to
+ * the capacity), read rows back (features, label log2(1 + d/h), hit flag, and
+ * the (pixel, sample index) each row came from). */
+int pm_dataset_clear(pm_ctx *ctx, uint64_t capacity);
+int pm_trace_append(pm_ctx *ctx, int32_t frame, const pm_settings *s, const float inv_pv[16], const float cam[3],
+                    uint32_t W, uint32_t H, uint32_t ray_stride, uint32_t k_stride, uint32_t seed,
+                    uint64_t max_new_rows, uint64_t *rows_total);
+int pm_dataset_get(pm_ctx *ctx, uint64_t offset, uint64_t n, float *feat, float *label, uint8_t *is_hit,
+                   uint32_t *meta);
+/* Adam on the device training set: epochs x (rows / batch) steps, batch a
+ * multiple of 64; mean batch loss per epoch into loss_per_epoch (may be NULL). */
+int pm_mlp_train(pm_ctx *ctx, int epochs, uint32_t batch, float lr, float w_over, uint64_t seed,
+                 float *loss_per_epoch);
+
+/* Measurement utility, no counterpart in the reference: times `iters`
+ * launches of a streaming copy of `bytes` bytes (read + write) on the context
+ * stream and returns the achieved bandwidth, counting 2*bytes per launch. */
+int pm_stream_copy_gbps(pm_ctx *ctx, uint64_t bytes, int iters, float *gbps);
 
 #ifdef __cplusplus
 }

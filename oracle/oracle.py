@@ -65,6 +65,7 @@ class GridStruct(C.Structure):
         ("dg_flag_box", C.POINTER(C.c_uint8)),
         ("dg_flag_cons", C.POINTER(C.c_uint8)),
         ("dg_count_box", C.POINTER(C.c_uint32)),
+        ("empty27", C.POINTER(C.c_uint8)),
     ]
 
 
@@ -197,6 +198,9 @@ class Grid:
                 np.ctypeslib.as_array(self.g.dg_count_box, shape=(n,)).copy(),
                 np.ctypeslib.as_array(self.g.dg_flag_box, shape=(n,)).copy(),
                 np.ctypeslib.as_array(self.g.dg_flag_cons, shape=(n,)).copy())
+
+    def empty27(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.g.empty27, shape=(self.g.ncells,)).copy()
 
     def neighbors(self, q) -> tuple[np.ndarray, int]:
         q = np.ascontiguousarray(q, np.float32)

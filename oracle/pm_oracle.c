@@ -121,6 +121,7 @@ static inline int stencil_centre(const or_grid *g, float x, float y, float z, in
 }
 
 static void densitygrid_build(or_grid *g);
+static void empty27_build(or_grid *g);
 
 int or_grid_build(const float *xyz, uint64_t n, float h, or_grid *g)
 {
@@ -179,13 +180,14 @@ int or_grid_build(const float *xyz, uint64_t n, float h, or_grid *g)
     free(cursor);
     free(cell);
     densitygrid_build(g);
+    empty27_build(g);
     return 0;
 }
 
 void or_grid_free(or_grid *g)
 {
     free(g->cell_start); free(g->sorted); free(g->sorted_index);
-    free(g->dg_flag_box); free(g->dg_flag_cons); free(g->dg_count_box);
+    free(g->dg_flag_box); free(g->dg_flag_cons); free(g->dg_count_box); free(g->empty27);
     memset(g, 0, sizeof(*g));
 }
 
@@ -484,6 +486,51 @@ static void densitygrid_build(or_grid *g)
             }
 }
 
+/* empty27[c] = 1 when cells c-1..c+1 on every axis hold no particle; cells on
+ * the grid border count their missing neighbours as empty. */
+static void empty27_build(or_grid *g)
+{
+    g->empty27 = (uint8_t *)calloc(g->ncells, 1);
+    for (int32_t z = 0; z < g->dims[2]; z++)
+        for (int32_t y = 0; y < g->dims[1]; y++)
+            for (int32_t x = 0; x < g->dims[0]; x++) {
+                uint32_t cnt = 0;
+                for (int dz = -1; dz <= 1; dz++)
+                    for (int dy = -1; dy <= 1; dy++) {
+                        int32_t yy = y + dy, zz = z + dz;
+                        if (yy < 0 || yy >= g->dims[1] || zz < 0 || zz >= g->dims[2]) continue;
+                        int32_t x0 = x > 0 ? x - 1 : 0, x1 = x < g->dims[0] - 1 ? x + 1 : g->dims[0] - 1;
+                        cnt += g->cell_start[cell_lin(g, x1, yy, zz) + 1] - g->cell_start[cell_lin(g, x0, yy, zz)];
+                    }
+                g->empty27[cell_lin(g, x, y, z)] = cnt == 0;
+            }
+}
+
+/* Empty-space skip of the configured march.  If the sample's cell has an empty
+ * 27-neighbourhood, every sample the ray places inside that cell has zero
+ * density and cannot be the crossing, so the march resumes at the last sample
+ * before the ray leaves the cell box [key h, (key+1) h]; the first-crossing
+ * sample, and so the whole result, is the same as without the skip.  Returns
+ * the next sample index (k when there is nothing to skip). */
+static int64_t skip_empty(const or_grid *g, const float o[3], const float d[3], const float x[3], float tn,
+                          float step, int64_t k)
+{
+    int32_t c[3];
+    if (!stencil_centre(g, x[0], x[1], x[2], c)) return k;
+    if (!g->empty27[cell_lin(g, c[0], c[1], c[2])]) return k;
+    float texit = INFINITY;
+    for (int a = 0; a < 3; a++) {
+        if (d[a] == 0.0f) continue;
+        int32_t key = c[a] + g->origin[a];
+        float lo = (float)key * g->h, hi = (float)(key + 1) * g->h;
+        float ta = ((d[a] > 0.0f ? hi : lo) - o[a]) / d[a];
+        if (ta < texit) texit = ta;
+    }
+    float kf = floorf((texit - tn) / step);
+    int64_t kn = kf > (float)k ? (int64_t)kf : k + 1;
+    return kn;
+}
+
 /* QueryDensityGrid (Dataset.cpp:26-47): returns the node id or -1. */
 static int64_t dg_query(const or_grid *g, const float p[3], float nmin[3], float nmax[3])
 {
@@ -681,14 +728,19 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
 
     float step = s->step;
     int64_t k = 0, hitk = -1;
-    for (;; k++) {
+    for (;;) {
         if (s->max_steps > 0 && k >= s->max_steps) break;
         float t = tn + (float)k * step;
         if (t > tf) break;
         float x[3] = { o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2] };
+        if (s->skip_mode == 1) {
+            int64_t kn = skip_empty(g, o, d, x, tn, step, k);
+            if (kn != k) { k = kn; continue; }
+        }
         float rho = dens_f(dc, x, cnt);
         cnt->march_samples++;
         if (rho >= s->iso) { hitk = k; break; }
+        k++;
     }
     if (hitk < 0) return;
     cnt->hits++;

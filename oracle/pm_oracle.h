@@ -69,6 +69,9 @@ typedef struct or_grid {
     uint8_t  *dg_flag_box;          /* any particle with |x_j-c|_inf < h      */
     uint8_t  *dg_flag_cons;         /* any particle with |x_j-c|_inf < 1.5h   */
     uint32_t *dg_count_box;
+    /* 1 where the cell and its 26 neighbours hold no particle: every point of
+     * the cell has zero density (empty-space skip of the configured march)   */
+    uint8_t  *empty27;
 } or_grid;
 
 /* ---- SPH cubic spline kernel (Kernel.cpp:8-52) ---- */

@@ -74,13 +74,13 @@ def test_density_bitexact(marcher, compensated):
     assert np.array_equal(marcher.density64(3, q64), G.density64(q64))
 
 
-@pytest.mark.parametrize("seed,cam", [(0, 0), (1, 1), (2, 0)])
-def test_march_lattice_cube(marcher, seed, cam):
+@pytest.mark.parametrize("seed,cam,variant", [(0, 0, 0), (1, 1, 0), (2, 0, 0), (0, 0, 1), (1, 1, 1)])
+def test_march_lattice_cube(marcher, seed, cam, variant):
     sc = scenes.lattice_cube(seed)
     c = sc.cameras[cam]
-    print(f"march lattice_cube seed={seed} cam={cam} {c.width}x{c.height}", flush=True)
+    print(f"march lattice_cube seed={seed} cam={cam} variant={variant} {c.width}x{c.height}", flush=True)
     marcher.set_frame(0, sc.xyz, sc.h)
-    st = sc.settings()
+    st = sc.settings(variant=variant)
     pos, nrm, t, stats = marcher.march(0, st, c.inv_pv, c.eye, c.width, c.height)
     G = O.Grid(sc.xyz, sc.h)
     opos, onrm, ot, ocnt = O.march(G, O.make_settings(**st), c.inv_pv, c.eye, c.width, c.height)
@@ -92,6 +92,19 @@ def test_march_lattice_cube(marcher, seed, cam):
     assert np.array_equal(pos, opos) and np.array_equal(nrm, onrm) and np.array_equal(t, ot)
     for k in ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples"):
         assert stats[k] == ocnt[k], (k, stats[k], ocnt[k])
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+def test_march_variants_identical(marcher, variant):
+    """Every kernel schedule produces the same image as the default one."""
+    sc = scenes.dam_break(1, nx=40, ny=80, nz=40, width=320, height=240)
+    c = sc.cameras[0]
+    marcher.set_frame(4, sc.xyz, sc.h)
+    ref = marcher.march(4, sc.settings(), c.inv_pv, c.eye, c.width, c.height)
+    got = marcher.march(4, sc.settings(variant=variant), c.inv_pv, c.eye, c.width, c.height)
+    for a, b in zip(ref[:3], got[:3]):
+        assert np.array_equal(a, b)
+    assert ref[3]["samples"] == got[3]["samples"] and ref[3]["cand"] == got[3]["cand"]
 
 
 def test_march_vs_fp64(marcher):

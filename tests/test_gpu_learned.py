@@ -1,0 +1,66 @@
+"""The learned step proposal on the GPU: trained here on traces of other seeded
+scenes, then compared with the plain march on a held-out view -- same image to
+within the march's tolerances, fewer density evaluations."""
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+from parity import assert_within_budget, compare, extent
+from pmarch import scenes, train
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def trained(marcher):
+    marcher.dataset_clear(600_000)
+    n = 0
+    for i, s in enumerate([scenes.dam_break(31 + i, nx=60, ny=120, nz=60, width=640, height=360) for i in range(3)]):
+        rng = np.random.default_rng(50 + i)
+        cam = scenes.random_views(rng, scenes.COLUMN_CENTRE, 1, 640, 640, 360, 360, radius=(1.1, 1.8))[0]
+        marcher.set_frame(30, s.xyz, s.h)
+        n = marcher.trace_append(30, s.settings(), cam.inv_pv, cam.eye, cam.width, cam.height, 2, 2, seed=i)
+    assert n > 50_000
+    marcher.mlp_set(train.he_init(0))
+    losses = marcher.mlp_train(6, batch=2048, lr=2e-3, w_over=train.W_OVER, seed=1)
+    print("rows", n, "loss per epoch", losses, flush=True)
+    assert losses[-1] < 0.6 * losses[0]
+    return marcher.mlp_get()
+
+
+def test_learned_matches_plain(marcher, trained):
+    sc = scenes.dam_break(0, nx=60, ny=120, nz=60, width=960, height=540)
+    c = sc.cameras[0]
+    marcher.set_frame(31, sc.xyz, sc.h)
+    marcher.mlp_set(trained)
+    plain = marcher.march(31, sc.settings(), c.inv_pv, c.eye, c.width, c.height)
+    lrn = marcher.march(31, sc.settings(learned=1), c.inv_pv, c.eye, c.width, c.height)
+    m = compare(*lrn[:3], *plain[:3])
+    sp, sl = plain[3], lrn[3]
+    print("plain", sp, "\nlearned", sl, "\n", m, flush=True)
+    assert_within_budget(m, extent(sc.xyz))
+    assert sl["mlp_evals"] > 0
+    assert sl["samples"] < sp["samples"]
+    # where both hit, most rays land on the identical sample lattice crossing
+    both = (plain[0][..., 3] > 0) & (lrn[0][..., 3] > 0)
+    same = np.mean(np.all(plain[0][both] == lrn[0][both], axis=-1))
+    print("bit-identical hit fraction", same, flush=True)
+    assert same > 0.995
+
+
+def test_learned_without_network_is_rejected(marcher):
+    import ctypes
+    import pmarch
+    m2 = pmarch.Marcher(0)
+    sc = scenes.lattice_cube(0, width=64, height=48)
+    m2.set_frame(0, sc.xyz, sc.h)
+    with pytest.raises(pmarch.PmError):
+        m2.march(0, sc.settings(learned=1), sc.cameras[0].inv_pv, sc.cameras[0].eye, 64, 48)
+    m2.close()
+    assert ctypes is not None
+
+
+def test_mlp_init_matches_oracle_layout():
+    p = train.he_init(0)
+    assert p.size == M.NPARAM
+    assert np.all(p[M.OFF_W1:M.OFF_B1].reshape(M.H, M.K1)[:, M.NIN:] == 0)
