@@ -44,7 +44,9 @@ struct Frame {
     pm_frame_info info{};
     GridDev g{};
     DevBuf sorted, cell_start, cell_of, counts, cursor, scan_tmp, xyz, bounds_partial, bounds;
-    DevBuf empty27, dg_box, dg_cons, dg_count, coarse;
+    DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound;
+    bool bound_ok = false;      // bound holds the sub-box bits for bound_iso
+    float bound_iso = 0.0f;
     int32_t dg_dims[3] = { 0, 0, 0 };
     CoarseDev cg{};
     void release()
@@ -52,7 +54,7 @@ struct Frame {
         coarse.release();
         sorted.release(); cell_start.release(); cell_of.release(); counts.release(); cursor.release();
         scan_tmp.release(); xyz.release(); bounds_partial.release(); bounds.release();
-        empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release();
+        empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release(); bound.release();
     }
 };
 
@@ -83,6 +85,18 @@ struct pm_ctx {
     // training set written by the trace logger
     DevBuf ds_feat, ds_label, ds_hit, ds_count, ds_meta;
     uint64_t ds_cap = 0;
+    // batched views: job and tile-chunk tables, all views' images, the captured launch
+    uint64_t frame_epoch = 0;           // bumped whenever a frame's device buffers may move
+    uint64_t batch_epoch = ~0ull;
+    DevBuf b_jobs, b_chunks, b_img, b_ctr;
+    std::vector<uint64_t> b_off;        // per view: first pixel in b_img
+    std::vector<uint32_t> b_w, b_h;
+    uint64_t b_npx = 0;
+    BatchLaunch b_launch{};
+    uint32_t b_blocks = 0;
+    hipGraph_t b_graph = nullptr;
+    hipGraphExec_t b_exec = nullptr;
+    bool b_ready = false;
 };
 
 static MlpDev mlp_dev(pm_ctx *c)
@@ -166,6 +180,9 @@ void pm_destroy(pm_ctx *c)
     c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release(); c->mlp_W1.release(); c->mlp_W2.release();
     c->mlp_W2T.release(); c->mlp_partial.release(); c->mlp_out.release(); c->mlp_x.release(); c->mlp_idx.release();
     c->ds_feat.release(); c->ds_label.release(); c->ds_hit.release(); c->ds_count.release(); c->ds_meta.release();
+    if (c->b_exec) hipGraphExecDestroy(c->b_exec);
+    if (c->b_graph) hipGraphDestroy(c->b_graph);
+    c->b_jobs.release(); c->b_chunks.release(); c->b_img.release(); c->b_ctr.release();
     if (c->h_stage) hipHostFree(c->h_stage);
     if (c->h_counters) hipHostFree(c->h_counters);
     if (c->ev_launch) hipEventDestroy(c->ev_launch);
@@ -184,6 +201,8 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     auto it = c->frames.find(frame);
     if (it == c->frames.end()) { f = new Frame(); c->frames[frame] = f; }
     else f = it->second;
+    c->frame_epoch++;   // a batch set before this holds the old grid's pointers
+    f->bound_ok = false;
 
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
@@ -351,6 +370,20 @@ int pm_get_grid(pm_ctx *c, int32_t frame, uint32_t *cell_start, float *sorted, u
     return PM_OK;
 }
 
+// Sub-box density bounds of a frame for one iso value, built on the context
+// stream when first needed (a batch holding the old bits is invalidated).
+static int ensure_bound(pm_ctx *c, Frame *f, float iso)
+{
+    if (f->bound_ok && f->bound_iso == iso) return PM_OK;
+    HIPCHK(c, f->bound.ensure(sizeof(uint64_t) * f->info.ncells));
+    f->g.bound = (const uint64_t *)f->bound.p;
+    HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, c->stream));
+    f->bound_ok = true;
+    f->bound_iso = iso;
+    c->frame_epoch++;
+    return PM_OK;
+}
+
 static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
 {
     if (!s) return fail(c, PM_ERR_ARG, "settings is NULL");
@@ -359,11 +392,12 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
     if (!std::isfinite(s->iso)) return fail(c, PM_ERR_ARG, "iso must be finite");
     if (s->bisect_iters < 0 || s->bisect_iters > 40) return fail(c, PM_ERR_ARG, "bisect_iters out of [0,40]");
     if (s->normal_mode != PM_NORMAL_CDIFF && s->normal_mode != PM_NORMAL_GRADW) return fail(c, PM_ERR_ARG, "unknown normal_mode");
-    if (s->skip_mode < PM_SKIP_NONE || s->skip_mode > PM_SKIP_DG_CONS) return fail(c, PM_ERR_ARG, "unknown skip_mode %d", s->skip_mode);
+    if (s->skip_mode < PM_SKIP_NONE || s->skip_mode > PM_SKIP_BOUND) return fail(c, PM_ERR_ARG, "unknown skip_mode %d", s->skip_mode);
     if (s->mode == PM_MODE_CONFIG && (s->skip_mode == PM_SKIP_DG_BOX || s->skip_mode == PM_SKIP_DG_CONS))
         return fail(c, PM_ERR_ARG, "the DensityGrid skip belongs to the reference march");
-    if (s->mode == PM_MODE_REFERENCE && s->skip_mode == PM_SKIP_EMPTY)
-        return fail(c, PM_ERR_ARG, "the empty-stencil skip belongs to the configured march");
+    if (s->mode == PM_MODE_REFERENCE && (s->skip_mode == PM_SKIP_EMPTY || s->skip_mode == PM_SKIP_BOUND))
+        return fail(c, PM_ERR_ARG, "the empty-stencil and bound skips belong to the configured march");
+    if (s->variant < 0 || s->variant > 8) return fail(c, PM_ERR_ARG, "unknown kernel variant %d", s->variant);
     if (s->learned && !c->mlp_ready) return fail(c, PM_ERR_STATE, "learned step: no network loaded (pm_mlp_set)");
     if (s->learned && s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "the learned step belongs to the configured march");
     if (s->learned && !(s->learned_alpha >= 0.0f && s->learned_alpha <= 4.0f)) return fail(c, PM_ERR_ARG, "learned_alpha out of [0,4]");
@@ -396,6 +430,10 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     if (s->mode == PM_MODE_REFERENCE && !depth_in) return fail(c, PM_ERR_ARG, "reference mode needs depth_in");
     HIPCHK(c, hipSetDevice(c->device));
 
+    if (s->skip_mode == PM_SKIP_BOUND) {
+        rc = ensure_bound(c, f, s->iso);
+        if (rc) return rc;
+    }
     uint64_t npx = (uint64_t)W * H;
     HIPCHK(c, c->d_pos.ensure(sizeof(float4) * npx));
     HIPCHK(c, c->d_nrm.ensure(sizeof(float4) * npx));
@@ -516,6 +554,181 @@ int pm_get_stats(pm_ctx *c, pm_stats *out)
 {
     if (!c || !out) return PM_ERR_ARG;
     *out = c->stats;
+    return PM_OK;
+}
+
+// ---- batched views x frames ----
+
+static constexpr uint32_t kChunkTiles = 16;   // 8x8 tiles per queue entry (1024 rays)
+
+static int batch_enqueue_launch(pm_ctx *c)
+{
+    HIPCHK(c, launch_zero_u64(c->b_launch.base.counters, C_COUNT, c->stream));
+    HIPCHK(c, launch_zero_u32(c->b_launch.chunk_ctr, 1, c->stream));
+    HIPCHK(c, launch_march_batch(c->b_launch, c->b_blocks, c->stream));
+    return PM_OK;
+}
+
+int pm_get_bound_grid(pm_ctx *c, int32_t frame, float iso, uint64_t *bits)
+{
+    if (!c || !bits) return PM_ERR_ARG;
+    if (c->pending) return fail(c, PM_ERR_STATE, "pm_get_bound_grid: a march is in flight");
+    if (!std::isfinite(iso)) return fail(c, PM_ERR_ARG, "iso must be finite");
+    Frame *f = get_frame(c, frame);
+    if (!f) return fail(c, PM_ERR_FRAME, "frame %d not set", frame);
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_bound(c, f, iso);
+    if (rc) return rc;
+    HIPCHK(c, copy_sync(c, bits, f->bound.p, sizeof(uint64_t) * f->info.ncells, hipMemcpyDeviceToHost));
+    return PM_OK;
+}
+
+int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t n_views, int use_graph)
+{
+    if (!c) return PM_ERR_ARG;
+    if (c->pending) return fail(c, PM_ERR_STATE, "pm_batch_set: a march is in flight; call pm_wait first");
+    if (!views || n_views == 0 || n_views > (1u << 20)) return fail(c, PM_ERR_ARG, "pm_batch_set: need 1..2^20 views");
+    if (!s) return fail(c, PM_ERR_ARG, "settings is NULL");
+    if (s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "pm_batch_set: the batch runs the configured march");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->b_ready = false;
+    if (c->b_exec) { hipGraphExecDestroy(c->b_exec); c->b_exec = nullptr; }
+    if (c->b_graph) { hipGraphDestroy(c->b_graph); c->b_graph = nullptr; }
+
+    std::vector<BatchJob> jobs(n_views);
+    c->b_off.assign(n_views, 0);
+    c->b_w.assign(n_views, 0);
+    c->b_h.assign(n_views, 0);
+    uint64_t npx = 0;
+    for (uint32_t v = 0; v < n_views; v++) {
+        const pm_view &vw = views[v];
+        if (vw.width == 0 || vw.height == 0 || (uint64_t)vw.width * vw.height > (1ull << 30))
+            return fail(c, PM_ERR_SIZE, "view %u: bad image size %ux%u", v, vw.width, vw.height);
+        Frame *f = get_frame(c, vw.frame);
+        if (!f) return fail(c, PM_ERR_FRAME, "view %u: frame %d not set", v, vw.frame);
+        int rc = validate_settings(c, s, f);
+        if (rc) return rc;
+        if (s->skip_mode == PM_SKIP_BOUND) {
+            rc = ensure_bound(c, f, s->iso);
+            if (rc) return rc;
+        }
+        c->b_off[v] = npx;
+        c->b_w[v] = vw.width;
+        c->b_h[v] = vw.height;
+        npx += (uint64_t)vw.width * vw.height;
+        if (npx > (1ull << 34)) return fail(c, PM_ERR_SIZE, "pm_batch_set: more than 2^34 rays");
+    }
+    c->b_npx = npx;
+    // images: pos (float4) | nrm (float4) | t (float), each over all views
+    HIPCHK(c, c->b_img.ensure(36 * npx));
+    float4 *pos = (float4 *)c->b_img.p;
+    float4 *nrm = pos + npx;
+    float *tt = (float *)(nrm + npx);
+    for (uint32_t v = 0; v < n_views; v++) {
+        const pm_view &vw = views[v];
+        Frame *f = get_frame(c, vw.frame);
+        BatchJob &J = jobs[v];
+        J.g = f->g;
+        J.cg = f->cg;
+        memcpy(J.M, vw.inv_pv, sizeof(J.M));
+        memcpy(J.cam, vw.cam, sizeof(J.cam));
+        J.W = vw.width;
+        J.H = vw.height;
+        J.tiles_x = (vw.width + 7) / 8;
+        J.pos = pos + c->b_off[v];
+        J.nrm = nrm + c->b_off[v];
+        J.t = tt + c->b_off[v];
+    }
+    // queue: views by decreasing ray count (longest first), each as runs of
+    // kChunkTiles consecutive tiles
+    std::vector<uint32_t> order(n_views);
+    for (uint32_t v = 0; v < n_views; v++) order[v] = v;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return (uint64_t)c->b_w[a] * c->b_h[a] > (uint64_t)c->b_w[b] * c->b_h[b];
+    });
+    std::vector<BatchChunk> chunks;
+    for (uint32_t v : order) {
+        uint32_t tiles = ((c->b_w[v] + 7) / 8) * ((c->b_h[v] + 7) / 8);
+        for (uint32_t t0 = 0; t0 < tiles; t0 += kChunkTiles)
+            chunks.push_back(BatchChunk{ v, t0, std::min(kChunkTiles, tiles - t0), 0u });
+    }
+    if (chunks.size() >= (1ull << 32)) return fail(c, PM_ERR_SIZE, "pm_batch_set: too many tiles");
+    HIPCHK(c, c->b_jobs.ensure(sizeof(BatchJob) * jobs.size()));
+    HIPCHK(c, c->b_chunks.ensure(sizeof(BatchChunk) * chunks.size()));
+    HIPCHK(c, c->b_ctr.ensure(256));
+    HIPCHK(c, copy_sync(c, c->b_jobs.p, jobs.data(), sizeof(BatchJob) * jobs.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_sync(c, c->b_chunks.p, chunks.data(), sizeof(BatchChunk) * chunks.size(), hipMemcpyHostToDevice));
+
+    BatchLaunch &B = c->b_launch;
+    B = BatchLaunch{};
+    B.base.s = *s;
+    B.base.counters = (unsigned long long *)c->d_counters.p;
+    if (s->learned) {
+        B.base.net = mlp_dev(c);
+        B.base.mlp_alpha = s->learned_alpha > 0.0f ? s->learned_alpha : 0.7f;
+        B.base.mlp_min_skip = s->learned_min_skip > 0 ? s->learned_min_skip : 4;
+        B.base.mlp_max_skip = s->learned_max_skip > 0 ? s->learned_max_skip : 24;
+    }
+    B.jobs = (const BatchJob *)c->b_jobs.p;
+    B.chunks = (const BatchChunk *)c->b_chunks.p;
+    B.nchunks = (uint32_t)chunks.size();
+    B.chunk_ctr = (uint32_t *)c->b_ctr.p;
+    c->b_blocks = std::min<uint32_t>(1024u, (B.nchunks + 3) / 4 * 4);
+
+    if (use_graph) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = batch_enqueue_launch(c);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) { if (g) hipGraphDestroy(g); return rc; }
+        HIPCHK(c, e);
+        c->b_graph = g;
+        HIPCHK(c, hipGraphInstantiate(&c->b_exec, c->b_graph, nullptr, nullptr, 0));
+    }
+    c->batch_epoch = c->frame_epoch;
+    c->b_ready = true;
+    return PM_OK;
+}
+
+int pm_batch_run(pm_ctx *c)
+{
+    if (!c) return PM_ERR_ARG;
+    if (c->pending) return fail(c, PM_ERR_STATE, "pm_batch_run: a march is in flight; call pm_wait first");
+    if (!c->b_ready) return fail(c, PM_ERR_STATE, "pm_batch_run: no batch (pm_batch_set)");
+    if (c->batch_epoch != c->frame_epoch)
+        return fail(c, PM_ERR_STATE, "pm_batch_run: frames (or their bound bits) changed after pm_batch_set; set the batch again");
+    if (c->b_launch.base.s.learned && !c->mlp_ready) return fail(c, PM_ERR_STATE, "learned step: no network loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipEventRecord(c->ev_launch, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
+    if (c->b_exec) {
+        HIPCHK(c, hipGraphLaunch(c->b_exec, c->stream));
+    } else {
+        int rc = batch_enqueue_launch(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->d_counters.p, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev_done, c->stream));
+    c->user_pos = c->user_nrm = c->user_t = nullptr;
+    c->pending = true;
+    c->finished = false;
+    return PM_OK;
+}
+
+int pm_batch_download(pm_ctx *c, uint32_t view, float *positions, float *normals, float *t)
+{
+    if (!c) return PM_ERR_ARG;
+    if (c->pending) return fail(c, PM_ERR_STATE, "pm_batch_download: march in flight; call pm_wait first");
+    if (!c->b_ready || view >= c->b_off.size()) return fail(c, PM_ERR_ARG, "pm_batch_download: no view %u", view);
+    HIPCHK(c, hipSetDevice(c->device));
+    uint64_t n = (uint64_t)c->b_w[view] * c->b_h[view], o = c->b_off[view];
+    const float4 *pos = (const float4 *)c->b_img.p;
+    const float4 *nrm = pos + c->b_npx;
+    const float *tt = (const float *)(nrm + c->b_npx);
+    if (positions) HIPCHK(c, copy_sync(c, positions, pos + o, 16 * n, hipMemcpyDeviceToHost));
+    if (normals) HIPCHK(c, copy_sync(c, normals, nrm + o, 16 * n, hipMemcpyDeviceToHost));
+    if (t) HIPCHK(c, copy_sync(c, t, tt + o, 4 * n, hipMemcpyDeviceToHost));
     return PM_OK;
 }
 

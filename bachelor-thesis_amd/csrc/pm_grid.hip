@@ -241,6 +241,66 @@ hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz
     return hipGetLastError();
 }
 
+// Density bound per sub-box.  Every cell of the dense grid is split into
+// 4x4x4 sub-boxes; sub-box l = sx + 4 sy + 16 sz holds the points whose
+// u = inv_h * x lies in [key + s/4, key + (s+1)/4) on every axis.  For the box
+// B widened by delta (which covers the rounding of u and of the march's sample
+// positions), U = sum over the cell's 27-cell stencil of W(dist(x_j, B)) bounds
+// the density at every point of the sub-box, since W falls with r and the
+// stencil holds every particle within h.  Bit l of bound[c] is set when
+// U < iso (1 - 2^-9); the margin covers the rounding of both sums.  Border
+// cells (no stencil, zero density) and cells with an empty 27-neighbourhood get
+// every bit.  One wave per cell: lane l evaluates sub-box l, all lanes read
+// the same particle, and the walk stops once every lane has reached the limit.
+__global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64_t *__restrict__ out)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny, nz = (uint32_t)g.nz;
+    if (c >= nx * ny * nz) return;   // whole wave
+    const int32_t x = (int32_t)(c % nx), y = (int32_t)((c / nx) % ny), z = (int32_t)(c / (nx * ny));
+    if (x < 1 || y < 1 || z < 1 || x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2 || g.empty27[c]) {
+        if (lane == 0) out[c] = ~0ull;
+        return;
+    }
+    const int32_t kx = x + g.ox, ky = y + g.oy, kz = z + g.oz;
+    int32_t km = abs(kx) > abs(ky) ? abs(kx) : abs(ky);
+    km = (km > abs(kz) ? km : abs(kz)) + 1;
+    const float delta = g.h * (0x1p-10f + (float)km * 0x1p-18f);
+    const float sx = (float)(lane & 3u), sy = (float)((lane >> 2) & 3u), sz = (float)(lane >> 4);
+    const float lx = ((float)kx + 0.25f * sx) * g.h - delta, hx = ((float)kx + 0.25f * (sx + 1.0f)) * g.h + delta;
+    const float ly = ((float)ky + 0.25f * sy) * g.h - delta, hy = ((float)ky + 0.25f * (sy + 1.0f)) * g.h + delta;
+    const float lz = ((float)kz + 0.25f * sz) * g.h - delta, hz = ((float)kz + 0.25f * (sz + 1.0f)) * g.h + delta;
+    const float lim = iso * (1.0f - 0x1p-9f);
+    float U = 0.0f;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            const uint32_t row = (uint32_t)x - 1u + nx * ((uint32_t)(y + dy) + ny * (uint32_t)(z + dz));
+            const uint32_t b = g.cell_start[row], e = g.cell_start[row + 3];
+            for (uint32_t j = b; j < e; j++) {
+                const float4 p = g.sorted[j];
+                const float ex = fmaxf(fmaxf(lx - p.x, p.x - hx), 0.0f);
+                const float ey = fmaxf(fmaxf(ly - p.y, p.y - hy), 0.0f);
+                const float ez = fmaxf(fmaxf(lz - p.z, p.z - hz), 0.0f);
+                const float d2 = (ex * ex + ey * ey) + ez * ez;
+                if (d2 < g.h2) U += pm_W_in(d2, g.inv_h, g.sig);
+            }
+            if (__ballot(U < lim) == 0) {
+                if (lane == 0) out[c] = 0ull;
+                return;
+            }
+        }
+    const unsigned long long m = __ballot(U < lim);
+    if (lane == 0) out[c] = m;
+}
+
+hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, hipStream_t st)
+{
+    uint32_t n = (uint32_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(k_bound_grid, dim3((n + 3) / 4), dim3(256), 0, st, g, iso, out);
+    return hipGetLastError();
+}
+
 // Reference DensityGrid node flags (Dataset.cpp:94-164).  Node (x,y,z) covers
 // [bmin + (x,y,z) h, +h); its centre is bmin + ((x,y,z) + 0.5) h.  Two flags:
 //   box  : some particle with |x_j - centre|_inf < h and count * W(0) > 1

@@ -229,6 +229,43 @@ __device__ __forceinline__ int pm_skip_empty(const GridDev &g, float o0, float o
     return kf > (float)r.k ? (int)kf : r.k + 1;
 }
 
+// Bound skip: when the sample's sub-box (4x4x4 per cell) has its bit set in
+// g.bound, no point of the sub-box reaches iso, so the march resumes at the
+// first sample past the sub-box exit; every sample it passes lies in the
+// sub-box (within the widening the bound was computed for) and would have been
+// below iso.  Returns the next sample index (r.k when there is nothing to skip).
+__device__ __forceinline__ float pm_box_exit(float lo, float hi, float o, float d, float texit)
+{
+    if (d == 0.0f) return texit;
+    float ta = ((d > 0.0f ? hi : lo) - o) / d;
+    return ta < texit ? ta : texit;
+}
+
+__device__ __forceinline__ int pm_skip_bound(const GridDev &g, float o0, float o1, float o2, const RayState &r, float qx,
+                                             float qy, float qz, float step)
+{
+    const float u0 = g.inv_h * qx, u1 = g.inv_h * qy, u2 = g.inv_h * qz;
+    const int32_t k0 = qx >= 0.0f ? (int32_t)u0 : (int32_t)u0 - 1;
+    const int32_t k1 = qy >= 0.0f ? (int32_t)u1 : (int32_t)u1 - 1;
+    const int32_t k2 = qz >= 0.0f ? (int32_t)u2 : (int32_t)u2 - 1;
+    const int32_t cx = k0 - g.ox, cy = k1 - g.oy, cz = k2 - g.oz;
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) return r.k;
+    const uint64_t m = g.bound[(uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz)];
+    if (m == 0ull) return r.k;
+    const int s0 = min(3, (int)((u0 - (float)k0) * 4.0f));
+    const int s1 = min(3, (int)((u1 - (float)k1) * 4.0f));
+    const int s2 = min(3, (int)((u2 - (float)k2) * 4.0f));
+    if (!((m >> (s0 + 4 * s1 + 16 * s2)) & 1ull)) return r.k;
+    float texit = INFINITY;
+    texit = pm_box_exit(((float)k0 + 0.25f * (float)s0) * g.h, ((float)k0 + 0.25f * (float)(s0 + 1)) * g.h, o0, r.d0, texit);
+    texit = pm_box_exit(((float)k1 + 0.25f * (float)s1) * g.h, ((float)k1 + 0.25f * (float)(s1 + 1)) * g.h, o1, r.d1, texit);
+    texit = pm_box_exit(((float)k2 + 0.25f * (float)s2) * g.h, ((float)k2 + 0.25f * (float)(s2 + 1)) * g.h, o2, r.d2, texit);
+    float kq = (texit - r.tn) / step;
+    kq -= 0x1p-8f + fabsf(kq) * 0x1p-18f;
+    const float kf = floorf(kq);
+    return kf >= (float)r.k ? (int)kf + 1 : r.k + 1;
+}
+
 __device__ __forceinline__ void to_normals(const pm_settings &s, RayState &r)
 {
     r.phase = s.normal_mode == PM_NORMAL_GRADW ? PH_GRADW : PH_NORMAL;
@@ -259,15 +296,16 @@ __device__ __forceinline__ void start_bracket(const MarchLaunch &L, RayState &r,
     else { set_hit_point(L, r, (r.lo + r.hi) * 0.5f); to_normals(s, r); }
 }
 
-// One density evaluation of this ray, whatever its phase, and the transition
-// it implies.  Dens(q, cand) is the FP32 density.
-template <class Dens>
-__device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens dens, Counts &cn)
+// First half of one step: phases that need no FP32 density (FP64 bisection
+// step, sum-of-gradW normals, the end of the ray) are completed here; otherwise
+// the lane's FP32 density sample point is returned in q and the result is
+// consumed by ray_finish.
+__device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, float &qx, float &qy, float &qz,
+                                            Counts &cn)
 {
     const pm_settings &s = L.s;
     const GridDev &g = L.g;
     const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
-    const int n32 = (s.final_fp64 && s.bisect_iters > 0) ? s.bisect_iters - 1 : s.bisect_iters;
 
     if (r.phase == PH_BISECT64) {
         double tmd = ((double)r.lo + (double)r.hi) * 0.5;
@@ -284,7 +322,7 @@ __device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens
         r.xs2 = (float)((double)o2 + tt * (double)r.d2);
         r.th = (float)tt;
         to_normals(s, r);
-        return;
+        return false;
     }
     if (r.phase == PH_GRADW) {
         float sx, sy, sz, nx, ny, nz;
@@ -296,10 +334,9 @@ __device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens
         if (L.t) L.t[r.idx] = r.th;
         cn.hits++;
         r.phase = PH_DONE;
-        return;
+        return false;
     }
 
-    float qx, qy, qz;
     if (r.phase == PH_LAND || r.phase == PH_BSEARCH) {
         int kq = r.phase == PH_LAND ? r.khi : (r.klo + r.khi) / 2;
         float t = r.tn + (float)kq * s.step;
@@ -310,11 +347,13 @@ __device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens
             if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) {
                 write_miss(L, r.idx);
                 r.phase = PH_DONE;
-                return;
+                return false;
             }
             qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
-            if (s.skip_mode != PM_SKIP_EMPTY) break;
-            int kn = pm_skip_empty(g, o0, o1, o2, r, qx, qy, qz, s.step);
+            int kn;
+            if (s.skip_mode == PM_SKIP_BOUND) kn = pm_skip_bound(g, o0, o1, o2, r, qx, qy, qz, s.step);
+            else if (s.skip_mode == PM_SKIP_EMPTY) kn = pm_skip_empty(g, o0, o1, o2, r, qx, qy, qz, s.step);
+            else break;
             if (kn == r.k) break;
             r.k = kn;
         }
@@ -330,8 +369,15 @@ __device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens
         else qz = (r.ncd & 1) ? r.xs2 - e : r.xs2 + e;
     }
 
-    uint32_t cand = 0;
-    float rho = dens(qx, qy, qz, cand);
+    return true;
+}
+
+// Second half: the sample's density rho (and its stencil candidate count)
+// drives the lane's state machine.
+__device__ __forceinline__ void ray_finish(const MarchLaunch &L, RayState &r, float rho, uint32_t cand, Counts &cn)
+{
+    const pm_settings &s = L.s;
+    const int n32 = (s.final_fp64 && s.bisect_iters > 0) ? s.bisect_iters - 1 : s.bisect_iters;
     cn.samples++;
     cn.cand += cand;
 
@@ -392,6 +438,18 @@ __device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens
     }
 }
 
+// One density evaluation of this ray, whatever its phase, and the transition
+// it implies.  Dens(q, cand) is the FP32 density.
+template <class Dens>
+__device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens dens, Counts &cn)
+{
+    float qx, qy, qz;
+    if (!ray_prepare(L, r, qx, qy, qz, cn)) return;
+    uint32_t cand = 0;
+    float rho = dens(qx, qy, qz, cand);
+    ray_finish(L, r, rho, cand, cn);
+}
+
 struct DensPlain {
     const GridDev *g;
     bool comp;
@@ -412,6 +470,181 @@ struct DensLds {
 };
 
 static constexpr int kLdsCap = 32;
+
+// ---- wave-cooperative density ----
+// The FP32 samples of one step of all 64 lanes are evaluated together.  Four
+// groups of 16 lanes take one sample (its owner lane) each per round, 16
+// rounds in all.  A group walks its owner's 9 stencil rows as one sequence of
+// candidates, 16 consecutive stored particles per step (coalesced float4
+// loads instead of 64 scattered ones), and writes W of each candidate inside
+// the support to the owner's list in LDS; the write position is the owner's
+// count plus the group's ballot prefix, so the list is in canonical order.
+// Each lane then sums its own list (Neumaier, or plain when compensated = 0):
+// the terms and their order are those of pm_density32, and so are the bits.
+// An owner with more than kCoopCap terms gets its density from pm_density32.
+static constexpr int kCoopCap = 32;
+static constexpr int kCoopStride = kCoopCap + 1;   // odd stride: conflict-free appends and drains
+struct CoopWave {
+    float w[64 * kCoopStride];   // list of owner l: w[l * kCoopStride + j]
+    uint32_t d[9][64];           // row r of owner l holds stored indices d[r][l] + f, f in [p[r-1][l], p[r][l])
+    uint32_t p[9][64];           // end of row r in owner l's candidate sequence
+    float q[3][64];
+    uint32_t n[64];              // list length (> kCoopCap: abandoned)
+};
+
+template <int U>
+__device__ __forceinline__ float coop_density(const GridDev &g, bool need, float qx, float qy, float qz, bool comp,
+                                              CoopWave &S, uint32_t &cand)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    int32_t cx = 0, cy = 0, cz = 0;
+    const bool in = need && pm_stencil(g, qx, qy, qz, cx, cy, cz);
+    wave_lds_sync();   // the previous step's readers are done with S
+    uint32_t P = 0;
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        uint32_t b = 0, e = 0;
+        if (in) {
+            const int dy = r % 3 - 1, dz = r / 3 - 1;
+            const uint32_t base = (uint32_t)(cx - 1) + (uint32_t)g.nx * ((uint32_t)(cy + dy) + (uint32_t)g.ny * (uint32_t)(cz + dz));
+            b = g.cell_start[base];
+            e = g.cell_start[base + 3];
+        }
+        S.d[r][lane] = b - P;
+        P += e - b;
+        S.p[r][lane] = P;
+    }
+    cand += P;
+    S.q[0][lane] = qx;
+    S.q[1][lane] = qy;
+    S.q[2][lane] = qz;
+    wave_lds_sync();
+
+    const uint32_t grp = lane >> 4, sub = lane & 15u;
+    const uint32_t below = (1u << sub) - 1u;
+    for (uint32_t round = 0; round < 16; round++) {
+        const uint32_t o = round * 4 + grp;
+        const uint32_t pe = S.p[8][o];
+        if (__ballot(pe > 0) == 0) continue;
+        const float ox = S.q[0][o], oy = S.q[1][o], oz = S.q[2][o];
+        uint32_t dr[9], pr[8];
+#pragma unroll
+        for (int r = 0; r < 9; r++) dr[r] = S.d[r][o];
+#pragma unroll
+        for (int r = 0; r < 8; r++) pr[r] = S.p[r][o];
+        uint32_t n = 0;
+        for (uint32_t f0 = 0; __ballot(f0 < pe) != 0; f0 += 16 * U) {
+            float4 pt[U];
+            bool valid[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t f = f0 + 16 * u + sub;
+                valid[u] = f < pe;
+                uint32_t dd = dr[0];
+#pragma unroll
+                for (int r = 1; r < 9; r++) dd = f >= pr[r - 1] ? dr[r] : dd;
+                if (valid[u]) pt[u] = g.sorted[dd + f];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                float w = 0.0f;
+                bool ins = false;
+                if (valid[u]) {
+                    const float rx = pt[u].x - ox, ry = pt[u].y - oy, rz = pt[u].z - oz;
+                    const float r2 = (rx * rx + ry * ry) + rz * rz;
+                    ins = r2 < g.h2;
+                    if (ins) w = pm_W_in(r2, g.inv_h, g.sig);
+                }
+                const uint32_t gm = (uint32_t)(__ballot(ins) >> (16u * grp)) & 0xffffu;
+                const uint32_t pos = n + (uint32_t)__popc(gm & below);
+                if (ins && pos < (uint32_t)kCoopCap) S.w[o * kCoopStride + pos] = w;
+                n += (uint32_t)__popc(gm);
+            }
+        }
+        if (sub == 0) S.n[o] = n;
+    }
+    wave_lds_sync();
+
+    float s = 0.0f, c = 0.0f;
+    const uint32_t nl = S.n[lane];
+    if (need) {
+        if (nl <= (uint32_t)kCoopCap) {
+            const float *wl = S.w + lane * kCoopStride;
+            for (uint32_t j = 0; j < nl; j++) {
+                if (comp) pm_neu_add(s, c, wl[j]);
+                else s += wl[j];
+            }
+        } else {
+            uint32_t dummy = 0;
+            s = pm_density32(g, qx, qy, qz, comp, dummy);
+            c = 0.0f;
+            return s;
+        }
+    }
+    return comp ? s + c : s;
+}
+
+// Schedule "coop": the persistent schedule of k_march_persist (per-XCD tile
+// bands, lane refill) with every FP32 density of a step evaluated by
+// coop_density.  WAVES waves per workgroup.
+template <int U, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_march_coop(MarchLaunch L, uint32_t *__restrict__ band_ctr)
+{
+    __shared__ CoopWave coop[WAVES];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    CoopWave &S = coop[wave];
+    const uint32_t tiles_x = (L.W + 7) / 8, tiles_y = (L.H + 7) / 8, T = tiles_x * tiles_y;
+    const uint32_t xcd = blockIdx.x & 7u;
+    const bool comp = L.s.compensated != 0;
+
+    Counts cn;
+    RayState r{};
+    r.phase = PH_DONE;
+    uint32_t tile = 0, steal = 0;
+    int qpos = 64;
+    bool exhausted = false;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+
+    while (true) {
+        while (!exhausted) {
+            unsigned long long needy = __ballot(r.phase == PH_DONE);
+            if (needy == 0) break;
+            if (qpos >= 64) {
+                uint32_t t = 0xffffffffu;
+                if (lane == 0) {
+                    while (steal < 8) {
+                        uint32_t b = (xcd + steal) & 7u;
+                        uint32_t lo = (uint32_t)(((uint64_t)T * b) >> 3), hi = (uint32_t)(((uint64_t)T * (b + 1)) >> 3);
+                        uint32_t v = atomicAdd(&band_ctr[b], 1u);
+                        if (v < hi - lo) { t = lo + v; break; }
+                        steal++;
+                    }
+                }
+                t = __shfl(t, 0, 64);
+                steal = __shfl(steal, 0, 64);
+                if (t == 0xffffffffu) { exhausted = true; break; }
+                tile = t;
+                qpos = 0;
+            }
+            int rank = __popcll(needy & lt_mask);
+            int take = __popcll(needy);
+            if (r.phase == PH_DONE && qpos + rank < 64) {
+                uint32_t j = (uint32_t)(qpos + rank);
+                uint32_t px = (tile % tiles_x) * 8 + (j & 7u), py = (tile / tiles_x) * 8 + (j >> 3);
+                if (px < L.W && py < L.H) ray_setup(L, r, px, py, cn);
+            }
+            qpos = qpos + take < 64 ? qpos + take : 64;
+        }
+        if (__ballot(r.phase != PH_DONE) == 0) break;
+        float qx = 0.0f, qy = 0.0f, qz = 0.0f;
+        bool need = false;
+        if (r.phase != PH_DONE) need = ray_prepare(L, r, qx, qy, qz, cn);
+        uint32_t cand = 0;
+        const float rho = coop_density<U>(L.g, need, qx, qy, qz, comp, S, cand);
+        if (need) ray_finish(L, r, rho, cand, cn);
+    }
+    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+}
 
 // Schedule "tile": one 8x8 tile per wave, one ray per lane, all lanes in
 // lockstep until the last ray of the tile is done.
@@ -745,7 +978,7 @@ __global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
                     if (__ballot(r.phase != PH_DONE) != 0) break;    // drain before changing job
                     uint32_t cidx = 0;
                     if (lane == 0) cidx = atomicAdd(B.chunk_ctr, 1u);
-                    cidx = __shfl(cidx, 0, 64);
+                    cidx = __builtin_amdgcn_readfirstlane(__shfl(cidx, 0, 64));
                     if (cidx >= B.nchunks) { exhausted = true; break; }
                     const BatchChunk ch = B.chunks[cidx];
                     const BatchJob &J = B.jobs[ch.job];
@@ -839,6 +1072,19 @@ hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
     if (blocks > need) blocks = (need + 7) & ~7u;
     if (L.s.learned) {
         hipLaunchKernelGGL((k_march_persist<4, true>), dim3(blocks), blk, 0, st, L, L.band_ctr);
+        return hipGetLastError();
+    }
+    if (L.s.variant >= 6 && L.s.variant <= 8) {
+        // two-wave workgroups: 28 KB of LDS each, five per CU
+        uint32_t cb = L.persist_blocks ? L.persist_blocks : 1280;
+        uint32_t cneed = (tiles + 1) / 2;
+        if (cb > cneed) cb = (cneed + 7) & ~7u;
+        dim3 cblk(128);
+        switch (L.s.variant) {
+        case 6: hipLaunchKernelGGL((k_march_coop<1, 2>), dim3(cb), cblk, 0, st, L, L.band_ctr); break;
+        case 7: hipLaunchKernelGGL((k_march_coop<2, 2>), dim3(cb), cblk, 0, st, L, L.band_ctr); break;
+        default: hipLaunchKernelGGL((k_march_coop<4, 2>), dim3(cb), cblk, 0, st, L, L.band_ctr); break;
+        }
         return hipGetLastError();
     }
     switch (L.s.variant) {

@@ -20,7 +20,7 @@ HEADER = ROOT / "include" / "pmarch.h"
 PM_OK, PM_PENDING = 0, 1
 MODE_CONFIG, MODE_REFERENCE = 0, 1
 NORMAL_CDIFF, NORMAL_GRADW = 0, 1
-SKIP_NONE, SKIP_EMPTY, SKIP_DG_BOX, SKIP_DG_CONS = 0, 1, 2, 3
+SKIP_NONE, SKIP_EMPTY, SKIP_DG_BOX, SKIP_DG_CONS, SKIP_BOUND = 0, 1, 2, 3, 4
 MLP_INPUTS, MLP_PARAMS = 24, 6337
 
 
@@ -52,6 +52,11 @@ class PmFrameInfo(C.Structure):
         ("h", C.c_float), ("pmin", C.c_float * 3), ("pmax", C.c_float * 3), ("bmin", C.c_float * 3),
         ("bmax", C.c_float * 3), ("build_ms", C.c_float),
     ]
+
+
+class PmView(C.Structure):
+    _fields_ = [("frame", C.c_int32), ("width", C.c_uint32), ("height", C.c_uint32), ("inv_pv", C.c_float * 16),
+                ("cam", C.c_float * 3)]
 
 
 _lib = None
@@ -99,6 +104,10 @@ def load():
     L.pm_trace_append.argtypes = [vp, i32, C.POINTER(PmSettings), vp, vp, u32, u32, u32, u32, u32, u64, C.POINTER(u64)]
     L.pm_dataset_get.argtypes = [vp, u64, u64, vp, vp, vp, vp]
     L.pm_mlp_train.argtypes = [vp, C.c_int, u32, f32, f32, u64, vp]
+    L.pm_get_bound_grid.argtypes = [vp, i32, f32, vp]
+    L.pm_batch_set.argtypes = [vp, C.POINTER(PmSettings), vp, u32, C.c_int]
+    L.pm_batch_run.argtypes = [vp]
+    L.pm_batch_download.argtypes = [vp, u32, vp, vp, vp]
     _lib = L
     return L
 
@@ -196,6 +205,36 @@ class Marcher:
         st = PmStats()
         self._check(self.L.pm_get_stats(self.ctx, C.byref(st)), "pm_get_stats")
         return st.as_dict()
+
+    def bound_grid(self, frame: int, iso: float) -> np.ndarray:
+        fi = self.frame_info(frame)
+        bits = np.zeros(fi.ncells, np.uint64)
+        self._check(self.L.pm_get_bound_grid(self.ctx, frame, float(iso), _ptr(bits)), "pm_get_bound_grid")
+        return bits
+
+    def batch_set(self, settings: dict, views, use_graph: bool = True):
+        """views: iterable of (frame, W, H, inv_pv, cam)."""
+        vs = list(views)
+        arr = (PmView * len(vs))()
+        for i, (frame, W, H, inv_pv, cam) in enumerate(vs):
+            arr[i].frame, arr[i].width, arr[i].height = int(frame), int(W), int(H)
+            arr[i].inv_pv[:] = [float(x) for x in np.asarray(inv_pv, np.float32).reshape(16)]
+            arr[i].cam[:] = [float(x) for x in np.asarray(cam, np.float32).reshape(3)]
+        s = settings_from(settings)
+        self._check(self.L.pm_batch_set(self.ctx, C.byref(s), C.cast(arr, C.c_void_p), len(vs), int(bool(use_graph))),
+                    "pm_batch_set")
+        self._batch_sizes = [(int(v[1]), int(v[2])) for v in vs]
+
+    def batch_run(self):
+        self._check(self.L.pm_batch_run(self.ctx), "pm_batch_run")
+
+    def batch_download(self, view: int):
+        W, H = self._batch_sizes[view]
+        pos = np.zeros((H, W, 4), np.float32)
+        nrm = np.zeros((H, W, 4), np.float32)
+        t = np.zeros((H, W), np.float32)
+        self._check(self.L.pm_batch_download(self.ctx, view, _ptr(pos), _ptr(nrm), _ptr(t)), "pm_batch_download")
+        return pos, nrm, t
 
     def download(self, W, H):
         pos = np.zeros((H, W, 4), np.float32)

@@ -49,6 +49,10 @@ extern "C" {
 #define PM_SKIP_EMPTY      1  /* configured mode: skip samples whose 27 cells are empty */
 #define PM_SKIP_DG_BOX     2  /* reference mode: DensityGrid, box flag                 */
 #define PM_SKIP_DG_CONS    3  /* reference mode: DensityGrid, conservative flag        */
+#define PM_SKIP_BOUND      4  /* configured mode: skip 4x4x4 sub-boxes of a cell whose
+                                 density bound (sum of W at each particle's distance to
+                                 the sub-box) stays below iso; the image is that of the
+                                 march without the skip                                */
 
 /* March settings.  The first four fields carry VisualizationSettings'
  * MaxSteps / StepSize / IsoDensity (RayMarcher.h:16-18); the rest select the
@@ -123,6 +127,10 @@ int pm_get_grid(pm_ctx *ctx, int32_t frame, uint32_t *cell_start, float *sorted,
  * to read dg_dims. */
 int pm_get_skip_grids(pm_ctx *ctx, int32_t frame, uint8_t *empty27, int32_t dg_dims[3], uint8_t *flag_box,
                       uint8_t *flag_cons, uint32_t *count_box);
+/* The frame's sub-box bound bits for iso (PM_SKIP_BOUND): bits[ncells], bit
+ * l = sx + 4 sy + 16 sz of a cell set when its sub-box stays below iso.  Builds
+ * them if the frame has none for this iso. */
+int pm_get_bound_grid(pm_ctx *ctx, int32_t frame, float iso, uint64_t *bits);
 
 /* RayMarcher::Prepare + Start (RayMarcher.cpp:76-112).  inv_pv is
  * Camera3D::GetInvProjectionView (column-major 16 floats), cam is
@@ -141,6 +149,30 @@ int pm_wait(pm_ctx *ctx);
 /* Copies the last march's device results to host memory (any pointer NULL). */
 int pm_download(pm_ctx *ctx, float *positions, float *normals, float *t);
 int pm_get_stats(pm_ctx *ctx, pm_stats *out);
+
+/* ---- batched views x frames in one persistent launch ----
+ * The reference renders one view per RayMarcher::Start, farmed out pixel by
+ * pixel to its ThreadPool (ThreadPool.cpp:5-55) and re-started once per
+ * second (AdvancedRenderer.cpp:257-298); the batch form marches many views of
+ * uneven size over many resident frames in one persistent kernel.  Waves take
+ * runs of 8x8 tiles of one view from a single queue ordered largest view
+ * first; every view's images equal those of pm_march_async for the same
+ * frame, settings and camera. */
+typedef struct pm_view {
+    int32_t frame;      /* a frame set by pm_set_frame */
+    uint32_t width, height;
+    float inv_pv[16];   /* as in pm_march_async */
+    float cam[3];
+} pm_view;
+/* Fixes the batch (configured march only): allocates the device images of
+ * all views, uploads the tile queue and, with use_graph != 0, captures the
+ * launch (counter fills + the persistent kernel) in a hipGraph. */
+int pm_batch_set(pm_ctx *ctx, const pm_settings *s, const pm_view *views, uint32_t n_views, int use_graph);
+/* Enqueues one run of the batch (graph replay or direct launch) and returns;
+ * complete it with pm_query / pm_wait, which also fill pm_get_stats. */
+int pm_batch_run(pm_ctx *ctx);
+/* Copies one view's device images to host memory (any pointer NULL). */
+int pm_batch_download(pm_ctx *ctx, uint32_t view, float *positions, float *normals, float *t);
 
 /* Dataset::GetNeighbors (Dataset.cpp:272-280) as counts: for each query point,
  * the number of particles with |q - x_j|^2 < h^2 and of stencil candidates. */

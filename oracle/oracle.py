@@ -66,6 +66,8 @@ class GridStruct(C.Structure):
         ("dg_flag_cons", C.POINTER(C.c_uint8)),
         ("dg_count_box", C.POINTER(C.c_uint32)),
         ("empty27", C.POINTER(C.c_uint8)),
+        ("bound", C.POINTER(C.c_uint64)),
+        ("bound_iso", C.c_float),
     ]
 
 
@@ -91,6 +93,7 @@ def lib():
         L.or_grid_struct_size.restype = C.c_uint64
         L.or_grid_build.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float, C.POINTER(GridStruct)]
         L.or_grid_free.argtypes = [C.POINTER(GridStruct)]
+        L.or_bound_build.argtypes = [C.POINTER(GridStruct), C.c_float]
         L.or_neighbors_brute.restype = C.c_uint32
         L.or_neighbors_brute.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float,
                                          fp(np.float32, flags="C"), fp(np.uint32, flags="C"), C.c_uint32]
@@ -199,6 +202,14 @@ class Grid:
                 np.ctypeslib.as_array(self.g.dg_flag_box, shape=(n,)).copy(),
                 np.ctypeslib.as_array(self.g.dg_flag_cons, shape=(n,)).copy())
 
+    def bound(self, iso: float) -> np.ndarray:
+        """Sub-box bound bits for iso (builds them when the grid holds none for iso)."""
+        iso = float(np.float32(iso))
+        if not self.g.bound or float(np.float32(self.g.bound_iso)) != iso:
+            if lib().or_bound_build(C.byref(self.g), iso) != 0:
+                raise RuntimeError("or_bound_build failed")
+        return np.ctypeslib.as_array(self.g.bound, shape=(self.g.ncells,)).copy()
+
     def empty27(self) -> np.ndarray:
         return np.ctypeslib.as_array(self.g.empty27, shape=(self.g.ncells,)).copy()
 
@@ -277,6 +288,8 @@ def march(grid: Grid, settings: Settings, inv_pv, cam, W: int, H: int, depth=Non
     if depth is not None:
         depth = np.ascontiguousarray(depth, np.float32)
         dptr = depth.ctypes.data
+    if settings.skip_mode == 4:
+        grid.bound(settings.iso)
     cnt = Counters()
     rc = lib().or_march(C.byref(grid.g), C.byref(settings), inv_pv, cam, W, H, dptr, row_step, threads,
                         pos.reshape(-1), nrm.reshape(-1), t.ctypes.data if want_t else None, C.byref(cnt))

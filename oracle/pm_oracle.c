@@ -187,7 +187,7 @@ int or_grid_build(const float *xyz, uint64_t n, float h, or_grid *g)
 void or_grid_free(or_grid *g)
 {
     free(g->cell_start); free(g->sorted); free(g->sorted_index);
-    free(g->dg_flag_box); free(g->dg_flag_cons); free(g->dg_count_box); free(g->empty27);
+    free(g->dg_flag_box); free(g->dg_flag_cons); free(g->dg_count_box); free(g->empty27); free(g->bound);
     memset(g, 0, sizeof(*g));
 }
 
@@ -531,6 +531,96 @@ static int64_t skip_empty(const or_grid *g, const float o[3], const float d[3], 
     return kn;
 }
 
+/* Sub-box density bound (configured-mode skip, no counterpart in the
+ * reference).  Cell c is split into 4x4x4 sub-boxes; sub-box (sx,sy,sz) holds
+ * the points whose u = inv_h * x lies in [key + s/4, key + (s+1)/4) on every
+ * axis.  With B that box widened by delta = h (2^-10 + (max|key| + 1) 2^-18),
+ * U = sum over the cell's 27-cell stencil of W(dist(x_j, B)) bounds the density
+ * of every point inside (W falls with r; the stencil holds every particle
+ * within h).  The bit is set when U < iso (1 - 2^-9).  Border cells and cells
+ * with an empty 27-neighbourhood get every bit.  U is summed in stencil row
+ * order (dz, dy outer; particles in storage order), as the GPU build does. */
+int or_bound_build(or_grid *g, float iso)
+{
+    free(g->bound);
+    g->bound = (uint64_t *)calloc(g->ncells ? g->ncells : 1, sizeof(uint64_t));
+    if (!g->bound) return -1;
+    g->bound_iso = iso;
+    const float lim = iso * (1.0f - 0x1p-9f), h = g->h, h2 = h * h, sig = or_sigma_f(h);
+    for (int32_t z = 0; z < g->dims[2]; z++)
+        for (int32_t y = 0; y < g->dims[1]; y++)
+            for (int32_t x = 0; x < g->dims[0]; x++) {
+                uint64_t c = cell_lin(g, x, y, z);
+                if (x < 1 || y < 1 || z < 1 || x > g->dims[0] - 2 || y > g->dims[1] - 2 || z > g->dims[2] - 2 ||
+                    g->empty27[c]) {
+                    g->bound[c] = ~0ull;
+                    continue;
+                }
+                int32_t k[3] = { x + g->origin[0], y + g->origin[1], z + g->origin[2] };
+                int32_t km = 0;
+                for (int a = 0; a < 3; a++) km = abs(k[a]) > km ? abs(k[a]) : km;
+                float delta = h * (0x1p-10f + (float)(km + 1) * 0x1p-18f);
+                uint64_t bits = 0;
+                for (int l = 0; l < 64; l++) {
+                    float sub[3] = { (float)(l & 3), (float)((l >> 2) & 3), (float)(l >> 4) }, lo[3], hi[3];
+                    for (int a = 0; a < 3; a++) {
+                        lo[a] = ((float)k[a] + 0.25f * sub[a]) * h - delta;
+                        hi[a] = ((float)k[a] + 0.25f * (sub[a] + 1.0f)) * h + delta;
+                    }
+                    float U = 0.0f;
+                    for (int dz = -1; dz <= 1 && U < lim; dz++)
+                        for (int dy = -1; dy <= 1 && U < lim; dy++) {
+                            uint64_t row = cell_lin(g, x - 1, y + dy, z + dz);
+                            for (uint32_t j = g->cell_start[row]; j < g->cell_start[row + 3]; j++) {
+                                const float *p = g->sorted + 4 * (uint64_t)j;
+                                float e[3];
+                                for (int a = 0; a < 3; a++) {
+                                    float v = lo[a] - p[a], w = p[a] - hi[a];
+                                    float m = v > w ? v : w;
+                                    e[a] = m > 0.0f ? m : 0.0f;
+                                }
+                                float d2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+                                U += W_f(d2, h2, g->inv_h, sig);
+                            }
+                        }
+                    if (U < lim) bits |= 1ull << l;
+                }
+                g->bound[c] = bits;
+            }
+    return 0;
+}
+
+/* Bound skip of the configured march: the sample's sub-box has its bit set, so
+ * the march resumes at the first sample past the sub-box exit (the exit index
+ * lowered by 2^-8 + |k| 2^-18 steps against rounding).  Returns the next sample
+ * index (k when there is nothing to skip). */
+static int64_t skip_bound(const or_grid *g, const float o[3], const float d[3], const float x[3], float tn,
+                          float step, int64_t k)
+{
+    int32_t key[3], c[3], sub[3];
+    for (int a = 0; a < 3; a++) {
+        float u = g->inv_h * x[a];
+        key[a] = x[a] >= 0.0f ? (int32_t)u : (int32_t)u - 1;
+        c[a] = key[a] - g->origin[a];
+        if (c[a] < 0 || c[a] >= g->dims[a]) return k;
+        int s4 = (int)((u - (float)key[a]) * 4.0f);
+        sub[a] = s4 < 3 ? s4 : 3;
+    }
+    uint64_t m = g->bound[cell_lin(g, c[0], c[1], c[2])];
+    if (!((m >> (sub[0] + 4 * sub[1] + 16 * sub[2])) & 1ull)) return k;
+    float texit = INFINITY;
+    for (int a = 0; a < 3; a++) {
+        if (d[a] == 0.0f) continue;
+        float lo = ((float)key[a] + 0.25f * (float)sub[a]) * g->h, hi = ((float)key[a] + 0.25f * (float)(sub[a] + 1)) * g->h;
+        float ta = ((d[a] > 0.0f ? hi : lo) - o[a]) / d[a];
+        if (ta < texit) texit = ta;
+    }
+    float kq = (texit - tn) / step;
+    kq -= 0x1p-8f + fabsf(kq) * 0x1p-18f;
+    float kf = floorf(kq);
+    return kf >= (float)k ? (int64_t)kf + 1 : k + 1;
+}
+
 /* QueryDensityGrid (Dataset.cpp:26-47): returns the node id or -1. */
 static int64_t dg_query(const or_grid *g, const float p[3], float nmin[3], float nmax[3])
 {
@@ -733,8 +823,8 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
         float t = tn + (float)k * step;
         if (t > tf) break;
         float x[3] = { o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2] };
-        if (s->skip_mode == 1) {
-            int64_t kn = skip_empty(g, o, d, x, tn, step, k);
+        if (s->skip_mode == 1 || s->skip_mode == 4) {
+            int64_t kn = s->skip_mode == 1 ? skip_empty(g, o, d, x, tn, step, k) : skip_bound(g, o, d, x, tn, step, k);
             if (kn != k) { k = kn; continue; }
         }
         float rho = dens_f(dc, x, cnt);
@@ -926,6 +1016,7 @@ static int run_march(const or_grid *g, const or_hash *hs, const or_settings *s, 
     if (!g || !s || !M || !cam || !pos || !nrm || W == 0 || H == 0) return -1;
     if (s->mode == OR_MODE_REFERENCE && !depth) return -1;
     if (!(s->step > 0.0f)) return -1;
+    if (s->skip_mode == 4 && (!g->bound || g->bound_iso != s->iso)) return -2;   /* or_bound_build first */
     if (row_step == 0) row_step = 1;
     march_job j;
     memset(&j, 0, sizeof(j));

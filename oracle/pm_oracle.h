@@ -35,7 +35,8 @@ typedef struct or_settings {
     int32_t normal_mode;   /* 0 = central difference of rho, 1 = sum of gradW     */
     int32_t skip_mode;     /* 0 none; 1 empty-stencil skip (configured mode);
                               2 DensityGrid, box flag; 3 DensityGrid, conservative
-                              flag (reference mode)                               */
+                              flag (reference mode); 4 sub-box bound skip
+                              (configured mode, needs or_bound_build)             */
     int32_t compensated;   /* 1 = Neumaier-compensated FP32 density sums          */
 } or_settings;
 
@@ -72,6 +73,10 @@ typedef struct or_grid {
     /* 1 where the cell and its 26 neighbours hold no particle: every point of
      * the cell has zero density (empty-space skip of the configured march)   */
     uint8_t  *empty27;
+    /* sub-box density bounds for one iso (or_bound_build): bit l = sx + 4 sy +
+     * 16 sz of bound[c] set when sub-box l of cell c stays below iso         */
+    uint64_t *bound;
+    float     bound_iso;
 } or_grid;
 
 /* ---- SPH cubic spline kernel (Kernel.cpp:8-52) ---- */
@@ -85,6 +90,9 @@ int32_t or_key(float x, float inv_h);
 int  or_grid_build(const float *xyz, uint64_t n, float h, or_grid *g);
 void or_grid_free(or_grid *g);
 uint64_t or_grid_struct_size(void);
+
+/* Sub-box density bounds for iso (skip_mode 4); replaces any earlier bits. */
+int  or_bound_build(or_grid *g, float iso);
 
 /* ---- neighbour search ---- */
 uint32_t or_neighbors_brute(const float *xyz, uint64_t n, float h, const float q[3],

@@ -94,17 +94,42 @@ def test_march_lattice_cube(marcher, seed, cam, variant):
         assert stats[k] == ocnt[k], (k, stats[k], ocnt[k])
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_march_variants_identical(marcher, variant):
-    """Every kernel schedule produces the same image as the default one."""
+    """Every kernel schedule produces the same image as the default one, with
+    and without the bound skip and with plain (uncompensated) sums."""
     sc = scenes.dam_break(1, nx=40, ny=80, nz=40, width=320, height=240)
     c = sc.cameras[0]
+    print(f"variant {variant}", flush=True)
     marcher.set_frame(4, sc.xyz, sc.h)
-    ref = marcher.march(4, sc.settings(), c.inv_pv, c.eye, c.width, c.height)
-    got = marcher.march(4, sc.settings(variant=variant), c.inv_pv, c.eye, c.width, c.height)
-    for a, b in zip(ref[:3], got[:3]):
-        assert np.array_equal(a, b)
-    assert ref[3]["samples"] == got[3]["samples"] and ref[3]["cand"] == got[3]["cand"]
+    for kw in (dict(), dict(skip_mode=4), dict(compensated=0)):
+        ref = marcher.march(4, sc.settings(**kw), c.inv_pv, c.eye, c.width, c.height)
+        got = marcher.march(4, sc.settings(variant=variant, **kw), c.inv_pv, c.eye, c.width, c.height)
+        for a, b in zip(ref[:3], got[:3]):
+            assert np.array_equal(a, b), kw
+        assert ref[3]["samples"] == got[3]["samples"] and ref[3]["cand"] == got[3]["cand"], kw
+
+
+def test_coop_variant_dense_overflow(marcher):
+    """The cooperative walk's fallback: a scene dense enough that many samples
+    hold more than 32 particles in their support gives the same image."""
+    rng = np.random.default_rng(9)
+    xyz = (rng.normal(0.0, 0.03, (60000, 3)) + 0.5).astype(np.float32)
+    h = 0.02
+    sc = scenes.tiny_scene(xyz, h, eye=(0.5, 0.6, -0.4), target=(0.5, 0.5, 0.5), width=160, height=120)
+    c = sc.cameras[0]
+    marcher.set_frame(5, sc.xyz, sc.h)
+    # iso 8x the lattice value puts the surface where ~130 particles lie within h
+    st = sc.settings(iso=sc.iso * 8)
+    n, _ = marcher.neighbor_counts(5, np.array([[0.5, 0.5 + 0.072, 0.5]], np.float32))
+    print("particles within h at the surface", n[0], flush=True)
+    assert n[0] > 64
+    for v in (6, 7, 8):
+        ref = marcher.march(5, st, c.inv_pv, c.eye, c.width, c.height)
+        got = marcher.march(5, dict(st, variant=v), c.inv_pv, c.eye, c.width, c.height)
+        for a, b in zip(ref[:3], got[:3]):
+            assert np.array_equal(a, b), v
+        assert ref[3]["hits"] > 1000
 
 
 def test_march_vs_fp64(marcher):
