@@ -67,7 +67,7 @@ struct pm_ctx {
     std::map<int32_t, Frame *> frames;
     std::string err;
     // outputs of the last march
-    DevBuf d_pos, d_nrm, d_t, d_depth, d_counters, d_q, d_out, d_band;
+    DevBuf d_pos, d_nrm, d_t, d_depth, d_counters, d_q, d_out, d_band, d_tiles;
     uint64_t npx = 0;
     bool have_t = false;
     // pinned staging for asynchronous hand-off
@@ -176,7 +176,7 @@ void pm_destroy(pm_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto &kv : c->frames) { kv.second->release(); delete kv.second; }
     c->d_pos.release(); c->d_nrm.release(); c->d_t.release(); c->d_depth.release();
-    c->d_counters.release(); c->d_q.release(); c->d_out.release(); c->d_band.release();
+    c->d_counters.release(); c->d_q.release(); c->d_out.release(); c->d_band.release(); c->d_tiles.release();
     c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release(); c->mlp_W1.release(); c->mlp_W2.release();
     c->mlp_W2T.release(); c->mlp_partial.release(); c->mlp_out.release(); c->mlp_x.release(); c->mlp_idx.release();
     c->ds_feat.release(); c->ds_label.release(); c->ds_hit.release(); c->ds_count.release(); c->ds_meta.release();
@@ -397,7 +397,7 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
         return fail(c, PM_ERR_ARG, "the DensityGrid skip belongs to the reference march");
     if (s->mode == PM_MODE_REFERENCE && (s->skip_mode == PM_SKIP_EMPTY || s->skip_mode == PM_SKIP_BOUND))
         return fail(c, PM_ERR_ARG, "the empty-stencil and bound skips belong to the configured march");
-    if (s->variant < 0 || s->variant > 8) return fail(c, PM_ERR_ARG, "unknown kernel variant %d", s->variant);
+    if (s->variant < 0 || s->variant > 14) return fail(c, PM_ERR_ARG, "unknown kernel variant %d", s->variant);
     if (s->learned && !c->mlp_ready) return fail(c, PM_ERR_STATE, "learned step: no network loaded (pm_mlp_set)");
     if (s->learned && s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "the learned step belongs to the configured march");
     if (s->learned && !(s->learned_alpha >= 0.0f && s->learned_alpha <= 4.0f)) return fail(c, PM_ERR_ARG, "learned_alpha out of [0,4]");
@@ -452,6 +452,11 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     L.t = (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
     L.band_ctr = (uint32_t *)c->d_band.p;
+    if (s->mode == PM_MODE_CONFIG && s->variant >= 13) {
+        uint64_t tiles = (uint64_t)((W + 7) / 8) * ((H + 7) / 8);
+        HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (2 * tiles + 1040)));
+        L.tile_buf = (uint32_t *)c->d_tiles.p;
+    }
     L.cg = f->cg;
     if (s->learned) {
         L.net = mlp_dev(c);

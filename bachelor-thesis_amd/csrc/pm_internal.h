@@ -38,6 +38,9 @@ struct MarchLaunch {
     float mlp_alpha;           // fraction of the predicted distance that is skipped
     int32_t mlp_min_skip;      // below this many steps the ray continues step by step
     int32_t mlp_max_skip;      // longest jump, in steps
+    // tile order (longest first inside each XCD band): scratch of 2 T + 1040 u32;
+    // null = row-major order
+    uint32_t *tile_buf;
 };
 
 // Batched launch: many (view, frame) jobs in one persistent kernel.

@@ -525,7 +525,10 @@ __device__ __forceinline__ float coop_density(const GridDev &g, bool need, float
     for (uint32_t round = 0; round < 16; round++) {
         const uint32_t o = round * 4 + grp;
         const uint32_t pe = S.p[8][o];
-        if (__ballot(pe > 0) == 0) continue;
+        if (__ballot(pe > 0) == 0) {          // four empty stencils: their lists are empty too
+            if (sub == 0) S.n[o] = 0;
+            continue;
+        }
         const float ox = S.q[0][o], oy = S.q[1][o], oz = S.q[2][o];
         uint32_t dr[9], pr[8];
 #pragma unroll
@@ -737,11 +740,14 @@ __device__ __noinline__ void wave_propose(const MarchLaunch &L, RayState &r, Cou
 // once no tile is left and its lanes are idle.
 // LEARNED adds the step proposals; its LDS region per wave holds either the
 // density compaction buffer or the network's activations (never both at once).
-template <int UNROLL, bool LEARNED>
-__global__ __launch_bounds__(256) void k_march_persist(MarchLaunch L, uint32_t *__restrict__ band_ctr)
+// CAP: compaction slots per lane; WPE: minimum waves per SIMD the register
+// allocation must allow (1 = no constraint).
+template <int UNROLL, bool LEARNED, int CAP = kLdsCap, int WPE = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_march_persist(
+    MarchLaunch L, uint32_t *__restrict__ band_ctr)
 {
     // floats per wave: the compaction buffer, or X[64][32] + H[32][64] bf16 + Y[64]
-    constexpr int kRegion = LEARNED ? 2112 : kLdsCap * 64;
+    constexpr int kRegion = LEARNED ? 2112 : CAP * 64;
     __shared__ float lds[4 * kRegion];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float *region = lds + wave * kRegion;
@@ -770,7 +776,7 @@ __global__ __launch_bounds__(256) void k_march_persist(MarchLaunch L, uint32_t *
                         uint32_t b = (xcd + steal) & 7u;
                         uint32_t lo = (uint32_t)(((uint64_t)T * b) >> 3), hi = (uint32_t)(((uint64_t)T * (b + 1)) >> 3);
                         uint32_t v = atomicAdd(&band_ctr[b], 1u);
-                        if (v < hi - lo) { t = lo + v; break; }
+                        if (v < hi - lo) { t = L.tile_buf ? L.tile_buf[T + 1040 + lo + v] : lo + v; break; }
                         steal++;
                     }
                 }
@@ -799,7 +805,7 @@ __global__ __launch_bounds__(256) void k_march_persist(MarchLaunch L, uint32_t *
                 wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
         }
         if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
-            if (use_lds) ray_step(L, r, DensLds<kLdsCap, UNROLL>{ &L.g, buf }, cn);
+            if (use_lds) ray_step(L, r, DensLds<CAP, UNROLL>{ &L.g, buf }, cn);
             else ray_step(L, r, DensPlain{ &L.g, false }, cn);
         }
     }
@@ -812,7 +818,60 @@ __global__ __launch_bounds__(256) void k_march_persist(MarchLaunch L, uint32_t *
 
 __global__ void k_zero_u32(uint32_t *p, int n)
 {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) p[i] = 0u;
+}
+
+// Band b of the T tiles is [T b / 8, T (b+1) / 8).
+__device__ __forceinline__ uint32_t band_lo(uint32_t T, uint32_t b) { return (uint32_t)(((uint64_t)T * b) >> 3); }
+
+__device__ __forceinline__ uint32_t band_of(uint32_t T, uint32_t t)
+{
+    uint32_t b = 7;
+    while (b > 0 && band_lo(T, b) > t) b--;
+    return b;
+}
+
+// Tile order of the persistent schedule, longest-processing-time first inside
+// each band: cost = rays of the tile that enter the padded AABB (0..64).
+// tile_buf: cost[T] | hist[8][65] | offs[8][65] | order[T].
+__global__ __launch_bounds__(256) void k_tile_cost(MarchLaunch L, uint32_t T)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t tiles_x = (L.W + 7) / 8;
+    uint32_t n = 0;
+    for (uint32_t j = 0; j < 64; j++) {
+        const uint32_t px = (t % tiles_x) * 8 + (j & 7u), py = (t / tiles_x) * 8 + (j >> 3);
+        if (px >= L.W || py >= L.H) continue;
+        const float cx = (float)px * (2.0f / (float)L.W) - 1.0f;
+        const float cy = (float)py * (2.0f / (float)L.H) - 1.0f;
+        float wx, wy, wz, d[3], tn, tf;
+        pm_unproject(L.M, cx, cy, 1.0f, wx, wy, wz);
+        pm_normalize(wx - L.cam[0], wy - L.cam[1], wz - L.cam[2], d[0], d[1], d[2]);
+        const float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
+        if (pm_slab(o, d, L.g.bmin, L.g.bmax, tn, tf)) n++;
+    }
+    L.tile_buf[t] = n;
+    atomicAdd(&L.tile_buf[T + band_of(T, t) * 65 + (64 - n)], 1u);
+}
+
+__global__ void k_tile_scan(uint32_t *tile_buf, uint32_t T)
+{
+    const uint32_t b = threadIdx.x;
+    if (b >= 8) return;
+    uint32_t acc = band_lo(T, b);
+    for (int k = 0; k < 65; k++) {
+        tile_buf[T + 520 + b * 65 + k] = acc;
+        acc += tile_buf[T + b * 65 + k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tile_scatter(uint32_t *tile_buf, uint32_t T)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t pos = atomicAdd(&tile_buf[T + 520 + band_of(T, t) * 65 + (64 - tile_buf[t])], 1u);
+    tile_buf[T + 1040 + pos] = t;
 }
 
 // QueryDensityGrid (Dataset.cpp:26-47) followed by the Flag test: true when p
@@ -1066,6 +1125,12 @@ hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
     default: break;
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L.band_ctr, 8);
+    if (L.tile_buf) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, st, L.tile_buf + tiles, 520);
+        hipLaunchKernelGGL(k_tile_cost, dim3((tiles + 255) / 256), dim3(256), 0, st, L, tiles);
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(64), 0, st, L.tile_buf, tiles);
+        hipLaunchKernelGGL(k_tile_scatter, dim3((tiles + 255) / 256), dim3(256), 0, st, L.tile_buf, tiles);
+    }
     // enough resident waves to fill the chip; extra waves find no tile and exit
     uint32_t blocks = L.persist_blocks ? L.persist_blocks : 1024;
     uint32_t need = (tiles + 3) / 4;
@@ -1087,9 +1152,28 @@ hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
         }
         return hipGetLastError();
     }
+    if (L.s.variant >= 9 && L.s.variant <= 12) {
+        // occupancy variants: fewer compaction slots and a register cap
+        const uint32_t per_cu[4] = { 5, 6, 6, 8 };
+        uint32_t ob = L.persist_blocks ? L.persist_blocks : 256u * per_cu[L.s.variant - 9];
+        if (ob > need) ob = (need + 7) & ~7u;
+        switch (L.s.variant) {
+        case 9: hipLaunchKernelGGL((k_march_persist<4, false, 32, 5>), dim3(ob), blk, 0, st, L, L.band_ctr); break;
+        case 10: hipLaunchKernelGGL((k_march_persist<4, false, 16, 6>), dim3(ob), blk, 0, st, L, L.band_ctr); break;
+        case 11: hipLaunchKernelGGL((k_march_persist<8, false, 16, 6>), dim3(ob), blk, 0, st, L, L.band_ctr); break;
+        default: hipLaunchKernelGGL((k_march_persist<4, false, 16, 8>), dim3(ob), blk, 0, st, L, L.band_ctr); break;
+        }
+        return hipGetLastError();
+    }
     switch (L.s.variant) {
     case 2: hipLaunchKernelGGL((k_march_persist<1, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
     case 3: hipLaunchKernelGGL((k_march_persist<8, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
+    case 14: {   // longest-first tile order with five waves per SIMD
+        uint32_t ob = L.persist_blocks ? L.persist_blocks : 1280u;
+        if (ob > need) ob = (need + 7) & ~7u;
+        hipLaunchKernelGGL((k_march_persist<4, false, 32, 5>), dim3(ob), blk, 0, st, L, L.band_ctr);
+        break;
+    }
     default: hipLaunchKernelGGL((k_march_persist<4, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
     }
     return hipGetLastError();
