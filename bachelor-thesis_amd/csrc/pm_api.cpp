@@ -45,6 +45,7 @@ struct Frame {
     GridDev g{};
     DevBuf sorted, cell_start, cell_of, counts, cursor, scan_tmp, xyz, bounds_partial, bounds;
     DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound;
+    DevBuf fsorted, fcell_start;    // march grid (cells of h/2)
     bool bound_ok = false;      // bound holds the sub-box bits for bound_iso
     float bound_iso = 0.0f;
     int32_t dg_dims[3] = { 0, 0, 0 };
@@ -55,6 +56,7 @@ struct Frame {
         sorted.release(); cell_start.release(); cell_of.release(); counts.release(); cursor.release();
         scan_tmp.release(); xyz.release(); bounds_partial.release(); bounds.release();
         empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release(); bound.release();
+        fsorted.release(); fcell_start.release();
     }
 };
 
@@ -237,13 +239,30 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     }
     if (nc >= (1ull << 31)) return fail(c, PM_ERR_SIZE, "pm_set_frame: grid of %llu cells exceeds 2^31", (unsigned long long)nc);
     I.ncells = nc;
+    // march grid: cells of h/2 (inv 2/h, exact) and four empty cells of
+    // margin, so the 5x5x5 fine stencil of any point within h of a particle
+    // stays inside it
+    const float inv_hf = 2.0f * inv_h;
+    uint64_t fnc = 1;
+    for (int ax = 0; ax < 3; ax++) {
+        int32_t kmin = host_key(I.pmin[ax], inv_hf), kmax = host_key(I.pmax[ax], inv_hf);
+        I.forigin[ax] = kmin - 4;
+        I.fdims[ax] = kmax - kmin + 9;
+        fnc *= (uint64_t)I.fdims[ax];
+    }
+    if (fnc >= (1ull << 31)) return fail(c, PM_ERR_SIZE, "pm_set_frame: march grid of %llu cells exceeds 2^31", (unsigned long long)fnc);
+    I.fncells = fnc;
 
+    // scratch of both builds sized once, before either is queued
+    const uint64_t mc = fnc > nc ? fnc : nc;
     HIPCHK(c, f->sorted.ensure(sizeof(float4) * n));
     HIPCHK(c, f->cell_start.ensure(sizeof(uint32_t) * (nc + 1)));
+    HIPCHK(c, f->fsorted.ensure(sizeof(float4) * n));
+    HIPCHK(c, f->fcell_start.ensure(sizeof(uint32_t) * (fnc + 1)));
     HIPCHK(c, f->cell_of.ensure(sizeof(uint32_t) * n));
-    HIPCHK(c, f->counts.ensure(sizeof(uint32_t) * (nc + 1)));
-    HIPCHK(c, f->cursor.ensure(sizeof(uint32_t) * nc));
-    HIPCHK(c, f->scan_tmp.ensure(sizeof(uint32_t) * scan_tmp_size((uint32_t)(nc + 1))));
+    HIPCHK(c, f->counts.ensure(sizeof(uint32_t) * (mc + 1)));
+    HIPCHK(c, f->cursor.ensure(sizeof(uint32_t) * mc));
+    HIPCHK(c, f->scan_tmp.ensure(sizeof(uint32_t) * scan_tmp_size((uint32_t)(mc + 1))));
 
     GridBuildArgs a{};
     a.xyz = d_src; a.n = n; a.inv_h = inv_h;
@@ -257,6 +276,18 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     a.sorted = (float4 *)f->sorted.p;
     a.cell_start = (uint32_t *)f->cell_start.p;
     HIPCHK(c, launch_grid_build(a, c->stream));
+
+    GridBuildArgs fa = a;   // the march grid: same build, cells of h/2
+    fa.inv_h = inv_hf;
+    fa.ox = I.forigin[0]; fa.oy = I.forigin[1]; fa.oz = I.forigin[2];
+    fa.nx = I.fdims[0]; fa.ny = I.fdims[1]; fa.nz = I.fdims[2];
+    fa.ncells = (uint32_t)fnc;
+    fa.counts = (uint32_t *)f->counts.p;
+    fa.cursor = (uint32_t *)f->cursor.p;
+    fa.scan_tmp = (uint32_t *)f->scan_tmp.p;
+    fa.sorted = (float4 *)f->fsorted.p;
+    fa.cell_start = (uint32_t *)f->fcell_start.p;
+    HIPCHK(c, launch_grid_build(fa, c->stream));
     HIPCHK(c, hipEventRecord(e1, c->stream));
     HIPCHK(c, hipEventSynchronize(e1));
     float ms = 0.f;
@@ -279,6 +310,11 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     g.hinv_d = 1.0 / g.h_d;
     g.sig_d = 8.0 / (PM_PI_D * g.h_d * g.h_d * g.h_d);
     for (int a2 = 0; a2 < 3; a2++) { g.bmin[a2] = I.bmin[a2]; g.bmax[a2] = I.bmax[a2]; }
+    g.fsorted = (const float4 *)f->fsorted.p;
+    g.fcell_start = (const uint32_t *)f->fcell_start.p;
+    g.fox = I.forigin[0]; g.foy = I.forigin[1]; g.foz = I.forigin[2];
+    g.fnx = I.fdims[0]; g.fny = I.fdims[1]; g.fnz = I.fdims[2];
+    g.inv_hf = inv_hf;
 
     // skip structures: empty 27-neighbourhoods and the reference DensityGrid
     HIPCHK(c, f->empty27.ensure(nc));
@@ -397,7 +433,7 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
         return fail(c, PM_ERR_ARG, "the DensityGrid skip belongs to the reference march");
     if (s->mode == PM_MODE_REFERENCE && (s->skip_mode == PM_SKIP_EMPTY || s->skip_mode == PM_SKIP_BOUND))
         return fail(c, PM_ERR_ARG, "the empty-stencil and bound skips belong to the configured march");
-    if (s->variant < 0 || s->variant > 14) return fail(c, PM_ERR_ARG, "unknown kernel variant %d", s->variant);
+    if (s->variant < 0 || s->variant > 7) return fail(c, PM_ERR_ARG, "unknown kernel variant %d", s->variant);
     if (s->learned && !c->mlp_ready) return fail(c, PM_ERR_STATE, "learned step: no network loaded (pm_mlp_set)");
     if (s->learned && s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "the learned step belongs to the configured march");
     if (s->learned && !(s->learned_alpha >= 0.0f && s->learned_alpha <= 4.0f)) return fail(c, PM_ERR_ARG, "learned_alpha out of [0,4]");
@@ -444,6 +480,7 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     L.g = f->g;
     L.g.dg_flag = (const uint8_t *)(s->skip_mode == PM_SKIP_DG_BOX ? f->dg_box.p : f->dg_cons.p);
     L.s = *s;
+    L.inv_step = 1.0f / s->step;
     memcpy(L.M, inv_pv, sizeof(L.M));
     memcpy(L.cam, cam, sizeof(L.cam));
     L.W = W; L.H = H;
@@ -452,7 +489,7 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     L.t = (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
     L.band_ctr = (uint32_t *)c->d_band.p;
-    if (s->mode == PM_MODE_CONFIG && s->variant >= 13) {
+    if (s->mode == PM_MODE_CONFIG && s->variant == 7) {
         uint64_t tiles = (uint64_t)((W + 7) / 8) * ((H + 7) / 8);
         HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (2 * tiles + 1040)));
         L.tile_buf = (uint32_t *)c->d_tiles.p;
@@ -574,6 +611,23 @@ static int batch_enqueue_launch(pm_ctx *c)
     return PM_OK;
 }
 
+int pm_get_march_grid(pm_ctx *c, int32_t frame, uint32_t *cell_start, float *sorted)
+{
+    if (!c) return PM_ERR_ARG;
+    if (c->pending) return fail(c, PM_ERR_STATE, "pm_get_march_grid: a march is in flight");
+    Frame *f = get_frame(c, frame);
+    if (!f) return fail(c, PM_ERR_FRAME, "frame %d not set", frame);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (cell_start)
+        HIPCHK(c, copy_sync(c, cell_start, f->fcell_start.p, sizeof(uint32_t) * (f->info.fncells + 1), hipMemcpyDeviceToHost));
+    if (sorted) {
+        uint64_t n = f->info.n;
+        HIPCHK(c, copy_sync(c, sorted, f->fsorted.p, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; i++) sorted[4 * i + 3] = 0.0f;
+    }
+    return PM_OK;
+}
+
 int pm_get_bound_grid(pm_ctx *c, int32_t frame, float iso, uint64_t *bits)
 {
     if (!c || !bits) return PM_ERR_ARG;
@@ -667,6 +721,7 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
     BatchLaunch &B = c->b_launch;
     B = BatchLaunch{};
     B.base.s = *s;
+    B.base.inv_step = 1.0f / s->step;
     B.base.counters = (unsigned long long *)c->d_counters.p;
     if (s->learned) {
         B.base.net = mlp_dev(c);

@@ -41,6 +41,7 @@ struct MarchLaunch {
     // tile order (longest first inside each XCD band): scratch of 2 T + 1040 u32;
     // null = row-major order
     uint32_t *tile_buf;
+    float inv_step;            // 1 / s.step (bound skip)
 };
 
 // Batched launch: many (view, frame) jobs in one persistent kernel.

@@ -69,16 +69,16 @@ __device__ __forceinline__ bool pm_slab(const float o[3], const float d[3], cons
 __device__ __forceinline__ void pm_sum_gradw(const GridDev &g, float qx, float qy, float qz, float &gx, float &gy,
                                              float &gz, uint32_t &cand)
 {
-    int32_t cx, cy, cz;
+    FineWalk fw;
     gx = gy = gz = 0.0f;
-    if (!pm_stencil(g, qx, qy, qz, cx, cy, cz)) return;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++) {
-            uint32_t base = (uint32_t)(cx - 1) + (uint32_t)g.nx * ((uint32_t)(cy + dy) + (uint32_t)g.ny * (uint32_t)(cz + dz));
-            uint32_t b = g.cell_start[base], e = g.cell_start[base + 3];
+    if (!pm_fine_begin(g, qx, qy, qz, fw)) return;
+    for (int dz = -2; dz <= 2; dz++)
+        for (int dy = -2; dy <= 2; dy++) {
+            uint32_t b, e;
+            pm_fine_row(g, fw, dy, dz, pm_fine_tz2(fw, dz), b, e);
             cand += e - b;
             for (uint32_t i = b; i < e; i++) {
-                float4 p = g.sorted[i];
+                float4 p = g.fsorted[i];
                 float r0 = p.x - qx, r1 = p.y - qy, r2c = p.z - qz;
                 float rn = (r0 * r0 + r1 * r1) + r2c * r2c;
                 if (rn < g.h2) {
@@ -158,6 +158,7 @@ struct RayState {
     uint32_t idx;
     int klo, khi;         // learned step: last sample below iso / landing sample
     int lrn;              // 1 while the ray still takes proposals
+    float id0, id1, id2;  // 1 / direction (bound skip)
 };
 
 struct Counts {
@@ -185,6 +186,7 @@ __device__ __forceinline__ void ray_setup(const MarchLaunch &L, RayState &r, uin
     float d[3];
     pm_normalize(wx - L.cam[0], wy - L.cam[1], wz - L.cam[2], d[0], d[1], d[2]);
     r.d0 = d[0]; r.d1 = d[1]; r.d2 = d[2];
+    r.id0 = 1.0f / d[0]; r.id1 = 1.0f / d[1]; r.id2 = 1.0f / d[2];
     float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
     if (pm_slab(o, d, L.g.bmin, L.g.bmax, r.tn, r.tf)) {
         r.phase = PH_MARCH;
@@ -234,15 +236,15 @@ __device__ __forceinline__ int pm_skip_empty(const GridDev &g, float o0, float o
 // first sample past the sub-box exit; every sample it passes lies in the
 // sub-box (within the widening the bound was computed for) and would have been
 // below iso.  Returns the next sample index (r.k when there is nothing to skip).
-__device__ __forceinline__ float pm_box_exit(float lo, float hi, float o, float d, float texit)
+__device__ __forceinline__ float pm_box_exit(float lo, float hi, float o, float d, float id, float texit)
 {
     if (d == 0.0f) return texit;
-    float ta = ((d > 0.0f ? hi : lo) - o) / d;
+    float ta = ((d > 0.0f ? hi : lo) - o) * id;
     return ta < texit ? ta : texit;
 }
 
 __device__ __forceinline__ int pm_skip_bound(const GridDev &g, float o0, float o1, float o2, const RayState &r, float qx,
-                                             float qy, float qz, float step)
+                                             float qy, float qz, float inv_step)
 {
     const float u0 = g.inv_h * qx, u1 = g.inv_h * qy, u2 = g.inv_h * qz;
     const int32_t k0 = qx >= 0.0f ? (int32_t)u0 : (int32_t)u0 - 1;
@@ -257,10 +259,10 @@ __device__ __forceinline__ int pm_skip_bound(const GridDev &g, float o0, float o
     const int s2 = min(3, (int)((u2 - (float)k2) * 4.0f));
     if (!((m >> (s0 + 4 * s1 + 16 * s2)) & 1ull)) return r.k;
     float texit = INFINITY;
-    texit = pm_box_exit(((float)k0 + 0.25f * (float)s0) * g.h, ((float)k0 + 0.25f * (float)(s0 + 1)) * g.h, o0, r.d0, texit);
-    texit = pm_box_exit(((float)k1 + 0.25f * (float)s1) * g.h, ((float)k1 + 0.25f * (float)(s1 + 1)) * g.h, o1, r.d1, texit);
-    texit = pm_box_exit(((float)k2 + 0.25f * (float)s2) * g.h, ((float)k2 + 0.25f * (float)(s2 + 1)) * g.h, o2, r.d2, texit);
-    float kq = (texit - r.tn) / step;
+    texit = pm_box_exit(((float)k0 + 0.25f * (float)s0) * g.h, ((float)k0 + 0.25f * (float)(s0 + 1)) * g.h, o0, r.d0, r.id0, texit);
+    texit = pm_box_exit(((float)k1 + 0.25f * (float)s1) * g.h, ((float)k1 + 0.25f * (float)(s1 + 1)) * g.h, o1, r.d1, r.id1, texit);
+    texit = pm_box_exit(((float)k2 + 0.25f * (float)s2) * g.h, ((float)k2 + 0.25f * (float)(s2 + 1)) * g.h, o2, r.d2, r.id2, texit);
+    float kq = (texit - r.tn) * inv_step;
     kq -= 0x1p-8f + fabsf(kq) * 0x1p-18f;
     const float kf = floorf(kq);
     return kf >= (float)r.k ? (int)kf + 1 : r.k + 1;
@@ -342,7 +344,9 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
         float t = r.tn + (float)kq * s.step;
         qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
     } else if (r.phase == PH_MARCH) {
-        for (;;) {
+        // at most 8 skips per step: a lane crossing much skippable space then
+        // idles for a step instead of holding the whole wave in this loop
+        for (int hop = 0;; hop++) {
             float t = r.tn + (float)r.k * s.step;
             if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) {
                 write_miss(L, r.idx);
@@ -351,11 +355,12 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
             }
             qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
             int kn;
-            if (s.skip_mode == PM_SKIP_BOUND) kn = pm_skip_bound(g, o0, o1, o2, r, qx, qy, qz, s.step);
+            if (s.skip_mode == PM_SKIP_BOUND) kn = pm_skip_bound(g, o0, o1, o2, r, qx, qy, qz, L.inv_step);
             else if (s.skip_mode == PM_SKIP_EMPTY) kn = pm_skip_empty(g, o0, o1, o2, r, qx, qy, qz, s.step);
             else break;
             if (kn == r.k) break;
             r.k = kn;
+            if (hop == 7) return false;
         }
     } else if (r.phase == PH_BISECT) {
         float tm = (r.lo + r.hi) * 0.5f;
@@ -471,184 +476,6 @@ struct DensLds {
 
 static constexpr int kLdsCap = 32;
 
-// ---- wave-cooperative density ----
-// The FP32 samples of one step of all 64 lanes are evaluated together.  Four
-// groups of 16 lanes take one sample (its owner lane) each per round, 16
-// rounds in all.  A group walks its owner's 9 stencil rows as one sequence of
-// candidates, 16 consecutive stored particles per step (coalesced float4
-// loads instead of 64 scattered ones), and writes W of each candidate inside
-// the support to the owner's list in LDS; the write position is the owner's
-// count plus the group's ballot prefix, so the list is in canonical order.
-// Each lane then sums its own list (Neumaier, or plain when compensated = 0):
-// the terms and their order are those of pm_density32, and so are the bits.
-// An owner with more than kCoopCap terms gets its density from pm_density32.
-static constexpr int kCoopCap = 32;
-static constexpr int kCoopStride = kCoopCap + 1;   // odd stride: conflict-free appends and drains
-struct CoopWave {
-    float w[64 * kCoopStride];   // list of owner l: w[l * kCoopStride + j]
-    uint32_t d[9][64];           // row r of owner l holds stored indices d[r][l] + f, f in [p[r-1][l], p[r][l])
-    uint32_t p[9][64];           // end of row r in owner l's candidate sequence
-    float q[3][64];
-    uint32_t n[64];              // list length (> kCoopCap: abandoned)
-};
-
-template <int U>
-__device__ __forceinline__ float coop_density(const GridDev &g, bool need, float qx, float qy, float qz, bool comp,
-                                              CoopWave &S, uint32_t &cand)
-{
-    const uint32_t lane = threadIdx.x & 63;
-    int32_t cx = 0, cy = 0, cz = 0;
-    const bool in = need && pm_stencil(g, qx, qy, qz, cx, cy, cz);
-    wave_lds_sync();   // the previous step's readers are done with S
-    uint32_t P = 0;
-#pragma unroll
-    for (int r = 0; r < 9; r++) {
-        uint32_t b = 0, e = 0;
-        if (in) {
-            const int dy = r % 3 - 1, dz = r / 3 - 1;
-            const uint32_t base = (uint32_t)(cx - 1) + (uint32_t)g.nx * ((uint32_t)(cy + dy) + (uint32_t)g.ny * (uint32_t)(cz + dz));
-            b = g.cell_start[base];
-            e = g.cell_start[base + 3];
-        }
-        S.d[r][lane] = b - P;
-        P += e - b;
-        S.p[r][lane] = P;
-    }
-    cand += P;
-    S.q[0][lane] = qx;
-    S.q[1][lane] = qy;
-    S.q[2][lane] = qz;
-    wave_lds_sync();
-
-    const uint32_t grp = lane >> 4, sub = lane & 15u;
-    const uint32_t below = (1u << sub) - 1u;
-    for (uint32_t round = 0; round < 16; round++) {
-        const uint32_t o = round * 4 + grp;
-        const uint32_t pe = S.p[8][o];
-        if (__ballot(pe > 0) == 0) {          // four empty stencils: their lists are empty too
-            if (sub == 0) S.n[o] = 0;
-            continue;
-        }
-        const float ox = S.q[0][o], oy = S.q[1][o], oz = S.q[2][o];
-        uint32_t dr[9], pr[8];
-#pragma unroll
-        for (int r = 0; r < 9; r++) dr[r] = S.d[r][o];
-#pragma unroll
-        for (int r = 0; r < 8; r++) pr[r] = S.p[r][o];
-        uint32_t n = 0;
-        for (uint32_t f0 = 0; __ballot(f0 < pe) != 0; f0 += 16 * U) {
-            float4 pt[U];
-            bool valid[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const uint32_t f = f0 + 16 * u + sub;
-                valid[u] = f < pe;
-                uint32_t dd = dr[0];
-#pragma unroll
-                for (int r = 1; r < 9; r++) dd = f >= pr[r - 1] ? dr[r] : dd;
-                if (valid[u]) pt[u] = g.sorted[dd + f];
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                float w = 0.0f;
-                bool ins = false;
-                if (valid[u]) {
-                    const float rx = pt[u].x - ox, ry = pt[u].y - oy, rz = pt[u].z - oz;
-                    const float r2 = (rx * rx + ry * ry) + rz * rz;
-                    ins = r2 < g.h2;
-                    if (ins) w = pm_W_in(r2, g.inv_h, g.sig);
-                }
-                const uint32_t gm = (uint32_t)(__ballot(ins) >> (16u * grp)) & 0xffffu;
-                const uint32_t pos = n + (uint32_t)__popc(gm & below);
-                if (ins && pos < (uint32_t)kCoopCap) S.w[o * kCoopStride + pos] = w;
-                n += (uint32_t)__popc(gm);
-            }
-        }
-        if (sub == 0) S.n[o] = n;
-    }
-    wave_lds_sync();
-
-    float s = 0.0f, c = 0.0f;
-    const uint32_t nl = S.n[lane];
-    if (need) {
-        if (nl <= (uint32_t)kCoopCap) {
-            const float *wl = S.w + lane * kCoopStride;
-            for (uint32_t j = 0; j < nl; j++) {
-                if (comp) pm_neu_add(s, c, wl[j]);
-                else s += wl[j];
-            }
-        } else {
-            uint32_t dummy = 0;
-            s = pm_density32(g, qx, qy, qz, comp, dummy);
-            c = 0.0f;
-            return s;
-        }
-    }
-    return comp ? s + c : s;
-}
-
-// Schedule "coop": the persistent schedule of k_march_persist (per-XCD tile
-// bands, lane refill) with every FP32 density of a step evaluated by
-// coop_density.  WAVES waves per workgroup.
-template <int U, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_march_coop(MarchLaunch L, uint32_t *__restrict__ band_ctr)
-{
-    __shared__ CoopWave coop[WAVES];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    CoopWave &S = coop[wave];
-    const uint32_t tiles_x = (L.W + 7) / 8, tiles_y = (L.H + 7) / 8, T = tiles_x * tiles_y;
-    const uint32_t xcd = blockIdx.x & 7u;
-    const bool comp = L.s.compensated != 0;
-
-    Counts cn;
-    RayState r{};
-    r.phase = PH_DONE;
-    uint32_t tile = 0, steal = 0;
-    int qpos = 64;
-    bool exhausted = false;
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
-
-    while (true) {
-        while (!exhausted) {
-            unsigned long long needy = __ballot(r.phase == PH_DONE);
-            if (needy == 0) break;
-            if (qpos >= 64) {
-                uint32_t t = 0xffffffffu;
-                if (lane == 0) {
-                    while (steal < 8) {
-                        uint32_t b = (xcd + steal) & 7u;
-                        uint32_t lo = (uint32_t)(((uint64_t)T * b) >> 3), hi = (uint32_t)(((uint64_t)T * (b + 1)) >> 3);
-                        uint32_t v = atomicAdd(&band_ctr[b], 1u);
-                        if (v < hi - lo) { t = lo + v; break; }
-                        steal++;
-                    }
-                }
-                t = __shfl(t, 0, 64);
-                steal = __shfl(steal, 0, 64);
-                if (t == 0xffffffffu) { exhausted = true; break; }
-                tile = t;
-                qpos = 0;
-            }
-            int rank = __popcll(needy & lt_mask);
-            int take = __popcll(needy);
-            if (r.phase == PH_DONE && qpos + rank < 64) {
-                uint32_t j = (uint32_t)(qpos + rank);
-                uint32_t px = (tile % tiles_x) * 8 + (j & 7u), py = (tile / tiles_x) * 8 + (j >> 3);
-                if (px < L.W && py < L.H) ray_setup(L, r, px, py, cn);
-            }
-            qpos = qpos + take < 64 ? qpos + take : 64;
-        }
-        if (__ballot(r.phase != PH_DONE) == 0) break;
-        float qx = 0.0f, qy = 0.0f, qz = 0.0f;
-        bool need = false;
-        if (r.phase != PH_DONE) need = ray_prepare(L, r, qx, qy, qz, cn);
-        uint32_t cand = 0;
-        const float rho = coop_density<U>(L.g, need, qx, qy, qz, comp, S, cand);
-        if (need) ray_finish(L, r, rho, cand, cn);
-    }
-    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
-}
-
 // Schedule "tile": one 8x8 tile per wave, one ray per lane, all lanes in
 // lockstep until the last ray of the tile is done.
 template <bool LDS, int UNROLL>
@@ -672,14 +499,6 @@ __global__ __launch_bounds__(256) void k_march_tile(MarchLaunch L)
     flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
 }
 
-// Schedule 0: persistent waves.  The image is cut into 8x8 tiles and the tiles
-// into 8 contiguous bands, one per XCD (workgroups are dispatched round-robin
-// over the XCDs, so blockIdx % 8 names the XCD); a wave takes tiles from its
-// band's counter and, once the band is drained, from the other bands.  A lane
-// whose ray is finished takes the next ray of the wave's current tile before
-// the next density evaluation, so all 64 lanes stay busy until the image runs
-// out.  Rays that miss the AABB are retired during refill.  Every wave exits
-// once no tile is left and its lanes are idle.
 // A step proposal for every lane in PH_PROPOSE: features at the lane's current
 // sample (one stencil pass for rho and grad rho), one MLP evaluation for the
 // whole wave (BF16 MFMA), and the jump to the proposed sample on the same
@@ -796,15 +615,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             qpos = qpos + take < 64 ? qpos + take : 64;
         }
         if (__ballot(r.phase != PH_DONE) == 0) break;
-        if (LEARNED) {
-            // Proposals are batched: the network runs once at least half of the busy
-            // lanes wait for one (or all of them do); meanwhile waiting lanes sit out.
-            const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
-            const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
-            if (nwant > 0 && (2 * nwant >= nbusy || exhausted))
-                wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
-        }
-        if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
+        // Two kinds of step are batched across the wave, since the wave pays for
+        // a kind whenever any lane needs it: the FP64 bisection step (a double
+        // precision stencil walk) runs once a quarter of the busy lanes wait for
+        // it, the network once half of them wait for a proposal; whenever every
+        // busy lane waits, both run.  Waiting lanes sit out.
+        const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
+        const int n64 = __popcll(__ballot(r.phase == PH_BISECT64));
+        const int nwant = LEARNED ? __popcll(__ballot(r.phase == PH_PROPOSE)) : 0;
+        const bool stalled = n64 + nwant == nbusy;
+        if (LEARNED && nwant > 0 && (2 * nwant >= nbusy || exhausted || stalled))
+            wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+        const bool run64 = 4 * n64 >= nbusy || stalled;
+        if (r.phase != PH_DONE && r.phase != PH_PROPOSE && (r.phase != PH_BISECT64 || run64)) {
             if (use_lds) ray_step(L, r, DensLds<CAP, UNROLL>{ &L.g, buf }, cn);
             else ray_step(L, r, DensPlain{ &L.g, false }, cn);
         }
@@ -1068,13 +891,15 @@ __global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
             if (exhausted) break;
             continue;
         }
-        if (LEARNED) {
-            const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
-            const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
-            if (nwant > 0 && (2 * nwant >= nbusy || exhausted || (qpos >= 64 && tile_i >= ntiles)))
-                wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
-        }
-        if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
+        // batched FP64 steps and proposals, as in k_march_persist
+        const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
+        const int n64 = __popcll(__ballot(r.phase == PH_BISECT64));
+        const int nwant = LEARNED ? __popcll(__ballot(r.phase == PH_PROPOSE)) : 0;
+        const bool stalled = n64 + nwant == nbusy;
+        if (LEARNED && nwant > 0 && (2 * nwant >= nbusy || exhausted || stalled || (qpos >= 64 && tile_i >= ntiles)))
+            wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+        const bool run64 = 4 * n64 >= nbusy || stalled;
+        if (r.phase != PH_DONE && r.phase != PH_PROPOSE && (r.phase != PH_BISECT64 || run64)) {
             if (use_lds) ray_step(L, r, DensLds<kLdsCap, 4>{ &L.g, buf }, cn);
             else ray_step(L, r, DensPlain{ &L.g, false }, cn);
         }
@@ -1139,36 +964,10 @@ hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
         hipLaunchKernelGGL((k_march_persist<4, true>), dim3(blocks), blk, 0, st, L, L.band_ctr);
         return hipGetLastError();
     }
-    if (L.s.variant >= 6 && L.s.variant <= 8) {
-        // two-wave workgroups: 28 KB of LDS each, five per CU
-        uint32_t cb = L.persist_blocks ? L.persist_blocks : 1280;
-        uint32_t cneed = (tiles + 1) / 2;
-        if (cb > cneed) cb = (cneed + 7) & ~7u;
-        dim3 cblk(128);
-        switch (L.s.variant) {
-        case 6: hipLaunchKernelGGL((k_march_coop<1, 2>), dim3(cb), cblk, 0, st, L, L.band_ctr); break;
-        case 7: hipLaunchKernelGGL((k_march_coop<2, 2>), dim3(cb), cblk, 0, st, L, L.band_ctr); break;
-        default: hipLaunchKernelGGL((k_march_coop<4, 2>), dim3(cb), cblk, 0, st, L, L.band_ctr); break;
-        }
-        return hipGetLastError();
-    }
-    if (L.s.variant >= 9 && L.s.variant <= 12) {
-        // occupancy variants: fewer compaction slots and a register cap
-        const uint32_t per_cu[4] = { 5, 6, 6, 8 };
-        uint32_t ob = L.persist_blocks ? L.persist_blocks : 256u * per_cu[L.s.variant - 9];
-        if (ob > need) ob = (need + 7) & ~7u;
-        switch (L.s.variant) {
-        case 9: hipLaunchKernelGGL((k_march_persist<4, false, 32, 5>), dim3(ob), blk, 0, st, L, L.band_ctr); break;
-        case 10: hipLaunchKernelGGL((k_march_persist<4, false, 16, 6>), dim3(ob), blk, 0, st, L, L.band_ctr); break;
-        case 11: hipLaunchKernelGGL((k_march_persist<8, false, 16, 6>), dim3(ob), blk, 0, st, L, L.band_ctr); break;
-        default: hipLaunchKernelGGL((k_march_persist<4, false, 16, 8>), dim3(ob), blk, 0, st, L, L.band_ctr); break;
-        }
-        return hipGetLastError();
-    }
     switch (L.s.variant) {
     case 2: hipLaunchKernelGGL((k_march_persist<1, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
     case 3: hipLaunchKernelGGL((k_march_persist<8, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
-    case 14: {   // longest-first tile order with five waves per SIMD
+    case 6: {   // registers capped for five waves per SIMD
         uint32_t ob = L.persist_blocks ? L.persist_blocks : 1280u;
         if (ob > need) ob = (need + 7) & ~7u;
         hipLaunchKernelGGL((k_march_persist<4, false, 32, 5>), dim3(ob), blk, 0, st, L, L.band_ctr);

@@ -50,7 +50,8 @@ class PmFrameInfo(C.Structure):
     _fields_ = [
         ("n", C.c_uint64), ("ncells", C.c_uint64), ("origin", C.c_int32 * 3), ("dims", C.c_int32 * 3),
         ("h", C.c_float), ("pmin", C.c_float * 3), ("pmax", C.c_float * 3), ("bmin", C.c_float * 3),
-        ("bmax", C.c_float * 3), ("build_ms", C.c_float),
+        ("bmax", C.c_float * 3), ("build_ms", C.c_float), ("forigin", C.c_int32 * 3), ("fdims", C.c_int32 * 3),
+        ("fncells", C.c_uint64),
     ]
 
 
@@ -105,6 +106,7 @@ def load():
     L.pm_dataset_get.argtypes = [vp, u64, u64, vp, vp, vp, vp]
     L.pm_mlp_train.argtypes = [vp, C.c_int, u32, f32, f32, u64, vp]
     L.pm_get_bound_grid.argtypes = [vp, i32, f32, vp]
+    L.pm_get_march_grid.argtypes = [vp, i32, vp, vp]
     L.pm_batch_set.argtypes = [vp, C.POINTER(PmSettings), vp, u32, C.c_int]
     L.pm_batch_run.argtypes = [vp]
     L.pm_batch_download.argtypes = [vp, u32, vp, vp, vp]
@@ -205,6 +207,13 @@ class Marcher:
         st = PmStats()
         self._check(self.L.pm_get_stats(self.ctx, C.byref(st)), "pm_get_stats")
         return st.as_dict()
+
+    def march_grid(self, frame: int):
+        fi = self.frame_info(frame)
+        cs = np.zeros(fi.fncells + 1, np.uint32)
+        srt = np.zeros((fi.n, 4), np.float32)
+        self._check(self.L.pm_get_march_grid(self.ctx, frame, _ptr(cs), _ptr(srt)), "pm_get_march_grid")
+        return fi, cs, srt
 
     def bound_grid(self, frame: int, iso: float) -> np.ndarray:
         fi = self.frame_info(frame)

@@ -69,8 +69,12 @@ typedef struct pm_settings {
     int32_t skip_mode;     /* PM_SKIP_*                                               */
     int32_t compensated;   /* 1 = Neumaier-compensated FP32 density sums              */
     int32_t learned;       /* 1 = learned step proposal (needs pm_set_mlp)            */
-    int32_t variant;       /* kernel schedule: 0 = persistent waves with ray refill,
-                              1 = one 8x8 tile per wave (same results, for A/B)       */
+    int32_t variant;       /* kernel schedule (same results, for A/B): 0 = persistent
+                              waves with ray refill; 1 = one 8x8 tile per wave, direct
+                              sums; 2, 3 = persistent with 1 or 8 loads in flight;
+                              4, 5 = one tile per wave with compaction; 6 = persistent
+                              with registers capped for 5 waves per SIMD; 7 = persistent
+                              with tiles handed out longest-first                      */
     float   learned_alpha; /* learned: fraction of the predicted distance skipped (0 = 0.7) */
     int32_t learned_min_skip; /* learned: shorter proposals end the proposals (0 = 4)  */
     int32_t learned_max_skip; /* learned: longest jump in steps (0 = 24)              */
@@ -101,6 +105,10 @@ typedef struct pm_frame_info {
     float    pmin[3], pmax[3];  /* particle bounds                           */
     float    bmin[3], bmax[3];  /* padded AABB (Frame::ComputeAABB, Dataset.cpp:78-92) */
     float    build_ms;      /* grid build time on the GPU                    */
+    /* march grid: cells of h/2, same key rule with 2/h, 4 cells of margin  */
+    int32_t  forigin[3];
+    int32_t  fdims[3];
+    uint64_t fncells;
 } pm_frame_info;
 
 typedef struct pm_ctx pm_ctx;
@@ -120,6 +128,9 @@ int pm_frame_info_get(pm_ctx *ctx, int32_t frame, pm_frame_info *out);
 /* The frame's grid: cell_start[ncells+1], sorted[4n] (x,y,z,0), index[n]
  * (input index of each stored particle).  Any pointer may be NULL. */
 int pm_get_grid(pm_ctx *ctx, int32_t frame, uint32_t *cell_start, float *sorted, uint32_t *index);
+/* The frame's march grid (cells of h/2): cell_start[fncells+1], sorted[4n]
+ * (x,y,z,0) fine-cell-major, by input index inside a cell.  Either may be NULL. */
+int pm_get_march_grid(pm_ctx *ctx, int32_t frame, uint32_t *cell_start, float *sorted);
 /* The frame's skip structures: empty27[ncells] (1 where a cell and its 26
  * neighbours are empty) and the reference DensityGrid (Frame::BuildDensityGrid,
  * Dataset.cpp:94-164) over the padded AABB: dims, box flag, conservative flag

@@ -68,6 +68,12 @@ class GridStruct(C.Structure):
         ("empty27", C.POINTER(C.c_uint8)),
         ("bound", C.POINTER(C.c_uint64)),
         ("bound_iso", C.c_float),
+        ("inv_hf", C.c_float),
+        ("forigin", C.c_int32 * 3),
+        ("fdims", C.c_int32 * 3),
+        ("fncells", C.c_uint64),
+        ("fcell_start", C.POINTER(C.c_uint32)),
+        ("fsorted", C.POINTER(C.c_float)),
     ]
 
 
@@ -94,6 +100,9 @@ def lib():
         L.or_grid_build.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float, C.POINTER(GridStruct)]
         L.or_grid_free.argtypes = [C.POINTER(GridStruct)]
         L.or_bound_build.argtypes = [C.POINTER(GridStruct), C.c_float]
+        L.or_neighbors_fine.restype = C.c_uint32
+        L.or_neighbors_fine.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), fp(np.float32, flags="C"),
+                                        C.c_uint32, C.POINTER(C.c_uint32)]
         L.or_neighbors_brute.restype = C.c_uint32
         L.or_neighbors_brute.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float,
                                          fp(np.float32, flags="C"), fp(np.uint32, flags="C"), C.c_uint32]
@@ -219,6 +228,22 @@ class Grid:
         cand = C.c_uint32(0)
         n = lib().or_neighbors_grid(C.byref(self.g), q, out, out.size, C.byref(cand))
         return out[:n].copy(), int(cand.value)
+
+    def march_grid(self):
+        """(origin, dims, cell_start, sorted) of the march grid (cells of h/2)."""
+        n = int(self.g.fncells)
+        return (tuple(self.g.forigin), tuple(self.g.fdims),
+                np.ctypeslib.as_array(self.g.fcell_start, shape=(n + 1,)).copy(),
+                np.ctypeslib.as_array(self.g.fsorted, shape=(self.g.n, 4)).copy())
+
+    def neighbors_fine(self, q) -> tuple[np.ndarray, int]:
+        """Positions of the particles within h of q, in the march grid's order,
+        and the number of candidates the trimmed fine stencil visits."""
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.zeros(3 * (1 << 15), np.float32)
+        cand = C.c_uint32(0)
+        n = lib().or_neighbors_fine(C.byref(self.g), q, out, 1 << 15, C.byref(cand))
+        return out[:3 * n].reshape(-1, 3).copy(), int(cand.value)
 
     def density(self, q, compensated=True, with_cand=False):
         q = np.ascontiguousarray(np.asarray(q, np.float32).reshape(-1, 3))

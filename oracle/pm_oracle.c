@@ -123,6 +123,52 @@ static inline int stencil_centre(const or_grid *g, float x, float y, float z, in
 static void densitygrid_build(or_grid *g);
 static void empty27_build(or_grid *g);
 
+/* Stable counting sort of the particles into cells of key rule inv (cell
+ * width 1/inv): histogram, exclusive scan, scatter in input order.  pad empty
+ * cells of margin on each side of the particles' key range. */
+static int sort_cells(const float *xyz, uint64_t n, float inv, const float pmin[3], const float pmax[3], int pad,
+                      int32_t origin[3], int32_t dims[3], uint64_t *ncells, uint32_t **cell_start, float **sorted,
+                      uint32_t **sorted_index)
+{
+    uint64_t nc = 1;
+    for (int a = 0; a < 3; a++) {
+        int32_t kmin = or_key(pmin[a], inv), kmax = or_key(pmax[a], inv);
+        origin[a] = kmin - pad;
+        dims[a] = kmax - kmin + 2 * pad + 1;
+        nc *= (uint64_t)dims[a];
+    }
+    if (nc > (1ull << 31)) return -2;
+    *ncells = nc;
+    uint32_t *cs = (uint32_t *)calloc(nc + 1, sizeof(uint32_t));
+    float *srt = (float *)malloc(4 * n * sizeof(float));
+    uint32_t *sidx = sorted_index ? (uint32_t *)malloc(n * sizeof(uint32_t)) : NULL;
+    uint32_t *cell = (uint32_t *)malloc(n * sizeof(uint32_t));
+    uint32_t *cursor = (uint32_t *)malloc(nc * sizeof(uint32_t));
+    *cell_start = cs;
+    *sorted = srt;
+    if (sorted_index) *sorted_index = sidx;
+    if (!cs || !srt || (sorted_index && !sidx) || !cell || !cursor) { free(cell); free(cursor); return -3; }
+    for (uint64_t i = 0; i < n; i++) {
+        int32_t c[3];
+        for (int a = 0; a < 3; a++) c[a] = or_key(xyz[3 * i + a], inv) - origin[a];
+        cell[i] = (uint32_t)((uint64_t)c[0] + (uint64_t)dims[0] * ((uint64_t)c[1] + (uint64_t)dims[1] * (uint64_t)c[2]));
+        cs[cell[i] + 1]++;
+    }
+    for (uint64_t c = 0; c < nc; c++) cs[c + 1] += cs[c];
+    memcpy(cursor, cs, nc * sizeof(uint32_t));
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t slot = cursor[cell[i]]++;
+        srt[4 * slot + 0] = xyz[3 * i + 0];
+        srt[4 * slot + 1] = xyz[3 * i + 1];
+        srt[4 * slot + 2] = xyz[3 * i + 2];
+        srt[4 * slot + 3] = 0.0f;
+        if (sidx) sidx[slot] = (uint32_t)i;
+    }
+    free(cursor);
+    free(cell);
+    return 0;
+}
+
 int or_grid_build(const float *xyz, uint64_t n, float h, or_grid *g)
 {
     memset(g, 0, sizeof(*g));
@@ -143,42 +189,14 @@ int or_grid_build(const float *xyz, uint64_t n, float h, or_grid *g)
 
     /* Keys are monotone in x, so the key range follows from the bounds.  Two
      * empty cells of margin on each side keep every stencil of a point inside
-     * the padded AABB within the grid. */
-    uint64_t nc = 1;
-    for (int a = 0; a < 3; a++) {
-        int32_t kmin = or_key(g->pmin[a], g->inv_h), kmax = or_key(g->pmax[a], g->inv_h);
-        g->origin[a] = kmin - 2;
-        g->dims[a] = kmax - kmin + 5;
-        nc *= (uint64_t)g->dims[a];
-    }
-    if (nc > (1ull << 31)) return -2;
-    g->ncells = nc;
-    g->cell_start = (uint32_t *)calloc(nc + 1, sizeof(uint32_t));
-    g->sorted = (float *)malloc(4 * n * sizeof(float));
-    g->sorted_index = (uint32_t *)malloc(n * sizeof(uint32_t));
-    uint32_t *cell = (uint32_t *)malloc(n * sizeof(uint32_t));
-    if (!g->cell_start || !g->sorted || !g->sorted_index || !cell) { free(cell); or_grid_free(g); return -3; }
-
-    /* Stable counting sort: histogram, exclusive scan, scatter in input order. */
-    for (uint64_t i = 0; i < n; i++) {
-        int32_t c[3];
-        for (int a = 0; a < 3; a++) c[a] = or_key(xyz[3 * i + a], g->inv_h) - g->origin[a];
-        cell[i] = (uint32_t)cell_lin(g, c[0], c[1], c[2]);
-        g->cell_start[cell[i] + 1]++;
-    }
-    for (uint64_t c = 0; c < nc; c++) g->cell_start[c + 1] += g->cell_start[c];
-    uint32_t *cursor = (uint32_t *)malloc(nc * sizeof(uint32_t));
-    memcpy(cursor, g->cell_start, nc * sizeof(uint32_t));
-    for (uint64_t i = 0; i < n; i++) {
-        uint32_t slot = cursor[cell[i]]++;
-        g->sorted[4 * slot + 0] = xyz[3 * i + 0];
-        g->sorted[4 * slot + 1] = xyz[3 * i + 1];
-        g->sorted[4 * slot + 2] = xyz[3 * i + 2];
-        g->sorted[4 * slot + 3] = 0.0f;
-        g->sorted_index[slot] = (uint32_t)i;
-    }
-    free(cursor);
-    free(cell);
+     * the padded AABB within the grid (four fine cells for the march grid). */
+    int rc = sort_cells(xyz, n, g->inv_h, g->pmin, g->pmax, 2, g->origin, g->dims, &g->ncells, &g->cell_start,
+                        &g->sorted, &g->sorted_index);
+    if (rc) { or_grid_free(g); return rc; }
+    g->inv_hf = 2.0f * g->inv_h;
+    rc = sort_cells(xyz, n, g->inv_hf, g->pmin, g->pmax, 4, g->forigin, g->fdims, &g->fncells, &g->fcell_start,
+                    &g->fsorted, NULL);
+    if (rc) { or_grid_free(g); return rc; }
     densitygrid_build(g);
     empty27_build(g);
     return 0;
@@ -188,6 +206,7 @@ void or_grid_free(or_grid *g)
 {
     free(g->cell_start); free(g->sorted); free(g->sorted_index);
     free(g->dg_flag_box); free(g->dg_flag_cons); free(g->dg_count_box); free(g->empty27); free(g->bound);
+    free(g->fcell_start); free(g->fsorted);
     memset(g, 0, sizeof(*g));
 }
 
@@ -354,6 +373,80 @@ uint32_t or_neighbors_hash(const float *xyz, uint64_t n, float h, const float q[
 /* order.  Compensated form: Neumaier's variant of Kahan summation.           */
 /* ------------------------------------------------------------------------- */
 
+/* ---- march-grid stencil -------------------------------------------------- */
+/* In fine-cell units u = inv_hf x, the point sits at offset f in [0,1] inside
+ * its cell (f = 1 only for CompactNSearch's negative-integer key).  The
+ * distance from the point to the cell at offset d along one axis is
+ * d - f (d > 0), f - (d + 1) (d < 0), 0 (d = 0).  A row or row end is kept
+ * when its squared distance is below 4 (1 + 2^-10) (h = 2 fine cells): the
+ * margin only keeps cells, so every particle within h is in a kept cell.    */
+
+#define OR_FINE_LIM 4.00390625f
+
+static inline float fine_off(int d, float f)
+{
+    return d > 0 ? (float)d - f : (d < 0 ? f - (float)(d + 1) : 0.0f);
+}
+
+static inline uint64_t fine_lin(const or_grid *g, int32_t x, int32_t y, int32_t z)
+{
+    return (uint64_t)x + (uint64_t)g->fdims[0] * ((uint64_t)y + (uint64_t)g->fdims[1] * (uint64_t)z);
+}
+
+int or_fine_centre(const or_grid *g, const float q[3], or_fine *fs)
+{
+    int ok = 1;
+    for (int a = 0; a < 3; a++) {
+        float u = g->inv_hf * q[a];
+        int32_t k = q[a] >= 0.0f ? (int32_t)u : (int32_t)u - 1;
+        fs->f[a] = u - (float)k;
+        fs->c[a] = k - g->forigin[a];
+        if (fs->c[a] < 2 || fs->c[a] > g->fdims[a] - 3) ok = 0;
+    }
+    return ok;
+}
+
+int or_fine_row(const or_grid *g, const or_fine *fs, int dy, int dz, uint32_t *b, uint32_t *e)
+{
+    float ty = fine_off(dy, fs->f[1]), tz = fine_off(dz, fs->f[2]);
+    float r2 = ty * ty + tz * tz;
+    if (!(r2 < OR_FINE_LIM)) return 0;
+    float lm2 = fine_off(-2, fs->f[0]), lm1 = fine_off(-1, fs->f[0]);
+    float lp1 = fine_off(1, fs->f[0]), lp2 = fine_off(2, fs->f[0]);
+    int lo = (lm2 * lm2 + r2 < OR_FINE_LIM) ? -2 : ((lm1 * lm1 + r2 < OR_FINE_LIM) ? -1 : 0);
+    int hi = (lp2 * lp2 + r2 < OR_FINE_LIM) ? 2 : ((lp1 * lp1 + r2 < OR_FINE_LIM) ? 1 : 0);
+    int32_t y = fs->c[1] + dy, z = fs->c[2] + dz;
+    *b = g->fcell_start[fine_lin(g, fs->c[0] + lo, y, z)];
+    *e = g->fcell_start[fine_lin(g, fs->c[0] + hi, y, z) + 1];
+    return 1;
+}
+
+uint32_t or_neighbors_fine(const or_grid *g, const float q[3], float *out, uint32_t cap, uint32_t *ncand)
+{
+    or_fine fs;
+    float h2 = g->h * g->h;
+    uint32_t cnt = 0, cand = 0;
+    if (or_fine_centre(g, q, &fs)) {
+        for (int dz = -2; dz <= 2; dz++)
+            for (int dy = -2; dy <= 2; dy++) {
+                uint32_t b, e;
+                if (!or_fine_row(g, &fs, dy, dz, &b, &e)) continue;
+                cand += e - b;
+                for (uint32_t s = b; s < e; s++) {
+                    const float *p = g->fsorted + 4 * (uint64_t)s;
+                    float rx = p[0] - q[0], ry = p[1] - q[1], rz = p[2] - q[2];
+                    float r2 = (rx * rx + ry * ry) + rz * rz;
+                    if (r2 < h2) {
+                        if (cnt < cap) { out[3 * cnt] = p[0]; out[3 * cnt + 1] = p[1]; out[3 * cnt + 2] = p[2]; }
+                        cnt++;
+                    }
+                }
+            }
+    }
+    if (ncand) *ncand = cand;
+    return cnt;
+}
+
 typedef struct { float s, c; } neu_f;
 static inline void neu_add(neu_f *a, float w)
 {
@@ -365,19 +458,19 @@ static inline void neu_add(neu_f *a, float w)
 
 float or_density_f(const or_grid *g, const float q[3], int compensated, uint32_t *ncand)
 {
-    int32_t c[3];
+    or_fine fs;
     float h2 = g->h * g->h, hinv = 1.0f / g->h, sig = or_sigma_f(g->h);
     neu_f acc = { 0.0f, 0.0f };
     float plain = 0.0f;
     uint32_t cand = 0;
-    if (stencil_centre(g, q[0], q[1], q[2], c)) {
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++) {
-                uint64_t base = cell_lin(g, c[0] - 1, c[1] + dy, c[2] + dz);
-                uint32_t b = g->cell_start[base], e = g->cell_start[base + 3];
+    if (or_fine_centre(g, q, &fs)) {
+        for (int dz = -2; dz <= 2; dz++)
+            for (int dy = -2; dy <= 2; dy++) {
+                uint32_t b, e;
+                if (!or_fine_row(g, &fs, dy, dz, &b, &e)) continue;
                 cand += e - b;
                 for (uint32_t s = b; s < e; s++) {
-                    const float *p = g->sorted + 4 * (uint64_t)s;
+                    const float *p = g->fsorted + 4 * (uint64_t)s;
                     float rx = p[0] - q[0], ry = p[1] - q[1], rz = p[2] - q[2];
                     float r2 = (rx * rx + ry * ry) + rz * rz;
                     if (r2 < h2) {
@@ -396,18 +489,19 @@ double or_density_d(const or_grid *g, const double q[3])
 {
     double h = (double)g->h;
     double h2 = h * h, hinv = 1.0 / h, sig = 8.0 / (OR_PI_D * h * h * h);
-    int32_t c[3];
+    or_fine fs;
     double acc = 0.0;
     /* The stencil is chosen from the FP32-rounded point; the FP32 and FP64
      * stencils can only differ for points within an ulp of a cell face, where
      * the difference is a cell at distance >= h and contributes nothing. */
-    if (stencil_centre(g, (float)q[0], (float)q[1], (float)q[2], c)) {
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++) {
-                uint64_t base = cell_lin(g, c[0] - 1, c[1] + dy, c[2] + dz);
-                uint32_t b = g->cell_start[base], e = g->cell_start[base + 3];
+    const float qf[3] = { (float)q[0], (float)q[1], (float)q[2] };
+    if (or_fine_centre(g, qf, &fs)) {
+        for (int dz = -2; dz <= 2; dz++)
+            for (int dy = -2; dy <= 2; dy++) {
+                uint32_t b, e;
+                if (!or_fine_row(g, &fs, dy, dz, &b, &e)) continue;
                 for (uint32_t s = b; s < e; s++) {
-                    const float *p = g->sorted + 4 * (uint64_t)s;
+                    const float *p = g->fsorted + 4 * (uint64_t)s;
                     double rx = (double)p[0] - q[0], ry = (double)p[1] - q[1], rz = (double)p[2] - q[2];
                     double r2 = (rx * rx + ry * ry) + rz * rz;
                     if (r2 < h2) acc += W_d(r2, h2, hinv, sig);
@@ -592,7 +686,8 @@ int or_bound_build(or_grid *g, float iso)
 
 /* Bound skip of the configured march: the sample's sub-box has its bit set, so
  * the march resumes at the first sample past the sub-box exit (the exit index
- * lowered by 2^-8 + |k| 2^-18 steps against rounding).  Returns the next sample
+ * lowered by 2^-8 + |k| 2^-18 steps against rounding; slab distances use the
+ * reciprocal direction and step, as the GPU computes them).  Returns the next sample
  * index (k when there is nothing to skip). */
 static int64_t skip_bound(const or_grid *g, const float o[3], const float d[3], const float x[3], float tn,
                           float step, int64_t k)
@@ -612,10 +707,11 @@ static int64_t skip_bound(const or_grid *g, const float o[3], const float d[3], 
     for (int a = 0; a < 3; a++) {
         if (d[a] == 0.0f) continue;
         float lo = ((float)key[a] + 0.25f * (float)sub[a]) * g->h, hi = ((float)key[a] + 0.25f * (float)(sub[a] + 1)) * g->h;
-        float ta = ((d[a] > 0.0f ? hi : lo) - o[a]) / d[a];
+        float id = 1.0f / d[a];
+        float ta = ((d[a] > 0.0f ? hi : lo) - o[a]) * id;
         if (ta < texit) texit = ta;
     }
-    float kq = (texit - tn) / step;
+    float kq = (texit - tn) * (1.0f / step);
     kq -= 0x1p-8f + fabsf(kq) * 0x1p-18f;
     float kf = floorf(kq);
     return kf >= (float)k ? (int64_t)kf + 1 : k + 1;
@@ -766,17 +862,17 @@ static void sum_gradW(dens_ctx *dc, const float q[3], float out[3], or_counters 
 {
     /* RayMarcher.cpp:331-338: normal += gradW(x_j - x) over the neighbours. */
     const or_grid *g = dc->g;
-    int32_t c[3];
+    or_fine fs;
     out[0] = out[1] = out[2] = 0.0f;
     uint32_t cand = 0;
-    if (stencil_centre(g, q[0], q[1], q[2], c)) {
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++) {
-                uint64_t base = cell_lin(g, c[0] - 1, c[1] + dy, c[2] + dz);
-                uint32_t b = g->cell_start[base], e = g->cell_start[base + 3];
+    if (or_fine_centre(g, q, &fs)) {
+        for (int dz = -2; dz <= 2; dz++)
+            for (int dy = -2; dy <= 2; dy++) {
+                uint32_t b, e;
+                if (!or_fine_row(g, &fs, dy, dz, &b, &e)) continue;
                 cand += e - b;
                 for (uint32_t s = b; s < e; s++) {
-                    const float *p = g->sorted + 4 * (uint64_t)s;
+                    const float *p = g->fsorted + 4 * (uint64_t)s;
                     float r[3] = { p[0] - q[0], p[1] - q[1], p[2] - q[2] };
                     float r2 = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
                     if (r2 < dc->h2) {

@@ -77,6 +77,17 @@ typedef struct or_grid {
      * 16 sz of bound[c] set when sub-box l of cell c stays below iso         */
     uint64_t *bound;
     float     bound_iso;
+    /* march grid: cells of h/2 (the same key rule with inv_hf = 2/h), four
+     * empty cells of margin on each side, particles fine-cell-major (x
+     * fastest) and by input index inside a cell.  Every density sum walks the
+     * 5x5x5 fine cells around the point's fine cell as x-rows (dz outer, dy),
+     * dropping rows and row ends farther than h (or_fine_row).               */
+    float     inv_hf;
+    int32_t   forigin[3];
+    int32_t   fdims[3];
+    uint64_t  fncells;
+    uint32_t *fcell_start;          /* fncells + 1                           */
+    float    *fsorted;              /* 4*n                                   */
 } or_grid;
 
 /* ---- SPH cubic spline kernel (Kernel.cpp:8-52) ---- */
@@ -101,6 +112,19 @@ uint32_t or_neighbors_grid(const or_grid *g, const float q[3], uint32_t *out, ui
                            uint32_t *ncand);
 uint32_t or_neighbors_hash(const float *xyz, uint64_t n, float h, const float q[3],
                            uint32_t *out, uint32_t cap);
+
+/* ---- march-grid stencil ---- */
+/* Fine-cell stencil centre of a point and its offsets inside the cell; 0 when
+ * the 5x5x5 stencil would leave the grid (no particle within h). */
+typedef struct or_fine { int32_t c[3]; float f[3]; } or_fine;
+int or_fine_centre(const or_grid *g, const float q[3], or_fine *fs);
+/* Row (dy, dz) of the stencil: 0 if no point of it lies within h; else the
+ * stored range [b, e) of its fine cells within reach. */
+int or_fine_row(const or_grid *g, const or_fine *fs, int dy, int dz, uint32_t *b, uint32_t *e);
+
+/* Particles with |q - x_j|^2 < h^2 in the march grid's order (input indices
+ * are not stored for the march grid, so positions are returned: 3 per hit). */
+uint32_t or_neighbors_fine(const or_grid *g, const float q[3], float *out, uint32_t cap, uint32_t *ncand);
 
 /* ---- density ---- */
 float  or_density_f(const or_grid *g, const float q[3], int compensated, uint32_t *ncand);

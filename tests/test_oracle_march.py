@@ -153,16 +153,25 @@ def test_fp32_vs_fp64_lattice_cube():
     assert np.all(pos[miss] == 0) and np.all(nrm[miss] == 0) and np.all(t[miss] == -1)
 
 
-def test_hashed_search_gives_identical_images():
+def test_hashed_search_matches_grid_march():
+    """The reference-style march (27 h-cells from a hashed table, summed in
+    h-cell order) against the grid march (fine-cell order over the trimmed
+    5x5x5 fine stencil): the same samples and hits, images equal to within the
+    last bits of the density sums, far fewer candidates for the fine stencil."""
     sc = scenes.lattice_cube(1, width=96, height=72)
     c = sc.cameras[1]
     G = O.Grid(sc.xyz, sc.h)
     st = O.make_settings(**sc.settings())
     a = O.march(G, st, c.inv_pv, c.eye, c.width, c.height)
     b = O.HashedSearch(sc.xyz, sc.h).march(G, st, c.inv_pv, c.eye, c.width, c.height)
-    for u, v in zip(a[:3], b[:3]):
-        assert np.array_equal(u, v)
-    assert a[3] == b[3]
+    m = compare(*a[:3], *b[:3])
+    assert_within(m, extent(sc.xyz))
+    same = np.mean(np.all(a[0] == b[0], axis=-1))
+    print("identical pixels", same, "candidates fine", a[3]["cand"], "hashed h-cells", b[3]["cand"])
+    assert same > 0.99
+    for k in ("rays", "rays_in_aabb", "hits", "samples", "march_samples"):
+        assert a[3][k] == b[3][k], k
+    assert a[3]["cand"] < 0.5 * b[3]["cand"]
 
 
 def test_thread_count_does_not_change_results():

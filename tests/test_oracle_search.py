@@ -95,3 +95,57 @@ def test_densitygrid_conservative_flag_clears_only_empty_space():
     cell = np.stack([x, y, z], -1)
     pts = G.bmin + (cell + rng.uniform(0, 1, cell.shape)) * h
     assert np.all(G.density64(pts) == 0.0)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fine_stencil_layout_and_neighbours(seed):
+    """The march grid: fine keys are the h-keys' rule with 2/h, stored
+    fine-cell-major by input index; the trimmed 5x5x5 fine stencil returns
+    exactly the brute-force neighbour set, ordered by (fine cell, input index),
+    and visits fewer candidates than the 27 h-cells."""
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(-0.2, 0.3, (4000, 3)).astype(np.float32)
+    xyz[:50] = np.round(xyz[:50] / 0.03) * 0.03               # points on fine-cell faces, negative ones included
+    h = 0.06
+    G = O.Grid(xyz, h)
+    kf = np_keys(xyz, h / 2)
+    inv_hf = np.float32(2.0) * (np.float32(1.0) / np.float32(h))
+    assert np.float32(G.g.inv_hf) == inv_hf
+    org = np.array(G.g.forigin[:])
+    dims = np.array(G.g.fdims[:])
+    assert np.array_equal(org, kf.min(0) - 4) and np.array_equal(dims, kf.max(0) - kf.min(0) + 9)
+    c = kf - org
+    lin = c[:, 0] + dims[0] * (c[:, 1] + dims[1] * c[:, 2])
+    order = np.lexsort((np.arange(len(xyz)), lin))
+    fs = np.ctypeslib.as_array(G.g.fsorted, shape=(len(xyz), 4))
+    assert np.array_equal(fs[:, :3], xyz[order])
+    rank = np.empty(len(xyz), np.int64)
+    rank[order] = np.arange(len(xyz))
+    q = rng.uniform(-0.3, 0.4, (400, 3)).astype(np.float32)
+    q[:30] = xyz[:30]
+    q[30:60] = np.round(q[30:60] / 0.03) * 0.03
+    tot_fine = tot_h = 0
+    for p in q:
+        pos, cand = G.neighbors_fine(p)
+        nb = O.neighbors_brute(xyz, h, p)
+        assert len(pos) == len(nb)
+        want = xyz[nb[np.argsort(rank[nb])]]                 # brute-force set in (fine cell, index) order
+        assert np.array_equal(pos, want)
+        tot_fine += cand
+        tot_h += G.neighbors(p)[1]
+    print("candidates fine", tot_fine, "27 h-cells", tot_h)
+    assert tot_fine < 0.6 * tot_h
+
+
+def test_fine_density_matches_fp64_brute_force():
+    rng = np.random.default_rng(4)
+    xyz = rng.normal(0.0, 0.05, (6000, 3)).astype(np.float32)
+    h = 0.04
+    G = O.Grid(xyz, h)
+    q = rng.normal(0.0, 0.06, (500, 3)).astype(np.float32)
+    rho = G.density(q)
+    d = np.linalg.norm(xyz[None, :, :].astype(np.float64) - q[:, None, :].astype(np.float64), axis=-1) / h
+    sig = 8.0 / (np.pi * h ** 3)
+    w = np.where(d < 0.5, 6.0 * (d ** 3 - d ** 2) + 1.0, 2.0 * np.clip(1.0 - d, 0.0, None) ** 3) * sig
+    ref = w.sum(1)
+    assert np.max(np.abs(rho - ref) / np.maximum(ref, sig)) < 2e-6
