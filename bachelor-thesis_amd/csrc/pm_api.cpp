@@ -489,9 +489,11 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     L.t = (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
     L.band_ctr = (uint32_t *)c->d_band.p;
-    if (s->mode == PM_MODE_CONFIG && s->variant == 7) {
+    // tile order and miss pre-pass for the persistent schedules (variant 7: without)
+    const bool persistent = s->learned || (s->variant != 1 && s->variant != 4 && s->variant != 5);
+    if (s->mode == PM_MODE_CONFIG && persistent && s->variant != 7) {
         uint64_t tiles = (uint64_t)((W + 7) / 8) * ((H + 7) / 8);
-        HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (2 * tiles + 1040)));
+        HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (3 * (tiles + 1) + scan_tmp_size((uint32_t)(tiles + 1)))));
         L.tile_buf = (uint32_t *)c->d_tiles.p;
     }
     L.cg = f->cg;

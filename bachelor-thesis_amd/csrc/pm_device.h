@@ -168,9 +168,9 @@ __device__ __forceinline__ float pm_density32(const GridDev &g, float qx, float 
     FineWalk fw;
     float s = 0.0f, c = 0.0f;
     if (!pm_fine_begin(g, qx, qy, qz, fw)) return 0.0f;
-    for (int dz = -2; dz <= 2; dz++) {
+    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++) {
         const float tz2 = pm_fine_tz2(fw, dz);
-        for (int dy = -2; dy <= 2; dy++) {
+        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
             uint32_t b, e;
             pm_fine_row(g, fw, dy, dz, tz2, b, e);
             cand += e - b;
@@ -232,12 +232,13 @@ __device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, fl
     bool in = pm_fine_begin(g, qx, qy, qz, fw);
     int n = 0;
     if (in) {
-        for (int dz = -2; dz <= 2; dz++) {
+        _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++) {
             const float tz2 = pm_fine_tz2(fw, dz);
-            for (int dy = -2; dy <= 2; dy++) {
+            _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
                 uint32_t b, e;
                 pm_fine_row(g, fw, dy, dz, tz2, b, e);
                 cand += e - b;
+                if (__ballot(e > b) == 0) continue;         // no lane has candidates in this row
                 if (__ballot(n + (int)(e - b) > CAP) != 0) {   // some lane might overflow:
                     pm_drain<CAP>(g, buf, n, s, c);           // all drain together
                     n = 0;
@@ -272,9 +273,9 @@ __device__ __forceinline__ double pm_density64(const GridDev &g, double qx, doub
     FineWalk fw;
     double acc = 0.0;
     if (!pm_fine_begin(g, (float)qx, (float)qy, (float)qz, fw)) return 0.0;
-    for (int dz = -2; dz <= 2; dz++) {
+    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++) {
         const float tz2 = pm_fine_tz2(fw, dz);
-        for (int dy = -2; dy <= 2; dy++) {
+        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
             uint32_t b, e;
             pm_fine_row(g, fw, dy, dz, tz2, b, e);
             for (uint32_t i = b; i < e; i++) {

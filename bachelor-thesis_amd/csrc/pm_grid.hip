@@ -195,6 +195,15 @@ __global__ __launch_bounds__(kThreads) void k_cell_sort(float4 *__restrict__ sor
     }
 }
 
+hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t len, uint32_t *tmp, hipStream_t st)
+{
+    uint32_t nb = (len + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kThreads), 0, st, in, out, len, tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, st, tmp, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kThreads), 0, st, out, len, tmp);
+    return hipGetLastError();
+}
+
 hipError_t launch_grid_build(const GridBuildArgs &a, hipStream_t st)
 {
     uint32_t len = a.ncells + 1;

@@ -38,8 +38,8 @@ struct MarchLaunch {
     float mlp_alpha;           // fraction of the predicted distance that is skipped
     int32_t mlp_min_skip;      // below this many steps the ray continues step by step
     int32_t mlp_max_skip;      // longest jump, in steps
-    // tile order (longest first inside each XCD band): scratch of 2 T + 1040 u32;
-    // null = row-major order
+    // tiles with a ray entering the AABB, in row order: scratch of
+    // 3 (T + 1) + scan_tmp_size(T + 1) u32; null = every tile
     uint32_t *tile_buf;
     float inv_step;            // 1 / s.step (bound skip)
 };
@@ -91,6 +91,8 @@ struct GridBuildArgs {
     uint32_t *cell_start;  // ncells + 1
 };
 size_t scan_tmp_size(uint32_t len);
+// Exclusive scan of len u32 (out may not alias in); tmp of scan_tmp_size(len).
+hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t len, uint32_t *tmp, hipStream_t st);
 hipError_t launch_grid_build(const GridBuildArgs &a, hipStream_t st);
 
 hipError_t launch_march(const MarchLaunch &L, hipStream_t st);

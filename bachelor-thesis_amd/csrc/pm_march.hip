@@ -72,8 +72,8 @@ __device__ __forceinline__ void pm_sum_gradw(const GridDev &g, float qx, float q
     FineWalk fw;
     gx = gy = gz = 0.0f;
     if (!pm_fine_begin(g, qx, qy, qz, fw)) return;
-    for (int dz = -2; dz <= 2; dz++)
-        for (int dy = -2; dy <= 2; dy++) {
+    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++)
+        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
             uint32_t b, e;
             pm_fine_row(g, fw, dy, dz, pm_fine_tz2(fw, dz), b, e);
             cand += e - b;
@@ -476,6 +476,66 @@ struct DensLds {
 
 static constexpr int kLdsCap = 32;
 
+// Band b of the T tiles is [T b / 8, T (b+1) / 8).
+__device__ __forceinline__ uint32_t band_lo(uint32_t T, uint32_t b) { return (uint32_t)(((uint64_t)T * b) >> 3); }
+
+// Tile pre-pass of the persistent schedules.  tile_buf = flag[T+1] | pos[T+1] |
+// order[T+1] | scan scratch: flag = 1 for a tile one of whose rays enters the
+// padded AABB (the ray set-up and slab test of ray_setup, bit for bit), pos
+// its exclusive scan, order the active tiles in row order (order[T] = their
+// count).  The persistent waves take only active tiles; k_miss_tiles writes
+// the misses of the others.
+__global__ __launch_bounds__(256) void k_tile_flags(MarchLaunch L, uint32_t T)
+{
+    // one wave per tile, one lane per ray
+    const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6), j = threadIdx.x & 63u;
+    if (t > T) return;   // whole wave
+    bool enters = false;
+    if (t < T) {
+        const uint32_t tiles_x = (L.W + 7) / 8;
+        const uint32_t px = (t % tiles_x) * 8 + (j & 7u), py = (t / tiles_x) * 8 + (j >> 3);
+        if (px < L.W && py < L.H) {
+            const float cx = (float)px * (2.0f / (float)L.W) - 1.0f;
+            const float cy = (float)py * (2.0f / (float)L.H) - 1.0f;
+            float wx, wy, wz, d[3], tn, tf;
+            pm_unproject(L.M, cx, cy, 1.0f, wx, wy, wz);
+            pm_normalize(wx - L.cam[0], wy - L.cam[1], wz - L.cam[2], d[0], d[1], d[2]);
+            const float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
+            enters = pm_slab(o, d, L.g.bmin, L.g.bmax, tn, tf);
+        }
+    }
+    const unsigned long long m = __ballot(enters);
+    if (j == 0) L.tile_buf[t] = m != 0ull ? 1u : 0u;   // flag[T] = 0 closes the scan
+}
+
+__global__ __launch_bounds__(256) void k_tile_compact(uint32_t *tile_buf, uint32_t T)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t > T) return;
+    const uint32_t *flag = tile_buf, *pos = tile_buf + (T + 1);
+    uint32_t *order = tile_buf + 2 * (T + 1);
+    if (t == T) order[T] = pos[T];
+    else if (flag[t]) order[pos[t]] = t;
+}
+
+// Pixels of the inactive tiles are written as misses (and counted) here.
+__global__ __launch_bounds__(256) void k_miss_tiles(MarchLaunch L, uint32_t T)
+{
+    const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6), j = threadIdx.x & 63u;
+    if (t >= T) return;   // whole wave
+    bool miss = false;
+    if (L.tile_buf[t] == 0) {
+        const uint32_t tiles_x = (L.W + 7) / 8;
+        const uint32_t px = (t % tiles_x) * 8 + (j & 7u), py = (t / tiles_x) * 8 + (j >> 3);
+        if (px < L.W && py < L.H) {
+            write_miss(L, py * L.W + px);
+            miss = true;
+        }
+    }
+    const unsigned long long m = __ballot(miss);
+    if (j == 0 && m) atomicAdd(&L.counters[C_RAYS], (unsigned long long)__popcll(m));
+}
+
 // Schedule "tile": one 8x8 tile per wave, one ray per lane, all lanes in
 // lockstep until the last ray of the tile is done.
 template <bool LDS, int UNROLL>
@@ -591,11 +651,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             if (qpos >= 64) {
                 uint32_t t = 0xffffffffu;
                 if (lane == 0) {
+                    // with the pre-pass the bands split the active tiles only
+                    const uint32_t TA = L.tile_buf ? L.tile_buf[2 * (T + 1) + T] : T;
                     while (steal < 8) {
                         uint32_t b = (xcd + steal) & 7u;
-                        uint32_t lo = (uint32_t)(((uint64_t)T * b) >> 3), hi = (uint32_t)(((uint64_t)T * (b + 1)) >> 3);
+                        uint32_t lo = band_lo(TA, b), hi = band_lo(TA, b + 1);
                         uint32_t v = atomicAdd(&band_ctr[b], 1u);
-                        if (v < hi - lo) { t = L.tile_buf ? L.tile_buf[T + 1040 + lo + v] : lo + v; break; }
+                        if (v < hi - lo) { t = L.tile_buf ? L.tile_buf[2 * (T + 1) + lo + v] : lo + v; break; }
                         steal++;
                     }
                 }
@@ -642,59 +704,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 __global__ void k_zero_u32(uint32_t *p, int n)
 {
     for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) p[i] = 0u;
-}
-
-// Band b of the T tiles is [T b / 8, T (b+1) / 8).
-__device__ __forceinline__ uint32_t band_lo(uint32_t T, uint32_t b) { return (uint32_t)(((uint64_t)T * b) >> 3); }
-
-__device__ __forceinline__ uint32_t band_of(uint32_t T, uint32_t t)
-{
-    uint32_t b = 7;
-    while (b > 0 && band_lo(T, b) > t) b--;
-    return b;
-}
-
-// Tile order of the persistent schedule, longest-processing-time first inside
-// each band: cost = rays of the tile that enter the padded AABB (0..64).
-// tile_buf: cost[T] | hist[8][65] | offs[8][65] | order[T].
-__global__ __launch_bounds__(256) void k_tile_cost(MarchLaunch L, uint32_t T)
-{
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= T) return;
-    const uint32_t tiles_x = (L.W + 7) / 8;
-    uint32_t n = 0;
-    for (uint32_t j = 0; j < 64; j++) {
-        const uint32_t px = (t % tiles_x) * 8 + (j & 7u), py = (t / tiles_x) * 8 + (j >> 3);
-        if (px >= L.W || py >= L.H) continue;
-        const float cx = (float)px * (2.0f / (float)L.W) - 1.0f;
-        const float cy = (float)py * (2.0f / (float)L.H) - 1.0f;
-        float wx, wy, wz, d[3], tn, tf;
-        pm_unproject(L.M, cx, cy, 1.0f, wx, wy, wz);
-        pm_normalize(wx - L.cam[0], wy - L.cam[1], wz - L.cam[2], d[0], d[1], d[2]);
-        const float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
-        if (pm_slab(o, d, L.g.bmin, L.g.bmax, tn, tf)) n++;
-    }
-    L.tile_buf[t] = n;
-    atomicAdd(&L.tile_buf[T + band_of(T, t) * 65 + (64 - n)], 1u);
-}
-
-__global__ void k_tile_scan(uint32_t *tile_buf, uint32_t T)
-{
-    const uint32_t b = threadIdx.x;
-    if (b >= 8) return;
-    uint32_t acc = band_lo(T, b);
-    for (int k = 0; k < 65; k++) {
-        tile_buf[T + 520 + b * 65 + k] = acc;
-        acc += tile_buf[T + b * 65 + k];
-    }
-}
-
-__global__ __launch_bounds__(256) void k_tile_scatter(uint32_t *tile_buf, uint32_t T)
-{
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= T) return;
-    const uint32_t pos = atomicAdd(&tile_buf[T + 520 + band_of(T, t) * 65 + (64 - tile_buf[t])], 1u);
-    tile_buf[T + 1040 + pos] = t;
 }
 
 // QueryDensityGrid (Dataset.cpp:26-47) followed by the Flag test: true when p
@@ -951,10 +960,12 @@ hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L.band_ctr, 8);
     if (L.tile_buf) {
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, st, L.tile_buf + tiles, 520);
-        hipLaunchKernelGGL(k_tile_cost, dim3((tiles + 255) / 256), dim3(256), 0, st, L, tiles);
-        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(64), 0, st, L.tile_buf, tiles);
-        hipLaunchKernelGGL(k_tile_scatter, dim3((tiles + 255) / 256), dim3(256), 0, st, L.tile_buf, tiles);
+        uint32_t *flag = L.tile_buf, *pos = L.tile_buf + (tiles + 1);
+        hipLaunchKernelGGL(k_tile_flags, dim3((tiles + 1 + 3) / 4), dim3(256), 0, st, L, tiles);
+        hipError_t e = launch_scan_u32(flag, pos, tiles + 1, L.tile_buf + 3 * (tiles + 1), st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_tile_compact, dim3((tiles + 1 + 255) / 256), dim3(256), 0, st, L.tile_buf, tiles);
+        hipLaunchKernelGGL(k_miss_tiles, dim3((tiles + 3) / 4), dim3(256), 0, st, L, tiles);
     }
     // enough resident waves to fill the chip; extra waves find no tile and exit
     uint32_t blocks = L.persist_blocks ? L.persist_blocks : 1024;

@@ -74,7 +74,9 @@ typedef struct pm_settings {
                               sums; 2, 3 = persistent with 1 or 8 loads in flight;
                               4, 5 = one tile per wave with compaction; 6 = persistent
                               with registers capped for 5 waves per SIMD; 7 = persistent
-                              with tiles handed out longest-first                      */
+                              over every tile (the persistent schedules otherwise
+                              take only tiles with a ray entering the AABB and write
+                              the others' misses in a pre-pass)                       */
     float   learned_alpha; /* learned: fraction of the predicted distance skipped (0 = 0.7) */
     int32_t learned_min_skip; /* learned: shorter proposals end the proposals (0 = 4)  */
     int32_t learned_max_skip; /* learned: longest jump in steps (0 = 24)              */
