@@ -47,6 +47,7 @@ struct Frame {
     DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound;
     DevBuf fsorted, fcell_start;    // march grid (cells of h/2)
     bool bound_ok = false;      // bound holds the sub-box bits for bound_iso
+    bool dg_ok = false;         // the DensityGrid flags are built
     float bound_iso = 0.0f;
     int32_t dg_dims[3] = { 0, 0, 0 };
     CoarseDev cg{};
@@ -90,7 +91,9 @@ struct pm_ctx {
     // batched views: job and tile-chunk tables, all views' images, the captured launch
     uint64_t frame_epoch = 0;           // bumped whenever a frame's device buffers may move
     uint64_t batch_epoch = ~0ull;
-    DevBuf b_jobs, b_chunks, b_img, b_ctr;
+    DevBuf b_jobs, b_chunks, b_img, b_ctr, b_flags;
+    unsigned long long b_rays0 = 0;     // rays of inactive tiles (written once as misses)
+    bool prepass = false;               // the last single-view march ran the tile pre-pass
     std::vector<uint64_t> b_off;        // per view: first pixel in b_img
     std::vector<uint32_t> b_w, b_h;
     uint64_t b_npx = 0;
@@ -184,7 +187,7 @@ void pm_destroy(pm_ctx *c)
     c->ds_feat.release(); c->ds_label.release(); c->ds_hit.release(); c->ds_count.release(); c->ds_meta.release();
     if (c->b_exec) hipGraphExecDestroy(c->b_exec);
     if (c->b_graph) hipGraphDestroy(c->b_graph);
-    c->b_jobs.release(); c->b_chunks.release(); c->b_img.release(); c->b_ctr.release();
+    c->b_jobs.release(); c->b_chunks.release(); c->b_img.release(); c->b_ctr.release(); c->b_flags.release();
     if (c->h_stage) hipHostFree(c->h_stage);
     if (c->h_counters) hipHostFree(c->h_counters);
     if (c->ev_launch) hipEventDestroy(c->ev_launch);
@@ -327,13 +330,9 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
         dgn *= (uint64_t)f->dg_dims[a2];
     }
     if (dgn >= (1ull << 31)) return fail(c, PM_ERR_SIZE, "pm_set_frame: DensityGrid too large");
-    HIPCHK(c, f->dg_box.ensure(dgn));
-    HIPCHK(c, f->dg_cons.ensure(dgn));
-    HIPCHK(c, f->dg_count.ensure(4 * dgn));
-    HIPCHK(c, launch_densitygrid(g, f->dg_dims[0], f->dg_dims[1], f->dg_dims[2], (uint8_t *)f->dg_box.p, (uint8_t *)f->dg_cons.p,
-                                 (uint32_t *)f->dg_count.p, c->stream));
     g.dg_nx = f->dg_dims[0]; g.dg_ny = f->dg_dims[1]; g.dg_nz = f->dg_dims[2];
     g.dg_flag = nullptr;
+    f->dg_ok = false;   // built on first use (reference march, pm_get_skip_grids)
     // coarse occupancy (4x4x4 dense cells) for the step-proposal features
     f->cg.nx = (I.dims[0] + 3) / 4; f->cg.ny = (I.dims[1] + 3) / 4; f->cg.nz = (I.dims[2] + 3) / 4;
     HIPCHK(c, f->coarse.ensure(4ull * f->cg.nx * f->cg.ny * f->cg.nz));
@@ -406,6 +405,21 @@ int pm_get_grid(pm_ctx *c, int32_t frame, uint32_t *cell_start, float *sorted, u
     return PM_OK;
 }
 
+// The reference's DensityGrid flags of a frame (Frame::BuildDensityGrid,
+// Dataset.cpp:94-164), built on the context stream when first needed.
+static int ensure_densitygrid(pm_ctx *c, Frame *f)
+{
+    if (f->dg_ok) return PM_OK;
+    const uint64_t dgn = (uint64_t)f->dg_dims[0] * f->dg_dims[1] * f->dg_dims[2];
+    HIPCHK(c, f->dg_box.ensure(dgn));
+    HIPCHK(c, f->dg_cons.ensure(dgn));
+    HIPCHK(c, f->dg_count.ensure(4 * dgn));
+    HIPCHK(c, launch_densitygrid(f->g, f->dg_dims[0], f->dg_dims[1], f->dg_dims[2], (uint8_t *)f->dg_box.p,
+                                 (uint8_t *)f->dg_cons.p, (uint32_t *)f->dg_count.p, c->stream));
+    f->dg_ok = true;
+    return PM_OK;
+}
+
 // Sub-box density bounds of a frame for one iso value, built on the context
 // stream when first needed (a batch holding the old bits is invalidated).
 static int ensure_bound(pm_ctx *c, Frame *f, float iso)
@@ -470,6 +484,10 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
         rc = ensure_bound(c, f, s->iso);
         if (rc) return rc;
     }
+    if (s->skip_mode == PM_SKIP_DG_BOX || s->skip_mode == PM_SKIP_DG_CONS) {
+        rc = ensure_densitygrid(c, f);
+        if (rc) return rc;
+    }
     uint64_t npx = (uint64_t)W * H;
     HIPCHK(c, c->d_pos.ensure(sizeof(float4) * npx));
     HIPCHK(c, c->d_nrm.ensure(sizeof(float4) * npx));
@@ -496,6 +514,7 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
         HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (3 * (tiles + 1) + scan_tmp_size((uint32_t)(tiles + 1)))));
         L.tile_buf = (uint32_t *)c->d_tiles.p;
     }
+    c->prepass = L.tile_buf != nullptr;
     L.cg = f->cg;
     if (s->learned) {
         L.net = mlp_dev(c);
@@ -549,7 +568,10 @@ static int complete(pm_ctx *c)
     c->user_pos = c->user_nrm = c->user_t = nullptr;
     pm_stats &S = c->stats;
     memset(&S, 0, sizeof(S));
-    S.rays = c->h_counters[C_RAYS];
+    // every pixel is answered once; the rays of the tiles the pre-pass writes
+    // as misses are not counted on the device, so a march with the pre-pass
+    // reports all W*H rays
+    S.rays = c->prepass ? c->npx : c->h_counters[C_RAYS];
     S.rays_in_aabb = c->h_counters[C_IN_AABB];
     S.hits = c->h_counters[C_HITS];
     S.samples = c->h_counters[C_SAMPLES];
@@ -607,7 +629,7 @@ static constexpr uint32_t kChunkTiles = 16;   // 8x8 tiles per queue entry (1024
 
 static int batch_enqueue_launch(pm_ctx *c)
 {
-    HIPCHK(c, launch_zero_u64(c->b_launch.base.counters, C_COUNT, c->stream));
+    HIPCHK(c, launch_init_counters(c->b_launch.base.counters, C_COUNT, c->b_rays0, c->stream));
     HIPCHK(c, launch_zero_u32(c->b_launch.chunk_ctr, 1, c->stream));
     HIPCHK(c, launch_march_batch(c->b_launch, c->b_blocks, c->stream));
     return PM_OK;
@@ -656,7 +678,7 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
     if (c->b_exec) { hipGraphExecDestroy(c->b_exec); c->b_exec = nullptr; }
     if (c->b_graph) { hipGraphDestroy(c->b_graph); c->b_graph = nullptr; }
 
-    std::vector<BatchJob> jobs(n_views);
+    std::vector<MarchLaunch> jobs(n_views);
     c->b_off.assign(n_views, 0);
     c->b_w.assign(n_views, 0);
     c->b_h.assign(n_views, 0);
@@ -688,14 +710,13 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
     for (uint32_t v = 0; v < n_views; v++) {
         const pm_view &vw = views[v];
         Frame *f = get_frame(c, vw.frame);
-        BatchJob &J = jobs[v];
+        MarchLaunch &J = jobs[v];
         J.g = f->g;
         J.cg = f->cg;
         memcpy(J.M, vw.inv_pv, sizeof(J.M));
         memcpy(J.cam, vw.cam, sizeof(J.cam));
         J.W = vw.width;
         J.H = vw.height;
-        J.tiles_x = (vw.width + 7) / 8;
         J.pos = pos + c->b_off[v];
         J.nrm = nrm + c->b_off[v];
         J.t = tt + c->b_off[v];
@@ -707,18 +728,49 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
         return (uint64_t)c->b_w[a] * c->b_h[a] > (uint64_t)c->b_w[b] * c->b_h[b];
     });
-    std::vector<BatchChunk> chunks;
-    for (uint32_t v : order) {
-        uint32_t tiles = ((c->b_w[v] + 7) / 8) * ((c->b_h[v] + 7) / 8);
-        for (uint32_t t0 = 0; t0 < tiles; t0 += kChunkTiles)
-            chunks.push_back(BatchChunk{ v, t0, std::min(kChunkTiles, tiles - t0), 0u });
+    // Tiles none of whose rays enters the frame's AABB are answered once, here:
+    // their misses are written now and the queue holds only the other tiles
+    // (the images are identical; a run marches only tiles with work).
+    std::vector<uint64_t> foff(n_views + 1, 0);
+    for (uint32_t v = 0; v < n_views; v++)
+        foff[v + 1] = foff[v] + (uint64_t)((c->b_w[v] + 7) / 8) * ((c->b_h[v] + 7) / 8) + 1;
+    if (foff[n_views] >= (1ull << 32)) return fail(c, PM_ERR_SIZE, "pm_batch_set: too many tiles");
+    HIPCHK(c, c->b_flags.ensure(sizeof(uint32_t) * foff[n_views]));
+    for (uint32_t v = 0; v < n_views; v++) {
+        MarchLaunch Lv{};
+        Lv.g = jobs[v].g;
+        memcpy(Lv.M, jobs[v].M, sizeof(Lv.M));
+        memcpy(Lv.cam, jobs[v].cam, sizeof(Lv.cam));
+        Lv.W = jobs[v].W; Lv.H = jobs[v].H;
+        Lv.pos = jobs[v].pos; Lv.nrm = jobs[v].nrm; Lv.t = jobs[v].t;
+        Lv.counters = (unsigned long long *)c->d_counters.p;
+        Lv.tile_buf = (uint32_t *)c->b_flags.p + foff[v];
+        uint32_t T = (uint32_t)(foff[v + 1] - foff[v] - 1);
+        HIPCHK(c, launch_tile_flags(Lv, T, c->stream));
+        HIPCHK(c, launch_miss_tiles(Lv, T, c->stream));
     }
+    std::vector<uint32_t> flags(foff[n_views]);
+    HIPCHK(c, copy_sync(c, flags.data(), c->b_flags.p, sizeof(uint32_t) * flags.size(), hipMemcpyDeviceToHost));
+    std::vector<BatchChunk> chunks;
+    c->b_rays0 = 0;
+    for (uint32_t v : order) {
+        const uint32_t W = c->b_w[v], H = c->b_h[v], tx = (W + 7) / 8;
+        const uint32_t tiles = (uint32_t)(foff[v + 1] - foff[v] - 1);
+        const uint32_t *fl = flags.data() + foff[v];
+        for (uint32_t t = 0; t < tiles;) {
+            if (!fl[t]) {
+                const uint32_t x0 = (t % tx) * 8, y0 = (t / tx) * 8;
+                c->b_rays0 += (uint64_t)std::min(8u, W - x0) * std::min(8u, H - y0);
+                t++;
+                continue;
+            }
+            uint32_t t0 = t, n = 0;
+            while (t < tiles && fl[t] && n < kChunkTiles) { t++; n++; }
+            chunks.push_back(BatchChunk{ v, t0, n, 0u });
+        }
+    }
+    if (chunks.empty()) chunks.push_back(BatchChunk{ order[0], 0u, 0u, 0u });   // nothing to march
     if (chunks.size() >= (1ull << 32)) return fail(c, PM_ERR_SIZE, "pm_batch_set: too many tiles");
-    HIPCHK(c, c->b_jobs.ensure(sizeof(BatchJob) * jobs.size()));
-    HIPCHK(c, c->b_chunks.ensure(sizeof(BatchChunk) * chunks.size()));
-    HIPCHK(c, c->b_ctr.ensure(256));
-    HIPCHK(c, copy_sync(c, c->b_jobs.p, jobs.data(), sizeof(BatchJob) * jobs.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, copy_sync(c, c->b_chunks.p, chunks.data(), sizeof(BatchChunk) * chunks.size(), hipMemcpyHostToDevice));
 
     BatchLaunch &B = c->b_launch;
     B = BatchLaunch{};
@@ -731,7 +783,21 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
         B.base.mlp_min_skip = s->learned_min_skip > 0 ? s->learned_min_skip : 4;
         B.base.mlp_max_skip = s->learned_max_skip > 0 ? s->learned_max_skip : 24;
     }
-    B.jobs = (const BatchJob *)c->b_jobs.p;
+    for (MarchLaunch &J : jobs) {   // the shared fields of every job
+        J.s = B.base.s;
+        J.inv_step = B.base.inv_step;
+        J.counters = B.base.counters;
+        J.net = B.base.net;
+        J.mlp_alpha = B.base.mlp_alpha;
+        J.mlp_min_skip = B.base.mlp_min_skip;
+        J.mlp_max_skip = B.base.mlp_max_skip;
+    }
+    HIPCHK(c, c->b_jobs.ensure(sizeof(MarchLaunch) * jobs.size()));
+    HIPCHK(c, c->b_chunks.ensure(sizeof(BatchChunk) * chunks.size()));
+    HIPCHK(c, c->b_ctr.ensure(256));
+    HIPCHK(c, copy_sync(c, c->b_jobs.p, jobs.data(), sizeof(MarchLaunch) * jobs.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, copy_sync(c, c->b_chunks.p, chunks.data(), sizeof(BatchChunk) * chunks.size(), hipMemcpyHostToDevice));
+    B.jobs = (const MarchLaunch *)c->b_jobs.p;
     B.chunks = (const BatchChunk *)c->b_chunks.p;
     B.nchunks = (uint32_t)chunks.size();
     B.chunk_ctr = (uint32_t *)c->b_ctr.p;
@@ -773,6 +839,7 @@ int pm_batch_run(pm_ctx *c)
     HIPCHK(c, hipMemcpyAsync(c->h_counters, c->d_counters.p, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_done, c->stream));
     c->user_pos = c->user_nrm = c->user_t = nullptr;
+    c->prepass = false;   // the batch counts its answered-once rays in the counters
     c->pending = true;
     c->finished = false;
     return PM_OK;
@@ -1100,6 +1167,10 @@ int pm_get_skip_grids(pm_ctx *c, int32_t frame, uint8_t *empty27, int32_t dg_dim
     HIPCHK(c, hipSetDevice(c->device));
     uint64_t dgn = (uint64_t)f->dg_dims[0] * f->dg_dims[1] * f->dg_dims[2];
     if (dg_dims) for (int a = 0; a < 3; a++) dg_dims[a] = f->dg_dims[a];
+    if (flag_box || flag_cons || count_box) {
+        int rc = ensure_densitygrid(c, f);
+        if (rc) return rc;
+    }
     if (empty27) HIPCHK(c, copy_sync(c, empty27, f->empty27.p, f->info.ncells, hipMemcpyDeviceToHost));
     if (flag_box) HIPCHK(c, copy_sync(c, flag_box, f->dg_box.p, dgn, hipMemcpyDeviceToHost));
     if (flag_cons) HIPCHK(c, copy_sync(c, flag_cons, f->dg_cons.p, dgn, hipMemcpyDeviceToHost));

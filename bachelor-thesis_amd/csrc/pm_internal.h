@@ -44,25 +44,17 @@ struct MarchLaunch {
     float inv_step;            // 1 / s.step (bound skip)
 };
 
-// Batched launch: many (view, frame) jobs in one persistent kernel.
-struct BatchJob {
-    GridDev g;
-    CoarseDev cg;
-    float M[16];
-    float cam[3];
-    uint32_t W, H, tiles_x;
-    float4 *pos;
-    float4 *nrm;
-    float *t;
-};
+// Batched launch: many (view, frame) jobs in one persistent kernel; every job
+// is a complete MarchLaunch in device memory (read through a wave-uniform
+// pointer, so its fields stay in scalar registers and the scalar cache).
 // A run of consecutive 8x8 tiles of one job; waves take chunks, so the job's
 // parameters stay uniform across a wave.
 struct BatchChunk {
     uint32_t job, tile0, ntiles, pad;
 };
 struct BatchLaunch {
-    MarchLaunch base;          // settings, counters and network; g/M/cam/W/H/outputs come from the job
-    const BatchJob *jobs;
+    MarchLaunch base;          // settings, counters and network (each job repeats them)
+    const MarchLaunch *jobs;
     const BatchChunk *chunks;
     uint32_t nchunks;
     uint32_t *chunk_ctr;
@@ -71,6 +63,12 @@ hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, hipStre
 hipError_t launch_march_batch(const BatchLaunch &B, uint32_t blocks, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st);
+// counters[0..n) = 0 except counters[C_RAYS] = rays0 (rays answered without marching)
+hipError_t launch_init_counters(unsigned long long *p, uint32_t n, unsigned long long rays0, hipStream_t st);
+// Tile pre-pass pieces: flags (tile_buf[t] = 1 if a ray of tile t enters the
+// AABB, t < T; tile_buf[T] = 0) and the misses of the inactive tiles.
+hipError_t launch_tile_flags(const MarchLaunch &L, uint32_t T, hipStream_t st);
+hipError_t launch_miss_tiles(const MarchLaunch &L, uint32_t T, hipStream_t st);
 
 // Grid build: bounds of n float3 points (device) into out[6] (device).
 hipError_t launch_bounds_reduce(const float *xyz, uint64_t n, float *partial, uint32_t npartial,

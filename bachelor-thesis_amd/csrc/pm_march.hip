@@ -518,22 +518,18 @@ __global__ __launch_bounds__(256) void k_tile_compact(uint32_t *tile_buf, uint32
     else if (flag[t]) order[pos[t]] = t;
 }
 
-// Pixels of the inactive tiles are written as misses (and counted) here.
+// Pixels of the inactive tiles are written as misses here.  (They are not
+// counted on the device: one counter for every wave would serialise on a
+// single address; the host knows the ray total.)
 __global__ __launch_bounds__(256) void k_miss_tiles(MarchLaunch L, uint32_t T)
 {
     const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6), j = threadIdx.x & 63u;
     if (t >= T) return;   // whole wave
-    bool miss = false;
     if (L.tile_buf[t] == 0) {
         const uint32_t tiles_x = (L.W + 7) / 8;
         const uint32_t px = (t % tiles_x) * 8 + (j & 7u), py = (t / tiles_x) * 8 + (j >> 3);
-        if (px < L.W && py < L.H) {
-            write_miss(L, py * L.W + px);
-            miss = true;
-        }
+        if (px < L.W && py < L.H) write_miss(L, py * L.W + px);
     }
-    const unsigned long long m = __ballot(miss);
-    if (j == 0 && m) atomicAdd(&L.counters[C_RAYS], (unsigned long long)__popcll(m));
 }
 
 // Schedule "tile": one 8x8 tile per wave, one ray per lane, all lanes in
@@ -848,7 +844,7 @@ __global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
     float *buf = region + lane;
     const bool use_lds = B.base.s.compensated != 0;
 
-    MarchLaunch L = B.base;
+    const MarchLaunch *Lp = B.jobs;
     Counts cn;
     RayState r{};
     r.phase = PH_DONE;
@@ -872,14 +868,7 @@ __global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
                     cidx = __builtin_amdgcn_readfirstlane(__shfl(cidx, 0, 64));
                     if (cidx >= B.nchunks) { exhausted = true; break; }
                     const BatchChunk ch = B.chunks[cidx];
-                    const BatchJob &J = B.jobs[ch.job];
-                    L.g = J.g;
-                    L.cg = J.cg;
-#pragma unroll
-                    for (int i = 0; i < 16; i++) L.M[i] = J.M[i];
-                    L.cam[0] = J.cam[0]; L.cam[1] = J.cam[1]; L.cam[2] = J.cam[2];
-                    L.W = J.W; L.H = J.H;
-                    L.pos = J.pos; L.nrm = J.nrm; L.t = J.t;
+                    Lp = B.jobs + __builtin_amdgcn_readfirstlane(ch.job);
                     tile0 = ch.tile0;
                     ntiles = ch.ntiles;
                     tile_i = 0;
@@ -890,9 +879,9 @@ __global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
             int take = __popcll(needy);
             if (r.phase == PH_DONE && qpos + rank < 64) {
                 uint32_t j = (uint32_t)(qpos + rank);
-                const uint32_t tiles_x = (L.W + 7) / 8;
+                const uint32_t tiles_x = (Lp->W + 7) / 8;
                 uint32_t px = (tile % tiles_x) * 8 + (j & 7u), py = (tile / tiles_x) * 8 + (j >> 3);
-                if (px < L.W && py < L.H) ray_setup(L, r, px, py, cn);
+                if (px < Lp->W && py < Lp->H) ray_setup(*Lp, r, px, py, cn);
             }
             qpos = qpos + take < 64 ? qpos + take : 64;
         }
@@ -906,17 +895,17 @@ __global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
         const int nwant = LEARNED ? __popcll(__ballot(r.phase == PH_PROPOSE)) : 0;
         const bool stalled = n64 + nwant == nbusy;
         if (LEARNED && nwant > 0 && (2 * nwant >= nbusy || exhausted || stalled || (qpos >= 64 && tile_i >= ntiles)))
-            wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+            wave_propose(*Lp, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
         const bool run64 = 4 * n64 >= nbusy || stalled;
         if (r.phase != PH_DONE && r.phase != PH_PROPOSE && (r.phase != PH_BISECT64 || run64)) {
-            if (use_lds) ray_step(L, r, DensLds<kLdsCap, 4>{ &L.g, buf }, cn);
-            else ray_step(L, r, DensPlain{ &L.g, false }, cn);
+            if (use_lds) ray_step(*Lp, r, DensLds<kLdsCap, 4>{ &Lp->g, buf }, cn);
+            else ray_step(*Lp, r, DensPlain{ &Lp->g, false }, cn);
         }
     }
-    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+    flush_counters(B.base.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
     if (LEARNED) {
         unsigned long long m = wave_sum(cn.mlp);
-        if (lane == 0) atomicAdd(&L.counters[C_MLP], m);
+        if (lane == 0) atomicAdd(&B.base.counters[C_MLP], m);
     }
 }
 
@@ -941,6 +930,29 @@ hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st)
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
+    return hipGetLastError();
+}
+
+__global__ void k_init_counters(unsigned long long *p, uint32_t n, unsigned long long rays0)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = i == (uint32_t)C_RAYS ? rays0 : 0ull;
+}
+
+hipError_t launch_init_counters(unsigned long long *p, uint32_t n, unsigned long long rays0, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, st, p, n, rays0);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_flags(const MarchLaunch &L, uint32_t T, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tile_flags, dim3((T + 1 + 3) / 4), dim3(256), 0, st, L, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_miss_tiles(const MarchLaunch &L, uint32_t T, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_miss_tiles, dim3((T + 3) / 4), dim3(256), 0, st, L, T);
     return hipGetLastError();
 }
 

@@ -51,8 +51,11 @@ def cpu_model() -> str:
 
 
 def alg_bytes(st: dict) -> float:
-    """12 B per stencil candidate, 72 B of row bounds per sample, 32 B of output per ray."""
-    return 12.0 * st["cand"] + 72.0 * st["samples"] + 32.0 * st["rays"]
+    """Bytes the march must read and write, from the launch's own counters: 12 B
+    (x, y, z) per stencil candidate, 200 B of row bounds per density sample (25
+    rows of the fine stencil, two u32 each) and 36 B of output per ray
+    (position and normal RGBA32F, t)."""
+    return 12.0 * st["cand"] + 200.0 * st["samples"] + 36.0 * st["rays"]
 
 
 def run_gpu(m, frame, settings, cam, steps, warmup):
