@@ -16,6 +16,9 @@ those whose image is within the tolerances of the plain image (mask >= 99.9 %,
 99.9th percentile depth error <= 1e-4 x extent and normal error <= 0.5 deg);
 both are reported.  Prints one JSON line.
 
+--workload clustered_4k: the 4,000,000-particle clustered pool at 3840x2160,
+same march and forms.
+
 --workload batch: 64 views (W ~ U{256..1920}, H ~ U{256..1080}, seeded orbit
 cameras) over 16 consecutive frames of the 1M dam break (1 % displacement per
 frame), all 1,024 view-frames in one persistent launch replayed from a
@@ -224,7 +227,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["dam_break_1080p", "batch"], default="dam_break_1080p")
+    ap.add_argument("--workload", choices=["dam_break_1080p", "clustered_4k", "batch"], default="dam_break_1080p")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
@@ -240,7 +243,15 @@ def main():
     import pmarch
     from pmarch import scenes, train
 
-    scene = scenes.dam_break(0)
+    if args.workload == "clustered_4k":
+        scene = scenes.clustered_pool(0)
+        wdesc = ("clustered pool 4,000,000 particles (pool layer, 200 droplets, splash cones), 3840x2160 view, "
+                 "configured march (AABB entry, step r/2, 8 bisections with the last in FP64, central-difference "
+                 "normals, exact sub-box bound skip)")
+    else:
+        scene = scenes.dam_break(0)
+        wdesc = ("dam_break 1,024,000 particles, 1920x1080 view, configured march (AABB entry, step r/2, 8 "
+                 "bisections with the last in FP64, central-difference normals, exact sub-box bound skip)")
     cam = scene.cameras[0]
     ext = float((scene.xyz.max(0) - scene.xyz.min(0)).max())
     rays = cam.width * cam.height
@@ -269,7 +280,7 @@ def main():
     st = f["stats"]
     B = alg_bytes(st)
     achieved = B / (f["kernel_ms"] * 1e-3) / 1e9
-    traffic = traffic_for(use, "dam_break_1080p")
+    traffic = traffic_for(use, args.workload)
     out = {
         "metric": "Mrays/s @1080p/1M particles, plain vs learned-step march, at fixed depth error",
         "value": round(f["mrays"], 3),
@@ -283,9 +294,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "dam_break 1,024,000 particles, 1920x1080 view, configured march (AABB entry, "
-                               "step r/2, 8 bisections with the last in FP64, central-difference normals, "
-                               "exact sub-box bound skip)",
+        "config": {"workload": wdesc,
                    "particles": scene.n, "width": cam.width, "height": cam.height, "h": scene.h,
                    "step": scene.step, "iso": scene.iso, "path": use},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
