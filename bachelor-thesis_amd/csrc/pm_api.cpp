@@ -43,8 +43,8 @@ struct DevBuf {
 struct Frame {
     pm_frame_info info{};
     GridDev g{};
-    DevBuf sorted, cell_start, cell_of, counts, cursor, scan_tmp, xyz, bounds_partial, bounds;
-    DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound;
+    DevBuf sorted, cell_start, cell_of, counts, cursor, scan_tmp, xyz, bounds_partial, bounds, scattered;
+    DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound, brick;
     DevBuf fsorted, fcell_start;    // march grid (cells of h/2)
     bool bound_ok = false;      // bound holds the sub-box bits for bound_iso
     bool dg_ok = false;         // the DensityGrid flags are built
@@ -55,8 +55,8 @@ struct Frame {
     {
         coarse.release();
         sorted.release(); cell_start.release(); cell_of.release(); counts.release(); cursor.release();
-        scan_tmp.release(); xyz.release(); bounds_partial.release(); bounds.release();
-        empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release(); bound.release();
+        scan_tmp.release(); xyz.release(); bounds_partial.release(); bounds.release(); scattered.release();
+        empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release(); bound.release(); brick.release();
         fsorted.release(); fcell_start.release();
     }
 };
@@ -71,6 +71,7 @@ struct pm_ctx {
     std::string err;
     // outputs of the last march
     DevBuf d_pos, d_nrm, d_t, d_depth, d_counters, d_q, d_out, d_band, d_tiles;
+    DevBuf d_hits;                     // per-pixel hit records of one march
     uint64_t npx = 0;
     bool have_t = false;
     // pinned staging for asynchronous hand-off
@@ -91,7 +92,7 @@ struct pm_ctx {
     // batched views: job and tile-chunk tables, all views' images, the captured launch
     uint64_t frame_epoch = 0;           // bumped whenever a frame's device buffers may move
     uint64_t batch_epoch = ~0ull;
-    DevBuf b_jobs, b_chunks, b_img, b_ctr, b_flags;
+    DevBuf b_jobs, b_chunks, b_img, b_ctr, b_flags, b_hits;
     unsigned long long b_rays0 = 0;     // rays of inactive tiles (written once as misses)
     bool prepass = false;               // the last single-view march ran the tile pre-pass
     std::vector<uint64_t> b_off;        // per view: first pixel in b_img
@@ -164,9 +165,9 @@ int pm_create(int device, pm_ctx **out)
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev_launch) != hipSuccess || hipEventCreate(&c->ev_k0) != hipSuccess ||
         hipEventCreate(&c->ev_k1) != hipSuccess || hipEventCreate(&c->ev_done) != hipSuccess ||
-        c->d_counters.ensure(sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
+        c->d_counters.ensure(sizeof(unsigned long long) * kCtrWords) != hipSuccess ||
         c->d_band.ensure(sizeof(uint32_t) * 64) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_counters, sizeof(unsigned long long) * C_COUNT, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&c->h_counters, sizeof(unsigned long long) * kCtrWords, hipHostMallocDefault) != hipSuccess) {
         pm_destroy(c);
         return PM_ERR_HIP;
     }
@@ -182,12 +183,14 @@ void pm_destroy(pm_ctx *c)
     for (auto &kv : c->frames) { kv.second->release(); delete kv.second; }
     c->d_pos.release(); c->d_nrm.release(); c->d_t.release(); c->d_depth.release();
     c->d_counters.release(); c->d_q.release(); c->d_out.release(); c->d_band.release(); c->d_tiles.release();
+    c->d_hits.release();
     c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release(); c->mlp_W1.release(); c->mlp_W2.release();
     c->mlp_W2T.release(); c->mlp_partial.release(); c->mlp_out.release(); c->mlp_x.release(); c->mlp_idx.release();
     c->ds_feat.release(); c->ds_label.release(); c->ds_hit.release(); c->ds_count.release(); c->ds_meta.release();
     if (c->b_exec) hipGraphExecDestroy(c->b_exec);
     if (c->b_graph) hipGraphDestroy(c->b_graph);
     c->b_jobs.release(); c->b_chunks.release(); c->b_img.release(); c->b_ctr.release(); c->b_flags.release();
+    c->b_hits.release();
     if (c->h_stage) hipHostFree(c->h_stage);
     if (c->h_counters) hipHostFree(c->h_counters);
     if (c->ev_launch) hipEventDestroy(c->ev_launch);
@@ -263,6 +266,7 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     HIPCHK(c, f->fsorted.ensure(sizeof(float4) * n));
     HIPCHK(c, f->fcell_start.ensure(sizeof(uint32_t) * (fnc + 1)));
     HIPCHK(c, f->cell_of.ensure(sizeof(uint32_t) * n));
+    HIPCHK(c, f->scattered.ensure(sizeof(float4) * n));
     HIPCHK(c, f->counts.ensure(sizeof(uint32_t) * (mc + 1)));
     HIPCHK(c, f->cursor.ensure(sizeof(uint32_t) * mc));
     HIPCHK(c, f->scan_tmp.ensure(sizeof(uint32_t) * scan_tmp_size((uint32_t)(mc + 1))));
@@ -278,6 +282,7 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     a.scan_tmp = (uint32_t *)f->scan_tmp.p;
     a.sorted = (float4 *)f->sorted.p;
     a.cell_start = (uint32_t *)f->cell_start.p;
+    a.scattered = (float4 *)f->scattered.p;
     HIPCHK(c, launch_grid_build(a, c->stream));
 
     GridBuildArgs fa = a;   // the march grid: same build, cells of h/2
@@ -427,7 +432,10 @@ static int ensure_bound(pm_ctx *c, Frame *f, float iso)
     if (f->bound_ok && f->bound_iso == iso) return PM_OK;
     HIPCHK(c, f->bound.ensure(sizeof(uint64_t) * f->info.ncells));
     f->g.bound = (const uint64_t *)f->bound.p;
-    HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, c->stream));
+    f->g.bnx = (f->g.nx + 3) / 4; f->g.bny = (f->g.ny + 3) / 4; f->g.bnz = (f->g.nz + 3) / 4;
+    HIPCHK(c, f->brick.ensure((uint64_t)f->g.bnx * f->g.bny * f->g.bnz));
+    f->g.brick = (const uint8_t *)f->brick.p;
+    HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, (uint8_t *)f->brick.p, c->stream));
     f->bound_ok = true;
     f->bound_iso = iso;
     c->frame_epoch++;
@@ -507,6 +515,10 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     L.t = (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
     L.band_ctr = (uint32_t *)c->d_band.p;
+    if (s->mode == PM_MODE_CONFIG) {
+        HIPCHK(c, c->d_hits.ensure(sizeof(uint4) * npx));
+        L.hit_rec = (uint4 *)c->d_hits.p;
+    }
     // tile order and miss pre-pass for the persistent schedules (variant 7: without)
     const bool persistent = s->learned || (s->variant != 1 && s->variant != 4 && s->variant != 5);
     if (s->mode == PM_MODE_CONFIG && persistent && s->variant != 7) {
@@ -529,11 +541,11 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
         HIPCHK(c, hipMemcpyAsync(c->d_depth.p, depth_in, sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
         L.depth = (const float *)c->d_depth.p;
     }
-    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, sizeof(unsigned long long) * C_COUNT, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, sizeof(unsigned long long) * kCtrWords, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
     HIPCHK(c, launch_march(L, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->d_counters.p, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->d_counters.p, sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, c->stream));
 
     c->user_pos = positions_out;
     c->user_nrm = normals_out;
@@ -571,13 +583,16 @@ static int complete(pm_ctx *c)
     // every pixel is answered once; the rays of the tiles the pre-pass writes
     // as misses are not counted on the device, so a march with the pre-pass
     // reports all W*H rays
-    S.rays = c->prepass ? c->npx : c->h_counters[C_RAYS];
-    S.rays_in_aabb = c->h_counters[C_IN_AABB];
-    S.hits = c->h_counters[C_HITS];
-    S.samples = c->h_counters[C_SAMPLES];
-    S.cand = c->h_counters[C_CAND];
-    S.march_samples = c->h_counters[C_MARCH_SAMPLES];
-    S.mlp_evals = c->h_counters[C_MLP];
+    unsigned long long sum[C_COUNT] = {};
+    for (int k = 0; k < kCtrSlots; k++)
+        for (int i = 0; i < C_COUNT; i++) sum[i] += c->h_counters[k * kCtrStride + i];
+    S.rays = c->prepass ? c->npx : sum[C_RAYS];
+    S.rays_in_aabb = sum[C_IN_AABB];
+    S.hits = sum[C_HITS];
+    S.samples = sum[C_SAMPLES];
+    S.cand = sum[C_CAND];
+    S.march_samples = sum[C_MARCH_SAMPLES];
+    S.mlp_evals = sum[C_MLP];
     hipEventElapsedTime(&S.kernel_ms, c->ev_k0, c->ev_k1);
     hipEventElapsedTime(&S.total_ms, c->ev_launch, c->ev_done);
     c->pending = false;
@@ -629,7 +644,7 @@ static constexpr uint32_t kChunkTiles = 16;   // 8x8 tiles per queue entry (1024
 
 static int batch_enqueue_launch(pm_ctx *c)
 {
-    HIPCHK(c, launch_init_counters(c->b_launch.base.counters, C_COUNT, c->b_rays0, c->stream));
+    HIPCHK(c, launch_init_counters(c->b_launch.base.counters, kCtrWords, c->b_rays0, c->stream));
     HIPCHK(c, launch_zero_u32(c->b_launch.chunk_ctr, 1, c->stream));
     HIPCHK(c, launch_march_batch(c->b_launch, c->b_blocks, c->stream));
     return PM_OK;
@@ -777,13 +792,16 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
     B.base.s = *s;
     B.base.inv_step = 1.0f / s->step;
     B.base.counters = (unsigned long long *)c->d_counters.p;
+    HIPCHK(c, c->b_hits.ensure(sizeof(uint4) * npx));
     if (s->learned) {
         B.base.net = mlp_dev(c);
         B.base.mlp_alpha = s->learned_alpha > 0.0f ? s->learned_alpha : 0.7f;
         B.base.mlp_min_skip = s->learned_min_skip > 0 ? s->learned_min_skip : 4;
         B.base.mlp_max_skip = s->learned_max_skip > 0 ? s->learned_max_skip : 24;
     }
-    for (MarchLaunch &J : jobs) {   // the shared fields of every job
+    for (uint32_t v = 0; v < n_views; v++) {   // the shared fields of every job
+        MarchLaunch &J = jobs[v];
+        J.hit_rec = (uint4 *)c->b_hits.p + c->b_off[v];
         J.s = B.base.s;
         J.inv_step = B.base.inv_step;
         J.counters = B.base.counters;
@@ -836,7 +854,7 @@ int pm_batch_run(pm_ctx *c)
         if (rc) return rc;
     }
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->d_counters.p, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->d_counters.p, sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_done, c->stream));
     c->user_pos = c->user_nrm = c->user_t = nullptr;
     c->prepass = false;   // the batch counts its answered-once rays in the counters

@@ -32,6 +32,8 @@ struct GridDev {
     float bmin[3], bmax[3];   // padded AABB
     const uint8_t *empty27;   // per cell: 1 if the cell and its 26 neighbours are empty
     const uint64_t *bound;    // per cell: bit l set when sub-box l (4x4x4) stays below iso
+    const uint8_t *brick;     // per 4x4x4 cells: 1 when every sub-box of every cell is set
+    int32_t bnx, bny, bnz;    // brick grid dims
     // reference DensityGrid over the padded AABB (Dataset.cpp:94-164)
     const uint8_t *dg_flag;   // flag per node for the selected skip rule
     int32_t dg_nx, dg_ny, dg_nz;
@@ -108,6 +110,25 @@ __device__ __forceinline__ void pm_fine_row(const GridDev &g, const FineWalk &w,
     const uint32_t row = w.base + (uint32_t)dy * w.sy + (uint32_t)dz * w.sz;
     b = g.fcell_start[row + (uint32_t)lo];
     e = g.fcell_start[row + (uint32_t)hi + 1];
+}
+
+// Row r of the canonical order (dz = r / 5 - 2 outer, dy = r % 5 - 2 inner),
+// without a branch: both bounds are loaded for every row (a dropped row loads
+// its first cell's bounds, in range whenever pm_fine_begin accepted the
+// centre) and a dropped row then gets the empty range.  r is wave-uniform.
+__device__ __forceinline__ void pm_fine_row_at(const GridDev &g, const FineWalk &w, int r, uint32_t &b, uint32_t &e)
+{
+    const int dz = r / 5 - 2, dy = r - 5 * (r / 5) - 2;
+    const float tz = pm_fine_off(dz, w.fz), ty = pm_fine_off(dy, w.fy);
+    const float r2 = ty * ty + tz * tz;
+    const bool kept = r2 < PM_FINE_LIM;
+    const int lo = !kept ? 0 : ((w.xm2 + r2 < PM_FINE_LIM) ? -2 : ((w.xm1 + r2 < PM_FINE_LIM) ? -1 : 0));
+    const int hi = !kept ? -1 : ((w.xp2 + r2 < PM_FINE_LIM) ? 2 : ((w.xp1 + r2 < PM_FINE_LIM) ? 1 : 0));
+    const uint32_t row = w.base + (uint32_t)(dy * (int)w.sy + dz * (int)w.sz);
+    const uint32_t lb = g.fcell_start[row + (uint32_t)lo];
+    const uint32_t le = g.fcell_start[row + (uint32_t)(hi + 1)];
+    b = kept ? lb : 0u;
+    e = kept ? le : 0u;
 }
 
 // CompactNSearch cell key: int(x / h), minus one for negative x.
@@ -267,22 +288,224 @@ __device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, fl
     return s + c;
 }
 
+// ---- fused central differences ----
+// The six samples q +- e along x, y, z (in this order: +x, -x, +y, -y, +z,
+// -z) lie in the box [q - e, q + e].  One walk over the march-grid cells near
+// that box -- per axis from the fine cell of the box's low end to that of its
+// high end, +-2 cells; rows and row ends kept by their distance to the box with
+// the limit of pm_fine_row -- visits in the canonical order every cell that any
+// of the six single-sample stencils keeps (the box is nearer to each cell than
+// any of the six samples).  A candidate within (h + e)(1 + 2^-10) of q is kept
+// and tested against each sample with that sample's own r^2 expression, so
+// every sample's sum is its single-sample sum, bit for bit.
+__device__ __forceinline__ float pm_box_off(float c, float ulo, float uhi, float kl, float kh)
+{
+    return c > kh ? c - uhi : (c < kl ? ulo - (c + 1.0f) : 0.0f);
+}
+
+struct BoxWalk {
+    float ulo[3], uhi[3];     // box in fine-cell units
+    float kl[3], kh[3];       // fine keys of the box ends
+    uint32_t base, sy, sz;    // dense index of cell kl - 2 (per axis); strides
+    int nx, ny, nz;           // cells per axis (kh - kl + 5)
+    float xm2, xm1, xp1, xp2; // squared x-distances of the four trimmable end cells
+};
+
+// false when the cells leave the march grid (the caller then takes the six
+// samples one at a time).
+__device__ __forceinline__ bool pm_box_begin(const GridDev &g, const float lo[3], const float hi[3], BoxWalk &w)
+{
+    const int32_t fo[3] = { g.fox, g.foy, g.foz }, fn[3] = { g.fnx, g.fny, g.fnz };
+    int32_t c0[3];
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        w.ulo[a] = g.inv_hf * lo[a];
+        w.uhi[a] = g.inv_hf * hi[a];
+        const int32_t kl = lo[a] >= 0.0f ? (int32_t)w.ulo[a] : (int32_t)w.ulo[a] - 1;
+        const int32_t kh = hi[a] >= 0.0f ? (int32_t)w.uhi[a] : (int32_t)w.uhi[a] - 1;
+        w.kl[a] = (float)kl;
+        w.kh[a] = (float)kh;
+        c0[a] = kl - 2 - fo[a];
+        ok = ok && c0[a] >= 0 && kh + 2 - fo[a] <= fn[a] - 1;
+        if (a == 0) w.nx = kh - kl + 5;
+        if (a == 1) w.ny = kh - kl + 5;
+        if (a == 2) w.nz = kh - kl + 5;
+    }
+    w.sy = (uint32_t)g.fnx;
+    w.sz = (uint32_t)g.fnx * (uint32_t)g.fny;
+    w.base = (uint32_t)c0[0] + w.sy * (uint32_t)c0[1] + w.sz * (uint32_t)c0[2];
+    float t;
+    t = pm_box_off(w.kl[0] - 2.0f, w.ulo[0], w.uhi[0], w.kl[0], w.kh[0]); w.xm2 = t * t;
+    t = pm_box_off(w.kl[0] - 1.0f, w.ulo[0], w.uhi[0], w.kl[0], w.kh[0]); w.xm1 = t * t;
+    t = pm_box_off(w.kh[0] + 1.0f, w.ulo[0], w.uhi[0], w.kl[0], w.kh[0]); w.xp1 = t * t;
+    t = pm_box_off(w.kh[0] + 2.0f, w.ulo[0], w.uhi[0], w.kl[0], w.kh[0]); w.xp2 = t * t;
+    return ok;
+}
+
+// Range [b, e) of row (jy, jz) (cell offsets from kl - 2), empty when the row
+// is dropped or jz is past the box's rows; both bounds are always loaded.
+__device__ __forceinline__ void pm_box_row(const GridDev &g, const BoxWalk &w, int jy, int jz, uint32_t &b,
+                                           uint32_t &e)
+{
+    const bool valid = jz < w.nz;
+    const float ty = pm_box_off(w.kl[1] - 2.0f + (float)jy, w.ulo[1], w.uhi[1], w.kl[1], w.kh[1]);
+    const float tz = pm_box_off(w.kl[2] - 2.0f + (float)jz, w.ulo[2], w.uhi[2], w.kl[2], w.kh[2]);
+    const float r2 = ty * ty + tz * tz;
+    const bool kept = valid && r2 < PM_FINE_LIM;
+    const int lo = !kept ? 0 : ((w.xm2 + r2 < PM_FINE_LIM) ? 0 : ((w.xm1 + r2 < PM_FINE_LIM) ? 1 : 2));
+    const int hi = !kept ? -1 : ((w.xp2 + r2 < PM_FINE_LIM) ? w.nx - 1 : ((w.xp1 + r2 < PM_FINE_LIM) ? w.nx - 2 : w.nx - 3));
+    const uint32_t row = valid ? w.base + (uint32_t)jy * w.sy + (uint32_t)jz * w.sz : w.base;
+    const uint32_t lb = g.fcell_start[row + (uint32_t)lo];
+    const uint32_t le = g.fcell_start[row + (uint32_t)(hi + 1)];
+    b = kept ? lb : 0u;
+    e = kept ? le : 0u;
+}
+
+struct Cdiff6 {
+    float v[6];      // densities at +x, -x, +y, -y, +z, -z
+    uint32_t cand;   // candidates of the walk; 0xffffffff: no fused walk
+};
+
+struct Acc6 {
+    float s[6], c[6];
+};
+
+// Adds candidate p to the six sums (each sample's own r^2 expression, as in
+// pm_density32: (rx rx + ry ry) + rz rz).
+__device__ __forceinline__ void pm_acc6(const GridDev &g, float px, float py, float pz, const float x[3],
+                                        const float xp[3], const float xm[3], bool comp, Acc6 &a)
+{
+    const float rx0 = px - x[0], ry0 = py - x[1], rz0 = pz - x[2];
+    const float sx0 = rx0 * rx0, sy0 = ry0 * ry0, sz0 = rz0 * rz0;
+    const float rxp = px - xp[0], rxm = px - xm[0];
+    const float ryp = py - xp[1], rym = py - xm[1];
+    const float rzp = pz - xp[2], rzm = pz - xm[2];
+    float r2[6];
+    r2[0] = (rxp * rxp + sy0) + sz0;
+    r2[1] = (rxm * rxm + sy0) + sz0;
+    r2[2] = (sx0 + ryp * ryp) + sz0;
+    r2[3] = (sx0 + rym * rym) + sz0;
+    const float sxy = sx0 + sy0;
+    r2[4] = sxy + rzp * rzp;
+    r2[5] = sxy + rzm * rzm;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        if (r2[k] < g.h2) {
+            const float w = pm_W_in(r2[k], g.inv_h, g.sig);
+            if (comp) pm_neu_add(a.s[k], a.c[k], w);
+            else a.s[k] += w;
+        }
+    }
+}
+
+// The six central-difference densities of q = (x[0], x[1], x[2]) in one walk.
+// buf: the lane's compaction slots (lane-interleaved, CAP floats; entries of
+// three floats), or nullptr to add every kept candidate at once.
+template <int CAP>
+__device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy, float qz, float eps, float *buf,
+                                         bool comp)
+{
+    constexpr int CAPE = CAP / 3;
+    Cdiff6 out;
+    const float x[3] = { qx, qy, qz };
+    const float xp[3] = { qx + eps, qy + eps, qz + eps }, xm[3] = { qx - eps, qy - eps, qz - eps };
+    BoxWalk w;
+    if (!pm_box_begin(g, xm, xp, w)) {
+        out.cand = 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < 6; k++) out.v[k] = 0.0f;
+        return out;
+    }
+    const float he = (g.h + eps) * 1.0009765625f;
+    const float R2 = he * he;
+    Acc6 a;
+#pragma unroll
+    for (int k = 0; k < 6; k++) { a.s[k] = 0.0f; a.c[k] = 0.0f; }
+    uint32_t cand = 0;
+    int n = 0;
+    int jy = 0, jz = 0;
+    uint32_t nb, ne;
+    pm_box_row(g, w, 0, 0, nb, ne);
+    const int nr = w.ny * w.nz;
+    _Pragma("unroll 1") for (int r = 0; __ballot(r < nr) != 0; r++) {
+        const uint32_t b = nb, e = ne;
+        cand += e - b;
+        const bool any = __ballot(e > b) != 0;
+        if (++jy == w.ny) { jy = 0; jz++; }
+        pm_box_row(g, w, jy, jz, nb, ne);                 // next row in flight
+        if (!any) continue;
+        const int m = (int)(e - b);
+        const float4 *const p0 = g.fsorted + b;
+        if (buf == nullptr) {
+            for (int i = 0; i < m; i++) {
+                const float4 p = p0[i];
+                const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+                if ((rx * rx + ry * ry) + rz * rz < R2) pm_acc6(g, p.x, p.y, p.z, x, xp, xm, comp, a);
+            }
+            continue;
+        }
+        if (__ballot(n + m >= CAPE) != 0) {              // drain all lanes together
+            for (int j = 0; j < n; j++)
+                pm_acc6(g, buf[(3 * j) * 64], buf[(3 * j + 1) * 64], buf[(3 * j + 2) * 64], x, xp, xm, comp, a);
+            n = 0;
+        }
+        if (m < CAPE) {
+            for (int i = 0; i < m; i += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
+                    buf[(3 * n) * 64] = v[u].x;
+                    buf[(3 * n + 1) * 64] = v[u].y;
+                    buf[(3 * n + 2) * 64] = v[u].z;
+                    n += (i + u < m && (rx * rx + ry * ry) + rz * rz < R2) ? 1 : 0;
+                }
+            }
+        } else {                                          // a row longer than the buffer
+            for (int i = 0; i < m; i++) {
+                const float4 p = p0[i];
+                const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+                if ((rx * rx + ry * ry) + rz * rz < R2) pm_acc6(g, p.x, p.y, p.z, x, xp, xm, comp, a);
+            }
+        }
+    }
+    if (buf != nullptr)
+        for (int j = 0; j < n; j++)
+            pm_acc6(g, buf[(3 * j) * 64], buf[(3 * j + 1) * 64], buf[(3 * j + 2) * 64], x, xp, xm, comp, a);
+    out.cand = cand;
+#pragma unroll
+    for (int k = 0; k < 6; k++) out.v[k] = comp ? a.s[k] + a.c[k] : a.s[k];
+    return out;
+}
+
 // FP64 density; the stencil is chosen from the FP32-rounded point.
 __device__ __forceinline__ double pm_density64(const GridDev &g, double qx, double qy, double qz)
 {
     FineWalk fw;
     double acc = 0.0;
     if (!pm_fine_begin(g, (float)qx, (float)qy, (float)qz, fw)) return 0.0;
-    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++) {
-        const float tz2 = pm_fine_tz2(fw, dz);
-        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
-            uint32_t b, e;
-            pm_fine_row(g, fw, dy, dz, tz2, b, e);
-            for (uint32_t i = b; i < e; i++) {
-                float4 p = g.fsorted[i];
-                double rx = (double)p.x - qx, ry = (double)p.y - qy, rz = (double)p.z - qz;
+    // candidates are loaded in groups of 4 (the last group re-reads one of
+    // the row's candidates in its unused slots) with the next row's bounds in
+    // flight: one round trip per row while it has <= 4 candidates
+    uint32_t nb, ne;
+    pm_fine_row_at(g, fw, 0, nb, ne);
+    _Pragma("unroll 1") for (int r = 0; r < 25; r++) {
+        const uint32_t b = nb, e = ne;
+        if (r < 24) pm_fine_row_at(g, fw, r + 1, nb, ne);
+        const float4 *const p0 = g.fsorted + b;
+        const int m = (int)(e - b);
+        for (int i = 0; i < m; i += 4) {
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                double rx = (double)v[u].x - qx, ry = (double)v[u].y - qy, rz = (double)v[u].z - qz;
                 double r2 = (rx * rx + ry * ry) + rz * rz;
-                if (r2 < g.h2_d) acc += pm_W_in_d(r2, g.hinv_d, g.sig_d);
+                if (i + u < m && r2 < g.h2_d) acc += pm_W_in_d(r2, g.hinv_d, g.sig_d);
             }
         }
     }

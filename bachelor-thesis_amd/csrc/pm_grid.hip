@@ -173,26 +173,25 @@ __global__ __launch_bounds__(kThreads) void k_scatter(GridBuildArgs a)
     if (i >= a.n) return;
     uint32_t c = a.cell_of[i];
     uint32_t slot = a.cell_start[c] + atomicAdd(&a.cursor[c], 1u);
-    a.sorted[slot] = make_float4(a.xyz[3 * i + 0], a.xyz[3 * i + 1], a.xyz[3 * i + 2], __uint_as_float((uint32_t)i));
+    a.scattered[slot] = make_float4(a.xyz[3 * i + 0], a.xyz[3 * i + 1], a.xyz[3 * i + 2], __uint_as_float((uint32_t)i));
 }
 
-// Insertion sort of each cell's particles by input index (cells hold O(10-100)).
-__global__ __launch_bounds__(kThreads) void k_cell_sort(float4 *__restrict__ sorted, const uint32_t *__restrict__ cell_start,
-                                                        uint32_t ncells)
+// Order inside each cell by input index: the particle at scattered slot s goes
+// to cell_start[c] + (number of particles of its cell with a smaller index).
+// One thread per particle, read-only over its own cell: no dependent swaps and
+// no atomics, so dense cells cost O(k) per particle in parallel rather than an
+// O(k^2) serial insertion sort.
+__global__ __launch_bounds__(kThreads) void k_cell_rank(GridBuildArgs a)
 {
-    uint32_t c = blockIdx.x * kThreads + threadIdx.x;
-    if (c >= ncells) return;
-    uint32_t b = cell_start[c], e = cell_start[c + 1];
-    for (uint32_t i = b + 1; i < e; i++) {
-        float4 v = sorted[i];
-        uint32_t key = __float_as_uint(v.w);
-        uint32_t j = i;
-        while (j > b && __float_as_uint(sorted[j - 1].w) > key) {
-            sorted[j] = sorted[j - 1];
-            j--;
-        }
-        sorted[j] = v;
-    }
+    uint64_t s = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (s >= a.n) return;
+    const float4 v = a.scattered[s];
+    const uint32_t key = __float_as_uint(v.w);
+    const uint32_t c = a.cell_of[key];
+    const uint32_t b = a.cell_start[c], e = a.cell_start[c + 1];
+    uint32_t rank = 0;
+    for (uint32_t j = b; j < e; j++) rank += __float_as_uint(a.scattered[j].w) < key ? 1u : 0u;
+    a.sorted[b + rank] = v;
 }
 
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t len, uint32_t *tmp, hipStream_t st)
@@ -218,8 +217,7 @@ hipError_t launch_grid_build(const GridBuildArgs &a, hipStream_t st)
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, st, a.scan_tmp, nb);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kThreads), 0, st, a.cell_start, len, a.scan_tmp);
     hipLaunchKernelGGL(k_scatter, dim3(pblocks), dim3(kThreads), 0, st, a);
-    uint32_t cblocks = (a.ncells + kThreads - 1) / kThreads;
-    hipLaunchKernelGGL(k_cell_sort, dim3(cblocks), dim3(kThreads), 0, st, a.sorted, a.cell_start, a.ncells);
+    hipLaunchKernelGGL(k_cell_rank, dim3(pblocks), dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
 
@@ -261,6 +259,17 @@ hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz
 // cells (no stencil, zero density) and cells with an empty 27-neighbourhood get
 // every bit.  One wave per cell: lane l evaluates sub-box l, all lanes read
 // the same particle, and the walk stops once every lane has reached the limit.
+// W at the distance from particle p to the box [l, h] (0 outside the support).
+__device__ __forceinline__ float bound_term(const GridDev &g, float4 p, float lx, float hx, float ly, float hy, float lz,
+                                            float hz)
+{
+    const float ex = fmaxf(fmaxf(lx - p.x, p.x - hx), 0.0f);
+    const float ey = fmaxf(fmaxf(ly - p.y, p.y - hy), 0.0f);
+    const float ez = fmaxf(fmaxf(lz - p.z, p.z - hz), 0.0f);
+    const float d2 = (ex * ex + ey * ey) + ez * ez;
+    return d2 < g.h2 ? pm_W_in(d2, g.inv_h, g.sig) : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64_t *__restrict__ out)
 {
     const uint32_t lane = threadIdx.x & 63;
@@ -286,14 +295,16 @@ __global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64
         for (int dy = -1; dy <= 1; dy++) {
             const uint32_t row = (uint32_t)x - 1u + nx * ((uint32_t)(y + dy) + ny * (uint32_t)(z + dz));
             const uint32_t b = g.cell_start[row], e = g.cell_start[row + 3];
-            for (uint32_t j = b; j < e; j++) {
-                const float4 p = g.sorted[j];
-                const float ex = fmaxf(fmaxf(lx - p.x, p.x - hx), 0.0f);
-                const float ey = fmaxf(fmaxf(ly - p.y, p.y - hy), 0.0f);
-                const float ez = fmaxf(fmaxf(lz - p.z, p.z - hz), 0.0f);
-                const float d2 = (ex * ex + ey * ey) + ez * ez;
-                if (d2 < g.h2) U += pm_W_in(d2, g.inv_h, g.sig);
+            // all lanes read the same particles (broadcast); 8 loads in flight
+            uint32_t j = b;
+            for (; j + 8 <= e; j += 8) {
+                float4 p[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) p[u] = g.sorted[j + u];
+#pragma unroll
+                for (int u = 0; u < 8; u++) U += bound_term(g, p[u], lx, hx, ly, hy, lz, hz);
             }
+            for (; j < e; j++) U += bound_term(g, g.sorted[j], lx, hx, ly, hy, lz, hz);
             if (__ballot(U < lim) == 0) {
                 if (lane == 0) out[c] = 0ull;
                 return;
@@ -303,10 +314,34 @@ __global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64
     if (lane == 0) out[c] = m;
 }
 
-hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, hipStream_t st)
+// brick[b] = 1 when brick b (4x4x4 cells) lies inside the grid and every
+// sub-box of its 64 cells is set. A brick reaching past the grid stays 0: the
+// march evaluates every sample outside the grid, so skipping one would change
+// which samples are evaluated.
+__global__ __launch_bounds__(256) void k_bound_bricks(GridDev g, const uint64_t *__restrict__ bound,
+                                                      uint8_t *__restrict__ brick)
+{
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= (uint32_t)g.bnx * g.bny * g.bnz) return;
+    const int32_t bx = (int32_t)(b % g.bnx), by = (int32_t)((b / g.bnx) % g.bny), bz = (int32_t)(b / (g.bnx * g.bny));
+    bool all = 4 * bx + 4 <= g.nx && 4 * by + 4 <= g.ny && 4 * bz + 4 <= g.nz;
+    for (int32_t z = 4 * bz; z < 4 * bz + 4 && all; z++)
+        for (int32_t y = 4 * by; y < 4 * by + 4 && all; y++)
+            for (int32_t x = 4 * bx; x < 4 * bx + 4; x++) {
+                if (bound[(uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z)] != ~0ull) {
+                    all = false;
+                    break;
+                }
+            }
+    brick[b] = all ? 1 : 0;
+}
+
+hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, uint8_t *brick, hipStream_t st)
 {
     uint32_t n = (uint32_t)g.nx * g.ny * g.nz;
     hipLaunchKernelGGL(k_bound_grid, dim3((n + 3) / 4), dim3(256), 0, st, g, iso, out);
+    uint32_t nb = (uint32_t)g.bnx * g.bny * g.bnz;
+    hipLaunchKernelGGL(k_bound_bricks, dim3((nb + 255) / 256), dim3(256), 0, st, g, (const uint64_t *)out, brick);
     return hipGetLastError();
 }
 

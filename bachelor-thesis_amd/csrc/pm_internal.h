@@ -12,6 +12,10 @@ namespace pm {
 
 // Per-launch counters (u64 each), accumulated by the march kernel.
 enum Counter { C_RAYS = 0, C_IN_AABB, C_HITS, C_SAMPLES, C_CAND, C_MARCH_SAMPLES, C_MLP, C_COUNT };
+// The counters live in kCtrSlots copies, one per XCD (a workgroup adds to
+// slot blockIdx % 8), kCtrStride u64 (one 128-byte line) apart: atomics of
+// different XCDs never meet on one address.  The host sums the slots.
+constexpr int kCtrSlots = 8, kCtrStride = 16, kCtrWords = kCtrSlots * kCtrStride;
 
 // Coarse occupancy grid of the step-proposal features: 4x4x4 dense cells.
 struct CoarseDev {
@@ -42,6 +46,9 @@ struct MarchLaunch {
     // 3 (T + 1) + scan_tmp_size(T + 1) u32; null = every tile
     uint32_t *tile_buf;
     float inv_step;            // 1 / s.step (bound skip)
+    // per pixel: the hit handed from the march to k_finish (1 | fp64 << 1,
+    // lo, hi, 0), zeros for a miss; written for every pixel of a marched tile
+    uint4 *hit_rec;
 };
 
 // Batched launch: many (view, frame) jobs in one persistent kernel; every job
@@ -59,7 +66,7 @@ struct BatchLaunch {
     uint32_t nchunks;
     uint32_t *chunk_ctr;
 };
-hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, hipStream_t st);
+hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, uint8_t *brick, hipStream_t st);
 hipError_t launch_march_batch(const BatchLaunch &B, uint32_t blocks, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st);
@@ -87,6 +94,7 @@ struct GridBuildArgs {
     uint32_t *scan_tmp;    // scan_tmp_size(ncells + 1)
     float4 *sorted;        // n
     uint32_t *cell_start;  // ncells + 1
+    float4 *scattered;     // n: cell-major, arbitrary order inside a cell
 };
 size_t scan_tmp_size(uint32_t len);
 // Exclusive scan of len u32 (out may not alias in); tmp of scan_tmp_size(len).

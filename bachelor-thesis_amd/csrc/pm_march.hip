@@ -18,12 +18,13 @@
 
 namespace pm {
 
+// PH_BISECT: FP32 bisection steps (the hit then goes to k_finish: the FP64
+// step, the normal and the outputs).
 // PH_PROPOSE: waiting for a step proposal (taken by the whole wave at once);
 // PH_LAND: density at the proposed sample; PH_BSEARCH: binary search for the
 // first sample at or above iso between the last sample below iso and the landing.
 enum Phase : int {
-    PH_MARCH = 0, PH_BISECT = 1, PH_BISECT64 = 2, PH_NORMAL = 3, PH_GRADW = 4, PH_DONE = 5,
-    PH_PROPOSE = 6, PH_LAND = 7, PH_BSEARCH = 8
+    PH_MARCH = 0, PH_BISECT = 1, PH_DONE = 5, PH_PROPOSE = 6, PH_LAND = 7, PH_BSEARCH = 8
 };
 
 // glm mat4 * vec4 (type_mat4x4.inl): (m0 x + m1 y) + (m2 z + m3 w), column-major M.
@@ -117,18 +118,27 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
+// This workgroup's counter slot.
+__device__ __forceinline__ unsigned long long *ctr_slot(unsigned long long *ctr)
+{
+    return ctr + (blockIdx.x & (kCtrSlots - 1)) * kCtrStride;
+}
+
+// Wave totals into the workgroup's slot; zero totals add nothing (atomics on
+// one line serialise, so each saved one counts).
 __device__ __forceinline__ void flush_counters(unsigned long long *ctr, uint32_t rays, uint32_t in_aabb, uint32_t hits,
                                                uint32_t samples, unsigned long long cand, uint32_t msamples)
 {
     unsigned long long r = wave_sum(rays), a = wave_sum(in_aabb), h = wave_sum(hits);
     unsigned long long s = wave_sum(samples), c = wave_sum(cand), m = wave_sum(msamples);
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&ctr[C_RAYS], r);
-        atomicAdd(&ctr[C_IN_AABB], a);
-        atomicAdd(&ctr[C_HITS], h);
-        atomicAdd(&ctr[C_SAMPLES], s);
-        atomicAdd(&ctr[C_CAND], c);
-        atomicAdd(&ctr[C_MARCH_SAMPLES], m);
+        unsigned long long *p = ctr_slot(ctr);
+        if (r) atomicAdd(&p[C_RAYS], r);
+        if (a) atomicAdd(&p[C_IN_AABB], a);
+        if (h) atomicAdd(&p[C_HITS], h);
+        if (s) atomicAdd(&p[C_SAMPLES], s);
+        if (c) atomicAdd(&p[C_CAND], c);
+        if (m) atomicAdd(&p[C_MARCH_SAMPLES], m);
     }
 }
 
@@ -147,14 +157,13 @@ __device__ __forceinline__ bool pm_tile(uint32_t W, uint32_t H, uint32_t &px, ui
     return true;
 }
 
-// Per-ray state of the configured march, carried across loop iterations.
+// Per-ray state of the march, carried across loop iterations.  The march of
+// a ray ends with its hit record for k_finish, or with its miss.
 struct RayState {
     float d0, d1, d2;     // direction (origin = camera)
     float tn, tf;         // AABB entry / exit
-    float lo, hi, th;     // bisection bracket, final t
-    float xs0, xs1, xs2;  // hit point
-    float gv0, gv1, gv2, rplus;
-    int k, it, ncd, phase;
+    float lo, hi;         // bisection bracket
+    int k, it, phase;     // sample index, bisection step
     uint32_t idx;
     int klo, khi;         // learned step: last sample below iso / landing sample
     int lrn;              // 1 while the ray still takes proposals
@@ -173,18 +182,31 @@ __device__ __forceinline__ void write_miss(const MarchLaunch &L, uint32_t idx)
     if (L.t) L.t[idx] = -1.0f;
 }
 
-// Ray of pixel (px, py) (RayMarcher.cpp:268-274, with clip z = 1) and its AABB
-// interval.  Leaves phase = PH_MARCH, or writes the miss and PH_DONE.
-__device__ __forceinline__ void ray_setup(const MarchLaunch &L, RayState &r, uint32_t px, uint32_t py, Counts &cn)
+// A miss of the configured march: the outputs, and no hit record.
+__device__ __forceinline__ void march_miss(const MarchLaunch &L, uint32_t idx)
 {
-    cn.rays++;
-    r.idx = py * L.W + px;
+    write_miss(L, idx);
+    L.hit_rec[idx] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Direction of the ray of pixel (px, py) (RayMarcher.cpp:268-274, with clip z = 1).
+__device__ __forceinline__ void pm_ray_dir(const MarchLaunch &L, uint32_t px, uint32_t py, float d[3])
+{
     float cx = (float)px * (2.0f / (float)L.W) - 1.0f;
     float cy = (float)py * (2.0f / (float)L.H) - 1.0f;
     float wx, wy, wz;
     pm_unproject(L.M, cx, cy, 1.0f, wx, wy, wz);
-    float d[3];
     pm_normalize(wx - L.cam[0], wy - L.cam[1], wz - L.cam[2], d[0], d[1], d[2]);
+}
+
+// Ray of pixel (px, py) and its AABB interval.  Leaves phase = PH_MARCH, or
+// writes the miss and PH_DONE.
+__device__ __forceinline__ void ray_setup(const MarchLaunch &L, RayState &r, uint32_t px, uint32_t py, Counts &cn)
+{
+    cn.rays++;
+    r.idx = py * L.W + px;
+    float d[3];
+    pm_ray_dir(L, px, py, d);
     r.d0 = d[0]; r.d1 = d[1]; r.d2 = d[2];
     r.id0 = 1.0f / d[0]; r.id1 = 1.0f / d[1]; r.id2 = 1.0f / d[2];
     float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
@@ -198,7 +220,7 @@ __device__ __forceinline__ void ray_setup(const MarchLaunch &L, RayState &r, uin
         }
         cn.in_aabb++;
     } else {
-        write_miss(L, r.idx);
+        march_miss(L, r.idx);
         r.phase = PH_DONE;
     }
 }
@@ -254,30 +276,44 @@ __device__ __forceinline__ int pm_skip_bound(const GridDev &g, float o0, float o
     if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) return r.k;
     const uint64_t m = g.bound[(uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz)];
     if (m == 0ull) return r.k;
-    const int s0 = min(3, (int)((u0 - (float)k0) * 4.0f));
-    const int s1 = min(3, (int)((u1 - (float)k1) * 4.0f));
-    const int s2 = min(3, (int)((u2 - (float)k2) * 4.0f));
-    if (!((m >> (s0 + 4 * s1 + 16 * s2)) & 1ull)) return r.k;
+    // the box to leave: the 4x4x4-cell brick or the cell when every sub-box
+    // of it is set, else the sample's own sub-box (key units: lo, lo + span)
+    float l0, l1, l2, span;
+    if (m == ~0ull) {
+        const int32_t bx = cx >> 2, by = cy >> 2, bz = cz >> 2;
+        if (g.brick[(uint32_t)bx + (uint32_t)g.bnx * ((uint32_t)by + (uint32_t)g.bny * (uint32_t)bz)]) {
+            l0 = (float)(4 * bx + g.ox); l1 = (float)(4 * by + g.oy); l2 = (float)(4 * bz + g.oz);
+            span = 4.0f;
+        } else {
+            l0 = (float)k0; l1 = (float)k1; l2 = (float)k2;
+            span = 1.0f;
+        }
+    } else {
+        const int s0 = min(3, (int)((u0 - (float)k0) * 4.0f));
+        const int s1 = min(3, (int)((u1 - (float)k1) * 4.0f));
+        const int s2 = min(3, (int)((u2 - (float)k2) * 4.0f));
+        if (!((m >> (s0 + 4 * s1 + 16 * s2)) & 1ull)) return r.k;
+        l0 = (float)k0 + 0.25f * (float)s0; l1 = (float)k1 + 0.25f * (float)s1; l2 = (float)k2 + 0.25f * (float)s2;
+        span = 0.25f;
+    }
     float texit = INFINITY;
-    texit = pm_box_exit(((float)k0 + 0.25f * (float)s0) * g.h, ((float)k0 + 0.25f * (float)(s0 + 1)) * g.h, o0, r.d0, r.id0, texit);
-    texit = pm_box_exit(((float)k1 + 0.25f * (float)s1) * g.h, ((float)k1 + 0.25f * (float)(s1 + 1)) * g.h, o1, r.d1, r.id1, texit);
-    texit = pm_box_exit(((float)k2 + 0.25f * (float)s2) * g.h, ((float)k2 + 0.25f * (float)(s2 + 1)) * g.h, o2, r.d2, r.id2, texit);
+    texit = pm_box_exit(l0 * g.h, (l0 + span) * g.h, o0, r.d0, r.id0, texit);
+    texit = pm_box_exit(l1 * g.h, (l1 + span) * g.h, o1, r.d1, r.id1, texit);
+    texit = pm_box_exit(l2 * g.h, (l2 + span) * g.h, o2, r.d2, r.id2, texit);
     float kq = (texit - r.tn) * inv_step;
     kq -= 0x1p-8f + fabsf(kq) * 0x1p-18f;
     const float kf = floorf(kq);
     return kf >= (float)r.k ? (int)kf + 1 : r.k + 1;
 }
 
-__device__ __forceinline__ void to_normals(const pm_settings &s, RayState &r)
-{
-    r.phase = s.normal_mode == PM_NORMAL_GRADW ? PH_GRADW : PH_NORMAL;
-    r.ncd = 0;
-}
 
-__device__ __forceinline__ void set_hit_point(const MarchLaunch &L, RayState &r, float t)
+// The ray's hit is known up to what k_finish does: its record (at the
+// pixel) holds t itself (fp64 = false), or the bracket [lo, hi] of the FP64
+// bisection step.  Record: (1 | fp64 << 1, lo, hi, 0); a miss leaves zeros.
+__device__ __forceinline__ void to_emit(const MarchLaunch &L, RayState &r, bool fp64, float lo, float hi)
 {
-    r.th = t;
-    r.xs0 = L.cam[0] + t * r.d0; r.xs1 = L.cam[1] + t * r.d1; r.xs2 = L.cam[2] + t * r.d2;
+    L.hit_rec[r.idx] = make_uint4(fp64 ? 3u : 1u, __float_as_uint(lo), __float_as_uint(hi), 0u);
+    r.phase = PH_DONE;
 }
 
 // The first sample k at or above iso is known: bracket [t_{k-1}, t_k] and
@@ -286,58 +322,26 @@ __device__ __forceinline__ void start_bracket(const MarchLaunch &L, RayState &r,
 {
     const pm_settings &s = L.s;
     if (r.k == 0) {
-        set_hit_point(L, r, r.tn);
-        to_normals(s, r);
+        to_emit(L, r, false, r.tn, r.tn);
         return;
     }
     r.lo = r.tn + (float)(r.k - 1) * s.step;
     r.hi = r.tn + (float)r.k * s.step;
     r.it = 0;
     if (n32 > 0) r.phase = PH_BISECT;
-    else if (s.final_fp64 && s.bisect_iters > 0) r.phase = PH_BISECT64;
-    else { set_hit_point(L, r, (r.lo + r.hi) * 0.5f); to_normals(s, r); }
+    else if (s.final_fp64 && s.bisect_iters > 0) to_emit(L, r, true, r.lo, r.hi);
+    else { const float t = (r.lo + r.hi) * 0.5f; to_emit(L, r, false, t, t); }
 }
 
-// First half of one step: phases that need no FP32 density (FP64 bisection
-// step, sum-of-gradW normals, the end of the ray) are completed here; otherwise
-// the lane's FP32 density sample point is returned in q and the result is
-// consumed by ray_finish.
+// First half of one step: the sample point of the lane's next FP32 density
+// evaluation, after the skips of the march phase; false when the lane takes
+// no sample this step (the ray ended, or it used up its skips).
 __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, float &qx, float &qy, float &qz,
                                             Counts &cn)
 {
     const pm_settings &s = L.s;
     const GridDev &g = L.g;
     const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
-
-    if (r.phase == PH_BISECT64) {
-        double tmd = ((double)r.lo + (double)r.hi) * 0.5;
-        double x = (double)o0 + tmd * (double)r.d0;
-        double y = (double)o1 + tmd * (double)r.d1;
-        double z = (double)o2 + tmd * (double)r.d2;
-        double rho = pm_density64(g, x, y, z);
-        cn.samples++;
-        double dlo = r.lo, dhi = r.hi;
-        if (rho >= (double)s.iso) dhi = tmd; else dlo = tmd;
-        double tt = (dlo + dhi) * 0.5;
-        r.xs0 = (float)((double)o0 + tt * (double)r.d0);
-        r.xs1 = (float)((double)o1 + tt * (double)r.d1);
-        r.xs2 = (float)((double)o2 + tt * (double)r.d2);
-        r.th = (float)tt;
-        to_normals(s, r);
-        return false;
-    }
-    if (r.phase == PH_GRADW) {
-        float sx, sy, sz, nx, ny, nz;
-        uint32_t cand = 0;
-        pm_sum_gradw(g, r.xs0, r.xs1, r.xs2, sx, sy, sz, cand);
-        pm_normalize(sx, sy, sz, nx, ny, nz);
-        L.pos[r.idx] = make_float4(r.xs0, r.xs1, r.xs2, 1.0f);
-        L.nrm[r.idx] = make_float4(nx, ny, nz, 1.0f);
-        if (L.t) L.t[r.idx] = r.th;
-        cn.hits++;
-        r.phase = PH_DONE;
-        return false;
-    }
 
     if (r.phase == PH_LAND || r.phase == PH_BSEARCH) {
         int kq = r.phase == PH_LAND ? r.khi : (r.klo + r.khi) / 2;
@@ -349,7 +353,7 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
         for (int hop = 0;; hop++) {
             float t = r.tn + (float)r.k * s.step;
             if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) {
-                write_miss(L, r.idx);
+                march_miss(L, r.idx);
                 r.phase = PH_DONE;
                 return false;
             }
@@ -362,18 +366,10 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
             r.k = kn;
             if (hop == 7) return false;
         }
-    } else if (r.phase == PH_BISECT) {
+    } else {  // PH_BISECT
         float tm = (r.lo + r.hi) * 0.5f;
         qx = o0 + tm * r.d0; qy = o1 + tm * r.d1; qz = o2 + tm * r.d2;
-    } else {  // PH_NORMAL: +x, -x, +y, -y, +z, -z
-        qx = r.xs0; qy = r.xs1; qz = r.xs2;
-        int a = r.ncd >> 1;
-        float e = s.normal_eps;
-        if (a == 0) qx = (r.ncd & 1) ? r.xs0 - e : r.xs0 + e;
-        else if (a == 1) qy = (r.ncd & 1) ? r.xs1 - e : r.xs1 + e;
-        else qz = (r.ncd & 1) ? r.xs2 - e : r.xs2 + e;
     }
-
     return true;
 }
 
@@ -382,7 +378,8 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
 __device__ __forceinline__ void ray_finish(const MarchLaunch &L, RayState &r, float rho, uint32_t cand, Counts &cn)
 {
     const pm_settings &s = L.s;
-    const int n32 = (s.final_fp64 && s.bisect_iters > 0) ? s.bisect_iters - 1 : s.bisect_iters;
+    const bool fp64 = s.final_fp64 && s.bisect_iters > 0;
+    const int n32 = fp64 ? s.bisect_iters - 1 : s.bisect_iters;
     cn.samples++;
     cn.cand += cand;
 
@@ -410,35 +407,13 @@ __device__ __forceinline__ void ray_finish(const MarchLaunch &L, RayState &r, fl
         int mid = (r.klo + r.khi) / 2;
         if (rho >= s.iso) r.khi = mid; else r.klo = mid;
         if (r.khi - r.klo == 1) { r.k = r.khi; start_bracket(L, r, n32); }
-    } else if (r.phase == PH_BISECT) {
+    } else {  // PH_BISECT
         float tm = (r.lo + r.hi) * 0.5f;
         if (rho >= s.iso) r.hi = tm; else r.lo = tm;
         r.it++;
         if (r.it >= n32) {
-            if (s.final_fp64 && s.bisect_iters > 0) r.phase = PH_BISECT64;
-            else { set_hit_point(L, r, (r.lo + r.hi) * 0.5f); to_normals(s, r); }
-        }
-    } else {  // PH_NORMAL
-        if ((r.ncd & 1) == 0) {
-            r.rplus = rho;
-        } else {
-            float gg = r.rplus - rho;
-            int a = r.ncd >> 1;
-            if (a == 0) r.gv0 = gg; else if (a == 1) r.gv1 = gg; else r.gv2 = gg;
-        }
-        r.ncd++;
-        if (r.ncd == 6) {
-            float dd = (r.gv0 * r.gv0 + r.gv1 * r.gv1) + r.gv2 * r.gv2;
-            float nx = 0.f, ny = 0.f, nz = 0.f;
-            if (dd > 0.0f) {
-                float inv = 1.0f / sqrtf(dd);
-                nx = -r.gv0 * inv; ny = -r.gv1 * inv; nz = -r.gv2 * inv;
-            }
-            L.pos[r.idx] = make_float4(r.xs0, r.xs1, r.xs2, 1.0f);
-            L.nrm[r.idx] = make_float4(nx, ny, nz, 1.0f);
-            if (L.t) L.t[r.idx] = r.th;
-            cn.hits++;
-            r.phase = PH_DONE;
+            if (fp64) to_emit(L, r, true, r.lo, r.hi);
+            else { const float t = (r.lo + r.hi) * 0.5f; to_emit(L, r, false, t, t); }
         }
     }
 }
@@ -616,8 +591,10 @@ __device__ __noinline__ void wave_propose(const MarchLaunch &L, RayState &r, Cou
 // LEARNED adds the step proposals; its LDS region per wave holds either the
 // density compaction buffer or the network's activations (never both at once).
 // CAP: compaction slots per lane; WPE: minimum waves per SIMD the register
-// allocation must allow (1 = no constraint).
-template <int UNROLL, bool LEARNED, int CAP = kLdsCap, int WPE = 1>
+// allocation must allow.  The walk waits on the cache far more than it
+// computes, so the default keeps 4 waves per SIMD (<= 128 VGPRs; all 1,024
+// workgroups resident at once) even at the price of a few spilled registers.
+template <int UNROLL, bool LEARNED, int CAP = kLdsCap, int WPE = 4>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_march_persist(
     MarchLaunch L, uint32_t *__restrict__ band_ctr)
 {
@@ -673,19 +650,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             qpos = qpos + take < 64 ? qpos + take : 64;
         }
         if (__ballot(r.phase != PH_DONE) == 0) break;
-        // Two kinds of step are batched across the wave, since the wave pays for
-        // a kind whenever any lane needs it: the FP64 bisection step (a double
-        // precision stencil walk) runs once a quarter of the busy lanes wait for
-        // it, the network once half of them wait for a proposal; whenever every
-        // busy lane waits, both run.  Waiting lanes sit out.
-        const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
-        const int n64 = __popcll(__ballot(r.phase == PH_BISECT64));
-        const int nwant = LEARNED ? __popcll(__ballot(r.phase == PH_PROPOSE)) : 0;
-        const bool stalled = n64 + nwant == nbusy;
-        if (LEARNED && nwant > 0 && (2 * nwant >= nbusy || exhausted || stalled))
-            wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
-        const bool run64 = 4 * n64 >= nbusy || stalled;
-        if (r.phase != PH_DONE && r.phase != PH_PROPOSE && (r.phase != PH_BISECT64 || run64)) {
+        // The network is batched across the wave (the wave pays for it
+        // whenever any lane needs it): it runs once half of the busy lanes wait
+        // for a proposal, or when every busy lane waits.  Waiting lanes sit out.
+        if (LEARNED) {
+            const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
+            const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
+            if (nwant > 0 && (2 * nwant >= nbusy || exhausted || nwant == nbusy))
+                wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+        }
+        if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
             if (use_lds) ray_step(L, r, DensLds<CAP, UNROLL>{ &L.g, buf }, cn);
             else ray_step(L, r, DensPlain{ &L.g, false }, cn);
         }
@@ -693,7 +667,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
     if (LEARNED) {
         unsigned long long m = wave_sum(cn.mlp);
-        if (lane == 0) atomicAdd(&L.counters[C_MLP], m);
+        if (lane == 0 && m) atomicAdd(&ctr_slot(L.counters)[C_MLP], m);
     }
 }
 
@@ -835,7 +809,7 @@ __global__ __launch_bounds__(256) void k_march_reference(MarchLaunch L)
 // ray by ray as in k_march_persist; a wave moves to its next chunk -- and
 // loads that job's grid and camera -- once all its lanes are idle.
 template <bool LEARNED>
-__global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1 : 4, 8))) void k_march_batch(BatchLaunch B)
 {
     constexpr int kRegion = LEARNED ? 2112 : kLdsCap * 64;
     __shared__ float lds[4 * kRegion];
@@ -889,15 +863,14 @@ __global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
             if (exhausted) break;
             continue;
         }
-        // batched FP64 steps and proposals, as in k_march_persist
-        const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
-        const int n64 = __popcll(__ballot(r.phase == PH_BISECT64));
-        const int nwant = LEARNED ? __popcll(__ballot(r.phase == PH_PROPOSE)) : 0;
-        const bool stalled = n64 + nwant == nbusy;
-        if (LEARNED && nwant > 0 && (2 * nwant >= nbusy || exhausted || stalled || (qpos >= 64 && tile_i >= ntiles)))
-            wave_propose(*Lp, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
-        const bool run64 = 4 * n64 >= nbusy || stalled;
-        if (r.phase != PH_DONE && r.phase != PH_PROPOSE && (r.phase != PH_BISECT64 || run64)) {
+        // batched proposals, as in k_march_persist
+        if (LEARNED) {
+            const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
+            const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
+            if (nwant > 0 && (2 * nwant >= nbusy || exhausted || nwant == nbusy || (qpos >= 64 && tile_i >= ntiles)))
+                wave_propose(*Lp, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+        }
+        if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
             if (use_lds) ray_step(*Lp, r, DensLds<kLdsCap, 4>{ &Lp->g, buf }, cn);
             else ray_step(*Lp, r, DensPlain{ &Lp->g, false }, cn);
         }
@@ -905,14 +878,141 @@ __global__ __launch_bounds__(256) void k_march_batch(BatchLaunch B)
     flush_counters(B.base.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
     if (LEARNED) {
         unsigned long long m = wave_sum(cn.mlp);
-        if (lane == 0) atomicAdd(&B.base.counters[C_MLP], m);
+        if (lane == 0 && m) atomicAdd(&ctr_slot(B.base.counters)[C_MLP], m);
     }
+}
+
+// ---- finish ----
+// Per hit record: the FP64 bisection step when the settings ask for it, the
+// normal (the six central differences in one fused walk, or the sum of gradW)
+// and the pixel's outputs.  Every lane of a wave does the same kind of work
+// and none of the march's state is held, so this runs packed and short; the
+// march kernels keep only the phases that decide where the hit is.
+__device__ __forceinline__ void finish_hit(const MarchLaunch &L, uint32_t px, uint32_t py, uint4 rec, float *buf,
+                                           Counts &cn)
+{
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
+    const uint32_t idx = py * L.W + px;
+    const float lo = __uint_as_float(rec.y), hi = __uint_as_float(rec.z);
+    float d[3];
+    pm_ray_dir(L, px, py, d);
+    const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
+    float xs[3], th;
+    if (rec.x & 2u) {
+        // the last bisection step in double (RayMarcher.cpp:316-326 in FP64)
+        const double tmd = ((double)lo + (double)hi) * 0.5;
+        const double x = (double)o0 + tmd * (double)d[0];
+        const double y = (double)o1 + tmd * (double)d[1];
+        const double z = (double)o2 + tmd * (double)d[2];
+        const double rho = pm_density64(g, x, y, z);
+        cn.samples++;
+        double dlo = lo, dhi = hi;
+        if (rho >= (double)s.iso) dhi = tmd; else dlo = tmd;
+        const double tt = (dlo + dhi) * 0.5;
+        xs[0] = (float)((double)o0 + tt * (double)d[0]);
+        xs[1] = (float)((double)o1 + tt * (double)d[1]);
+        xs[2] = (float)((double)o2 + tt * (double)d[2]);
+        th = (float)tt;
+    } else {
+        th = lo;
+        xs[0] = o0 + th * d[0]; xs[1] = o1 + th * d[1]; xs[2] = o2 + th * d[2];
+    }
+    float nx, ny, nz;
+    if (s.normal_mode == PM_NORMAL_GRADW) {
+        float sx, sy, sz;
+        uint32_t cand = 0;
+        pm_sum_gradw(g, xs[0], xs[1], xs[2], sx, sy, sz, cand);
+        pm_normalize(sx, sy, sz, nx, ny, nz);
+    } else {
+        const bool comp = s.compensated != 0;
+        const float e = s.normal_eps;
+        float gv[3];
+        const Cdiff6 o = pm_cdiff6<kLdsCap>(g, xs[0], xs[1], xs[2], e, buf, comp);
+        if (o.cand != 0xffffffffu) {
+            cn.samples += 6;
+            cn.cand += o.cand;
+            gv[0] = o.v[0] - o.v[1]; gv[1] = o.v[2] - o.v[3]; gv[2] = o.v[4] - o.v[5];
+        } else {
+            // the fused walk's cells leave the grid: +x, -x, +y, -y, +z, -z one at a time
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                float q[3] = { xs[0], xs[1], xs[2] };
+                q[k >> 1] = (k & 1) ? xs[k >> 1] - e : xs[k >> 1] + e;
+                uint32_t cand = 0;
+                const float rho = comp ? pm_density32_lds<kLdsCap, 4>(g, q[0], q[1], q[2], buf, cand)
+                                       : pm_density32(g, q[0], q[1], q[2], false, cand);
+                cn.samples++;
+                cn.cand += cand;
+                if (k & 1) gv[k >> 1] -= rho; else gv[k >> 1] = rho;
+            }
+        }
+        // RayMarcher.cpp:331-338 with central differences
+        const float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
+        nx = ny = nz = 0.0f;
+        if (dd > 0.0f) {
+            const float inv = 1.0f / sqrtf(dd);
+            nx = -gv[0] * inv; ny = -gv[1] * inv; nz = -gv[2] * inv;
+        }
+    }
+    L.pos[idx] = make_float4(xs[0], xs[1], xs[2], 1.0f);
+    L.nrm[idx] = make_float4(nx, ny, nz, 1.0f);
+    if (L.t) L.t[idx] = th;
+    cn.hits++;
+}
+
+// One 8x8 tile per wave, one pixel per lane: the lanes whose pixel holds a
+// hit record finish it.
+__device__ __forceinline__ void finish_tile(const MarchLaunch &L, uint32_t tile, float *buf, Counts &cn)
+{
+    const uint32_t lane = threadIdx.x & 63, tiles_x = (L.W + 7) / 8;
+    const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
+    if (px >= L.W || py >= L.H) return;
+    const uint4 rec = L.hit_rec[py * L.W + px];
+    if (rec.x & 1u) finish_hit(L, px, py, rec, buf, cn);
+}
+
+// Single launch: the tiles the march visited (its active-tile list, or all);
+// batch: the tiles of the chunks, one chunk per workgroup at a time.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_finish(MarchLaunch L0, const MarchLaunch *__restrict__ jobs,
+                                                const BatchChunk *__restrict__ chunks, uint32_t nchunks)
+{
+    __shared__ float lds[4 * kLdsCap * 64];
+    const uint32_t wave = threadIdx.x >> 6;
+    float *buf = lds + wave * kLdsCap * 64 + (threadIdx.x & 63);
+    Counts cn;
+    if (BATCH) {
+        for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+            const BatchChunk ch = chunks[c];
+            const MarchLaunch *Lp = jobs + __builtin_amdgcn_readfirstlane(ch.job);
+            for (uint32_t t = wave; t < ch.ntiles; t += 4) finish_tile(*Lp, ch.tile0 + t, buf, cn);
+        }
+    } else {
+        const uint32_t T = ((L0.W + 7) / 8) * ((L0.H + 7) / 8);
+        const uint32_t TA = L0.tile_buf ? L0.tile_buf[2 * (T + 1) + T] : T;
+        for (uint32_t i = blockIdx.x * 4 + wave; i < TA; i += gridDim.x * 4)
+            finish_tile(L0, L0.tile_buf ? L0.tile_buf[2 * (T + 1) + i] : i, buf, cn);
+    }
+    flush_counters(L0.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+}
+
+static hipError_t launch_finish(const MarchLaunch &L0, hipStream_t st)
+{
+    const uint32_t T = ((L0.W + 7) / 8) * ((L0.H + 7) / 8);
+    const uint32_t b = (T + 3) / 4;
+    hipLaunchKernelGGL((k_finish<false>), dim3(b < 2048 ? b : 2048), dim3(256), 0, st, L0, nullptr, nullptr, 0u);
+    return hipGetLastError();
 }
 
 hipError_t launch_march_batch(const BatchLaunch &B, uint32_t blocks, hipStream_t st)
 {
     if (B.base.s.learned) hipLaunchKernelGGL((k_march_batch<true>), dim3(blocks), dim3(256), 0, st, B);
     else hipLaunchKernelGGL((k_march_batch<false>), dim3(blocks), dim3(256), 0, st, B);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint32_t fb = B.nchunks < 8192 ? B.nchunks : 8192;
+    hipLaunchKernelGGL((k_finish<true>), dim3(fb), dim3(256), 0, st, B.base, B.jobs, B.chunks, B.nchunks);
     return hipGetLastError();
 }
 
@@ -935,7 +1035,7 @@ hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st)
 
 __global__ void k_init_counters(unsigned long long *p, uint32_t n, unsigned long long rays0)
 {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = i == (uint32_t)C_RAYS ? rays0 : 0ull;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = i == (uint32_t)C_RAYS ? rays0 : 0ull;   // slot 0 holds rays0
 }
 
 hipError_t launch_init_counters(unsigned long long *p, uint32_t n, unsigned long long rays0, hipStream_t st)
@@ -956,14 +1056,24 @@ hipError_t launch_miss_tiles(const MarchLaunch &L, uint32_t T, hipStream_t st)
     return hipGetLastError();
 }
 
+static hipError_t launch_march_kernel(const MarchLaunch &L, hipStream_t st);
+
 hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
 {
     uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
-    dim3 tb((tiles + 3) / 4), blk(256);
     if (L.s.mode == PM_MODE_REFERENCE) {
-        hipLaunchKernelGGL(k_march_reference, tb, blk, 0, st, L);
+        hipLaunchKernelGGL(k_march_reference, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
         return hipGetLastError();
     }
+    hipError_t e = launch_march_kernel(L, st);
+    if (e != hipSuccess) return e;
+    return launch_finish(L, st);
+}
+
+static hipError_t launch_march_kernel(const MarchLaunch &L, hipStream_t st)
+{
+    uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
+    dim3 tb((tiles + 3) / 4), blk(256);
     switch (L.s.learned ? 0 : L.s.variant) {
     case 1: hipLaunchKernelGGL((k_march_tile<false, 1>), tb, blk, 0, st, L); return hipGetLastError();
     case 4: hipLaunchKernelGGL((k_march_tile<true, 4>), tb, blk, 0, st, L); return hipGetLastError();
@@ -984,7 +1094,7 @@ hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
     uint32_t need = (tiles + 3) / 4;
     if (blocks > need) blocks = (need + 7) & ~7u;
     if (L.s.learned) {
-        hipLaunchKernelGGL((k_march_persist<4, true>), dim3(blocks), blk, 0, st, L, L.band_ctr);
+        hipLaunchKernelGGL((k_march_persist<4, true, kLdsCap, 1>), dim3(blocks), blk, 0, st, L, L.band_ctr);
         return hipGetLastError();
     }
     switch (L.s.variant) {

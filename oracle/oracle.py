@@ -103,6 +103,9 @@ def lib():
         L.or_neighbors_fine.restype = C.c_uint32
         L.or_neighbors_fine.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), fp(np.float32, flags="C"),
                                         C.c_uint32, C.POINTER(C.c_uint32)]
+        L.or_cdiff_union.restype = C.c_int
+        L.or_cdiff_union.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), C.c_float,
+                                     C.POINTER(C.c_uint32), C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
         L.or_neighbors_brute.restype = C.c_uint32
         L.or_neighbors_brute.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float,
                                          fp(np.float32, flags="C"), fp(np.uint32, flags="C"), C.c_uint32]
@@ -244,6 +247,21 @@ class Grid:
         cand = C.c_uint32(0)
         n = lib().or_neighbors_fine(C.byref(self.g), q, out, 1 << 15, C.byref(cand))
         return out[:3 * n].reshape(-1, 3).copy(), int(cand.value)
+
+    def cdiff_union(self, q, eps):
+        """The fused central-difference walk around q: (candidate count,
+        positions of the candidates within (h + eps)(1 + 2^-10) of q), or None
+        when its cells leave the grid."""
+        q = np.ascontiguousarray(q, np.float32)
+        cap = 1 << 15
+        out = np.zeros(3 * cap, np.float32)
+        cand, n = C.c_uint32(0), C.c_uint32(0)
+        ok = lib().or_cdiff_union(C.byref(self.g), q, float(np.float32(eps)), C.byref(cand), out.ctypes.data, cap,
+                                  C.byref(n))
+        if not ok:
+            return None
+        assert n.value <= cap
+        return int(cand.value), out[:3 * n.value].reshape(-1, 3).copy()
 
     def density(self, q, compensated=True, with_cand=False):
         q = np.ascontiguousarray(np.asarray(q, np.float32).reshape(-1, 3))

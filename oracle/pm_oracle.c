@@ -421,6 +421,70 @@ int or_fine_row(const or_grid *g, const or_fine *fs, int dy, int dz, uint32_t *b
     return 1;
 }
 
+/* Fused central differences (the GPU takes the six samples q +- eps e_a in
+ * one walk): the march-grid cells near the box [q - eps, q + eps], per axis
+ * from the fine cell of the box's low end to that of its high end, +-2 cells;
+ * rows and row ends kept by their distance to the box (fine units) below
+ * OR_FINE_LIM.  The box is never farther from a cell than any of the six
+ * samples, so these cells include every cell their own stencils keep.  The
+ * walk adds the candidates of its kept rows once; returns 0 when the cells
+ * leave the grid (then the samples are taken one at a time).  With out != NULL
+ * the positions of the candidates within (h + eps)(1 + 2^-10) of q are listed
+ * (3 floats each, up to cap; the count is returned in *nout).               */
+static inline float box_off(float c, float ulo, float uhi, float kl, float kh)
+{
+    return c > kh ? c - uhi : (c < kl ? ulo - (c + 1.0f) : 0.0f);
+}
+
+int or_cdiff_union(const or_grid *g, const float q[3], float eps, uint32_t *ncand, float *out, uint32_t cap,
+                   uint32_t *nout)
+{
+    float ulo[3], uhi[3], kl[3], kh[3];
+    int32_t ikl[3], ikh[3];
+    for (int a = 0; a < 3; a++) {
+        float lo = q[a] - eps, hi = q[a] + eps;
+        ulo[a] = g->inv_hf * lo;
+        uhi[a] = g->inv_hf * hi;
+        ikl[a] = lo >= 0.0f ? (int32_t)ulo[a] : (int32_t)ulo[a] - 1;
+        ikh[a] = hi >= 0.0f ? (int32_t)uhi[a] : (int32_t)uhi[a] - 1;
+        kl[a] = (float)ikl[a];
+        kh[a] = (float)ikh[a];
+        if (ikl[a] - 2 - g->forigin[a] < 0 || ikh[a] + 2 - g->forigin[a] > g->fdims[a] - 1) return 0;
+    }
+    float t;
+    t = box_off(kl[0] - 2.0f, ulo[0], uhi[0], kl[0], kh[0]); const float xm2 = t * t;
+    t = box_off(kl[0] - 1.0f, ulo[0], uhi[0], kl[0], kh[0]); const float xm1 = t * t;
+    t = box_off(kh[0] + 1.0f, ulo[0], uhi[0], kl[0], kh[0]); const float xp1 = t * t;
+    t = box_off(kh[0] + 2.0f, ulo[0], uhi[0], kl[0], kh[0]); const float xp2 = t * t;
+    const float he = (g->h + eps) * 1.0009765625f, R2 = he * he;
+    uint32_t cand = 0, n = 0;
+    for (int32_t cz = ikl[2] - 2; cz <= ikh[2] + 2; cz++) {
+        for (int32_t cy = ikl[1] - 2; cy <= ikh[1] + 2; cy++) {
+            float ty = box_off((float)cy, ulo[1], uhi[1], kl[1], kh[1]);
+            float tz = box_off((float)cz, ulo[2], uhi[2], kl[2], kh[2]);
+            float r2 = ty * ty + tz * tz;
+            if (!(r2 < OR_FINE_LIM)) continue;
+            int32_t lo = (xm2 + r2 < OR_FINE_LIM) ? ikl[0] - 2 : ((xm1 + r2 < OR_FINE_LIM) ? ikl[0] - 1 : ikl[0]);
+            int32_t hi = (xp2 + r2 < OR_FINE_LIM) ? ikh[0] + 2 : ((xp1 + r2 < OR_FINE_LIM) ? ikh[0] + 1 : ikh[0]);
+            int32_t y = cy - g->forigin[1], z = cz - g->forigin[2];
+            uint32_t b = g->fcell_start[fine_lin(g, lo - g->forigin[0], y, z)];
+            uint32_t e = g->fcell_start[fine_lin(g, hi - g->forigin[0], y, z) + 1];
+            cand += e - b;
+            if (!out) continue;
+            for (uint32_t i = b; i < e; i++) {
+                const float *p = g->fsorted + 4 * (uint64_t)i;
+                float rx = p[0] - q[0], ry = p[1] - q[1], rz = p[2] - q[2];
+                if (!((rx * rx + ry * ry) + rz * rz < R2)) continue;
+                if (n < cap) { out[3 * n] = p[0]; out[3 * n + 1] = p[1]; out[3 * n + 2] = p[2]; }
+                n++;
+            }
+        }
+    }
+    *ncand = cand;
+    if (nout) *nout = n;
+    return 1;
+}
+
 uint32_t or_neighbors_fine(const or_grid *g, const float q[3], float *out, uint32_t cap, uint32_t *ncand)
 {
     or_fine fs;
@@ -963,14 +1027,26 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
 
     float n[3] = { 0.0f, 0.0f, 0.0f };
     if (s->normal_mode == 0) {
+        /* The six samples' candidates are counted as the GPU's fused walk
+         * reads them (or_cdiff_union); their densities are the single-sample
+         * ones.  The hashed search keeps one count per sample. */
+        uint32_t ucand;
+        const int fused = !dc->hs && or_cdiff_union(dc->g, xs, s->normal_eps, &ucand, NULL, 0, NULL);
+        or_counters one;
+        memset(&one, 0, sizeof one);
+        or_counters *c6 = fused ? &one : cnt;
         float gv[3];
         for (int a = 0; a < 3; a++) {
             float xp[3] = { xs[0], xs[1], xs[2] }, xm[3] = { xs[0], xs[1], xs[2] };
             xp[a] = xs[a] + s->normal_eps;
             xm[a] = xs[a] - s->normal_eps;
-            float rp = dens_f(dc, xp, cnt);
-            float rm = dens_f(dc, xm, cnt);
+            float rp = dens_f(dc, xp, c6);
+            float rm = dens_f(dc, xm, c6);
             gv[a] = rp - rm;
+        }
+        if (fused) {
+            cnt->samples += 6;
+            cnt->cand += ucand;
         }
         float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
         if (dd > 0.0f) {

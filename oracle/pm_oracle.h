@@ -126,6 +126,12 @@ int or_fine_row(const or_grid *g, const or_fine *fs, int dy, int dz, uint32_t *b
  * are not stored for the march grid, so positions are returned: 3 per hit). */
 uint32_t or_neighbors_fine(const or_grid *g, const float q[3], float *out, uint32_t cap, uint32_t *ncand);
 
+/* Candidates of the fused central-difference walk around q (box [q - eps,
+ * q + eps]); 0 when its cells leave the grid.  out (optional): positions of
+ * the candidates within (h + eps)(1 + 2^-10) of q, 3 floats each. */
+int or_cdiff_union(const or_grid *g, const float q[3], float eps, uint32_t *ncand, float *out, uint32_t cap,
+                   uint32_t *nout);
+
 /* ---- density ---- */
 float  or_density_f(const or_grid *g, const float q[3], int compensated, uint32_t *ncand);
 double or_density_d(const or_grid *g, const double q[3]);
