@@ -44,7 +44,7 @@ struct Frame {
     pm_frame_info info{};
     GridDev g{};
     DevBuf sorted, cell_start, cell_of, counts, cursor, scan_tmp, xyz, bounds_partial, bounds, scattered;
-    DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound, brick;
+    DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound;
     DevBuf fsorted, fcell_start;    // march grid (cells of h/2)
     bool bound_ok = false;      // bound holds the sub-box bits for bound_iso
     bool dg_ok = false;         // the DensityGrid flags are built
@@ -56,7 +56,7 @@ struct Frame {
         coarse.release();
         sorted.release(); cell_start.release(); cell_of.release(); counts.release(); cursor.release();
         scan_tmp.release(); xyz.release(); bounds_partial.release(); bounds.release(); scattered.release();
-        empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release(); bound.release(); brick.release();
+        empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release(); bound.release();
         fsorted.release(); fcell_start.release();
     }
 };
@@ -432,10 +432,7 @@ static int ensure_bound(pm_ctx *c, Frame *f, float iso)
     if (f->bound_ok && f->bound_iso == iso) return PM_OK;
     HIPCHK(c, f->bound.ensure(sizeof(uint64_t) * f->info.ncells));
     f->g.bound = (const uint64_t *)f->bound.p;
-    f->g.bnx = (f->g.nx + 3) / 4; f->g.bny = (f->g.ny + 3) / 4; f->g.bnz = (f->g.nz + 3) / 4;
-    HIPCHK(c, f->brick.ensure((uint64_t)f->g.bnx * f->g.bny * f->g.bnz));
-    f->g.brick = (const uint8_t *)f->brick.p;
-    HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, (uint8_t *)f->brick.p, c->stream));
+    HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, c->stream));
     f->bound_ok = true;
     f->bound_iso = iso;
     c->frame_epoch++;

@@ -32,8 +32,6 @@ struct GridDev {
     float bmin[3], bmax[3];   // padded AABB
     const uint8_t *empty27;   // per cell: 1 if the cell and its 26 neighbours are empty
     const uint64_t *bound;    // per cell: bit l set when sub-box l (4x4x4) stays below iso
-    const uint8_t *brick;     // per 4x4x4 cells: 1 when every sub-box of every cell is set
-    int32_t bnx, bny, bnz;    // brick grid dims
     // reference DensityGrid over the padded AABB (Dataset.cpp:94-164)
     const uint8_t *dg_flag;   // flag per node for the selected skip rule
     int32_t dg_nx, dg_ny, dg_nz;
@@ -275,10 +273,22 @@ __device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, fl
                         for (int u = 0; u < UNROLL; u++) pm_test_append(g, v[u], qx, qy, qz, buf, n);
                     }
                     for (; p < pe; p++) pm_test_append(g, *p, qx, qy, qz, buf, n);
-                } else {                                // a row longer than the buffer
-                    for (uint32_t i = b; i < e; i++) {
-                        pm_test_append(g, g.fsorted[i], qx, qy, qz, buf, n);
-                        if (n == CAP) { pm_drain<CAP>(g, buf, n, s, c); n = 0; }
+                } else {
+                    // a row longer than the buffer (dense clusters hold hundreds of
+                    // particles per row): in chunks that fit, loaded UNROLL at a time
+                    const float4 *p = g.fsorted + b;
+                    const float4 *const pe = g.fsorted + e;
+                    while (p < pe) {
+                        const float4 *const ce = p + (CAP - n) < pe ? p + (CAP - n) : pe;
+                        for (; p + UNROLL <= ce; p += UNROLL) {
+                            float4 v[UNROLL];
+#pragma unroll
+                            for (int u = 0; u < UNROLL; u++) v[u] = p[u];
+#pragma unroll
+                            for (int u = 0; u < UNROLL; u++) pm_test_append(g, v[u], qx, qy, qz, buf, n);
+                        }
+                        for (; p < ce; p++) pm_test_append(g, *p, qx, qy, qz, buf, n);
+                        if (p < pe) { pm_drain<CAP>(g, buf, n, s, c); n = 0; }
                     }
                 }
             }
@@ -465,10 +475,15 @@ __device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy
                 }
             }
         } else {                                          // a row longer than the buffer
-            for (int i = 0; i < m; i++) {
-                const float4 p = p0[i];
-                const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-                if ((rx * rx + ry * ry) + rz * rz < R2) pm_acc6(g, p.x, p.y, p.z, x, xp, xm, comp, a);
+            for (int i = 0; i < m; i += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
+                    if (i + u < m && (rx * rx + ry * ry) + rz * rz < R2) pm_acc6(g, v[u].x, v[u].y, v[u].z, x, xp, xm, comp, a);
+                }
             }
         }
     }

@@ -314,34 +314,10 @@ __global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64
     if (lane == 0) out[c] = m;
 }
 
-// brick[b] = 1 when brick b (4x4x4 cells) lies inside the grid and every
-// sub-box of its 64 cells is set. A brick reaching past the grid stays 0: the
-// march evaluates every sample outside the grid, so skipping one would change
-// which samples are evaluated.
-__global__ __launch_bounds__(256) void k_bound_bricks(GridDev g, const uint64_t *__restrict__ bound,
-                                                      uint8_t *__restrict__ brick)
-{
-    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= (uint32_t)g.bnx * g.bny * g.bnz) return;
-    const int32_t bx = (int32_t)(b % g.bnx), by = (int32_t)((b / g.bnx) % g.bny), bz = (int32_t)(b / (g.bnx * g.bny));
-    bool all = 4 * bx + 4 <= g.nx && 4 * by + 4 <= g.ny && 4 * bz + 4 <= g.nz;
-    for (int32_t z = 4 * bz; z < 4 * bz + 4 && all; z++)
-        for (int32_t y = 4 * by; y < 4 * by + 4 && all; y++)
-            for (int32_t x = 4 * bx; x < 4 * bx + 4; x++) {
-                if (bound[(uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z)] != ~0ull) {
-                    all = false;
-                    break;
-                }
-            }
-    brick[b] = all ? 1 : 0;
-}
-
-hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, uint8_t *brick, hipStream_t st)
+hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, hipStream_t st)
 {
     uint32_t n = (uint32_t)g.nx * g.ny * g.nz;
     hipLaunchKernelGGL(k_bound_grid, dim3((n + 3) / 4), dim3(256), 0, st, g, iso, out);
-    uint32_t nb = (uint32_t)g.bnx * g.bny * g.bnz;
-    hipLaunchKernelGGL(k_bound_bricks, dim3((nb + 255) / 256), dim3(256), 0, st, g, (const uint64_t *)out, brick);
     return hipGetLastError();
 }
 

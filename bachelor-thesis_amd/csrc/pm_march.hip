@@ -276,26 +276,12 @@ __device__ __forceinline__ int pm_skip_bound(const GridDev &g, float o0, float o
     if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) return r.k;
     const uint64_t m = g.bound[(uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz)];
     if (m == 0ull) return r.k;
-    // the box to leave: the 4x4x4-cell brick or the cell when every sub-box
-    // of it is set, else the sample's own sub-box (key units: lo, lo + span)
-    float l0, l1, l2, span;
-    if (m == ~0ull) {
-        const int32_t bx = cx >> 2, by = cy >> 2, bz = cz >> 2;
-        if (g.brick[(uint32_t)bx + (uint32_t)g.bnx * ((uint32_t)by + (uint32_t)g.bny * (uint32_t)bz)]) {
-            l0 = (float)(4 * bx + g.ox); l1 = (float)(4 * by + g.oy); l2 = (float)(4 * bz + g.oz);
-            span = 4.0f;
-        } else {
-            l0 = (float)k0; l1 = (float)k1; l2 = (float)k2;
-            span = 1.0f;
-        }
-    } else {
-        const int s0 = min(3, (int)((u0 - (float)k0) * 4.0f));
-        const int s1 = min(3, (int)((u1 - (float)k1) * 4.0f));
-        const int s2 = min(3, (int)((u2 - (float)k2) * 4.0f));
-        if (!((m >> (s0 + 4 * s1 + 16 * s2)) & 1ull)) return r.k;
-        l0 = (float)k0 + 0.25f * (float)s0; l1 = (float)k1 + 0.25f * (float)s1; l2 = (float)k2 + 0.25f * (float)s2;
-        span = 0.25f;
-    }
+    const int s0 = min(3, (int)((u0 - (float)k0) * 4.0f));
+    const int s1 = min(3, (int)((u1 - (float)k1) * 4.0f));
+    const int s2 = min(3, (int)((u2 - (float)k2) * 4.0f));
+    if (!((m >> (s0 + 4 * s1 + 16 * s2)) & 1ull)) return r.k;
+    const float l0 = (float)k0 + 0.25f * (float)s0, l1 = (float)k1 + 0.25f * (float)s1, l2 = (float)k2 + 0.25f * (float)s2;
+    const float span = 0.25f;
     float texit = INFINITY;
     texit = pm_box_exit(l0 * g.h, (l0 + span) * g.h, o0, r.d0, r.id0, texit);
     texit = pm_box_exit(l1 * g.h, (l1 + span) * g.h, o1, r.d1, r.id1, texit);
@@ -306,31 +292,32 @@ __device__ __forceinline__ int pm_skip_bound(const GridDev &g, float o0, float o
     return kf >= (float)r.k ? (int)kf + 1 : r.k + 1;
 }
 
-
 // The ray's hit is known up to what k_finish does: its record (at the
 // pixel) holds t itself (fp64 = false), or the bracket [lo, hi] of the FP64
 // bisection step.  Record: (1 | fp64 << 1, lo, hi, 0); a miss leaves zeros.
-__device__ __forceinline__ void to_emit(const MarchLaunch &L, RayState &r, bool fp64, float lo, float hi)
+// lead: this lane writes (the others of a lane group share the ray).
+__device__ __forceinline__ void to_emit(const MarchLaunch &L, RayState &r, bool fp64, float lo, float hi,
+                                       bool lead = true)
 {
-    L.hit_rec[r.idx] = make_uint4(fp64 ? 3u : 1u, __float_as_uint(lo), __float_as_uint(hi), 0u);
+    if (lead) L.hit_rec[r.idx] = make_uint4(fp64 ? 3u : 1u, __float_as_uint(lo), __float_as_uint(hi), 0u);
     r.phase = PH_DONE;
 }
 
 // The first sample k at or above iso is known: bracket [t_{k-1}, t_k] and
 // start the bisection (or take t_0 when the ray starts inside the surface).
-__device__ __forceinline__ void start_bracket(const MarchLaunch &L, RayState &r, int n32)
+__device__ __forceinline__ void start_bracket(const MarchLaunch &L, RayState &r, int n32, bool lead = true)
 {
     const pm_settings &s = L.s;
     if (r.k == 0) {
-        to_emit(L, r, false, r.tn, r.tn);
+        to_emit(L, r, false, r.tn, r.tn, lead);
         return;
     }
     r.lo = r.tn + (float)(r.k - 1) * s.step;
     r.hi = r.tn + (float)r.k * s.step;
     r.it = 0;
     if (n32 > 0) r.phase = PH_BISECT;
-    else if (s.final_fp64 && s.bisect_iters > 0) to_emit(L, r, true, r.lo, r.hi);
-    else { const float t = (r.lo + r.hi) * 0.5f; to_emit(L, r, false, t, t); }
+    else if (s.final_fp64 && s.bisect_iters > 0) to_emit(L, r, true, r.lo, r.hi, lead);
+    else { const float t = (r.lo + r.hi) * 0.5f; to_emit(L, r, false, t, t, lead); }
 }
 
 // First half of one step: the sample point of the lane's next FP32 density
@@ -428,6 +415,157 @@ __device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens
     uint32_t cand = 0;
     float rho = dens(qx, qy, qz, cand);
     ray_finish(L, r, rho, cand, cn);
+}
+
+// ---- lane groups: the end of a persistent launch ----
+// Once no tile is left, a wave's remaining rays are spread over its idle lanes:
+// G lanes (a power of two) per ray, every lane of a group holding the same
+// RayState.  A march step then evaluates the G lattice samples k .. k+G-1 at
+// once and takes the first at or above iso; a bisection step evaluates the
+// 2^L - 1 midpoints of the next L levels (L = floor(log2(G + 1))) and descends
+// through them.  Each sample is computed as the one-lane march computes it,
+// and a sample the skip would pass has its density below iso, so the first
+// crossing, the bracket and the image are those of the one-lane march.
+// Counters count what the one-lane march evaluates: the samples the skip
+// keeps, up to the crossing, and the bisection's path.
+
+// The ray's state to LDS and back (region: 64 floats per field).
+__device__ __forceinline__ void state_put(float *region, uint32_t slot, const RayState &r)
+{
+    float *p = region + slot;
+    p[0 * 64] = r.d0; p[1 * 64] = r.d1; p[2 * 64] = r.d2; p[3 * 64] = r.tn; p[4 * 64] = r.tf;
+    p[5 * 64] = r.lo; p[6 * 64] = r.hi; p[7 * 64] = __int_as_float(r.k); p[8 * 64] = __int_as_float(r.it);
+    p[9 * 64] = __int_as_float(r.phase); p[10 * 64] = __uint_as_float(r.idx); p[11 * 64] = r.id0;
+    p[12 * 64] = r.id1; p[13 * 64] = r.id2;
+}
+
+__device__ __forceinline__ void state_get(const float *region, uint32_t slot, RayState &r)
+{
+    const float *p = region + slot;
+    r.d0 = p[0 * 64]; r.d1 = p[1 * 64]; r.d2 = p[2 * 64]; r.tn = p[3 * 64]; r.tf = p[4 * 64];
+    r.lo = p[5 * 64]; r.hi = p[6 * 64]; r.k = __float_as_int(p[7 * 64]); r.it = __float_as_int(p[8 * 64]);
+    r.phase = __float_as_int(p[9 * 64]); r.idx = __float_as_uint(p[10 * 64]); r.id0 = p[11 * 64];
+    r.id1 = p[12 * 64]; r.id2 = p[13 * 64];
+    r.lrn = 0;
+}
+
+// Widens the groups when the busy rays fit in fewer lanes: G becomes the
+// largest power of two with (busy rays) x G <= 64.  Called by all 64 lanes.
+__device__ __forceinline__ void regroup(RayState &r, int &G, float *region)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long lead = __ballot(r.phase != PH_DONE && (lane & (uint32_t)(G - 1)) == 0);
+    const int nb = __popcll(lead);
+    int Gn = 1;
+    while (Gn < 64 && nb * Gn * 2 <= 64) Gn *= 2;
+    if (Gn <= G || nb == 0) return;
+    if ((lead >> lane) & 1ull) state_put(region, (uint32_t)__popcll(lead & ((1ull << lane) - 1ull)), r);
+    wave_lds_sync();
+    if ((int)(lane / (uint32_t)Gn) < nb) state_get(region, lane / (uint32_t)Gn, r);
+    else r.phase = PH_DONE;
+    wave_lds_sync();
+    G = Gn;
+}
+
+// Lane j of its group: its sample of this step; false when it takes none.
+// nonskip: the sample is one the one-lane march would evaluate.
+__device__ __forceinline__ bool group_prepare(const MarchLaunch &L, RayState &r, int G, int j, float &qx, float &qy,
+                                              float &qz, bool &nonskip)
+{
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
+    const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
+    nonskip = false;
+    if (r.phase == PH_MARCH) {
+        // the group's first sample, as ray_prepare finds it; the wave has few
+        // rays left, so a group may take 8 G skips in one step
+        const int hop_cap = 8 * G - 1;
+        for (int hop = 0;; hop++) {
+            float t = r.tn + (float)r.k * s.step;
+            if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) {
+                if (j == 0) march_miss(L, r.idx);
+                r.phase = PH_DONE;
+                return false;
+            }
+            qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
+            int kn = r.k;
+            if (s.skip_mode == PM_SKIP_BOUND) kn = pm_skip_bound(g, o0, o1, o2, r, qx, qy, qz, L.inv_step);
+            if (kn == r.k) break;
+            r.k = kn;
+            if (hop == hop_cap) return false;
+        }
+        if (j == 0) { nonskip = true; return true; }
+        const int kj = r.k + j;
+        const float t = r.tn + (float)kj * s.step;
+        if (t > r.tf || (s.max_steps > 0 && kj >= s.max_steps)) return false;
+        qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
+        nonskip = s.skip_mode != PM_SKIP_BOUND || pm_skip_bound(g, o0, o1, o2, r, qx, qy, qz, L.inv_step) == r.k;
+        return true;
+    }
+    if (r.phase == PH_BISECT) {
+        const bool fp64 = s.final_fp64 && s.bisect_iters > 0;
+        const int n32 = fp64 ? s.bisect_iters - 1 : s.bisect_iters;
+        int lv = 31 - __clz(G + 1);                 // floor(log2(G + 1))
+        if (lv > n32 - r.it) lv = n32 - r.it;
+        const int node = j + 1;                     // heap order: 1 = the next midpoint
+        if (node >= (1 << lv)) return false;
+        const int depth = 31 - __clz(node);
+        float lo = r.lo, hi = r.hi;
+        for (int b = depth - 1; b >= 0; b--) {      // left child: the crossing lies below the midpoint
+            const float tm = (lo + hi) * 0.5f;
+            if ((node >> b) & 1) lo = tm; else hi = tm;
+        }
+        const float tm = (lo + hi) * 0.5f;
+        qx = o0 + tm * r.d0; qy = o1 + tm * r.d1; qz = o2 + tm * r.d2;
+        nonskip = true;
+        return true;
+    }
+    return false;
+}
+
+// The group's transition from its lanes' densities.  Called by all 64 lanes.
+__device__ __forceinline__ void group_finish(const MarchLaunch &L, RayState &r, int G, int j, bool want, bool nonskip,
+                                             float rho, uint32_t cand, Counts &cn)
+{
+    const pm_settings &s = L.s;
+    const bool fp64 = s.final_fp64 && s.bisect_iters > 0;
+    const int n32 = fp64 ? s.bisect_iters - 1 : s.bisect_iters;
+    const uint32_t lane = threadIdx.x & 63, gbase = lane & ~(uint32_t)(G - 1);
+    const unsigned long long gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const unsigned long long cross = (__ballot(want && rho >= s.iso) >> gbase) & gmask;
+    const unsigned long long wants = (__ballot(want) >> gbase) & gmask;
+    // bisection: descend through the evaluated midpoints (uniform trip count)
+    const int lvmax = 31 - __clz(G + 1);
+    const bool bis = r.phase == PH_BISECT && (wants & 1ull);
+    int node = 1;
+    for (int l = 0; l < lvmax; l++) {
+        const float rn = __shfl(rho, (int)gbase + node - 1, 64);
+        const uint32_t cn_n = __shfl(cand, (int)gbase + node - 1, 64);
+        if (bis && r.it < n32 && ((wants >> (node - 1)) & 1ull)) {
+            const float tm = (r.lo + r.hi) * 0.5f;
+            if (rn >= s.iso) { r.hi = tm; node = 2 * node; } else { r.lo = tm; node = 2 * node + 1; }
+            r.it++;
+            if (j == 0) { cn.samples++; cn.cand += cn_n; }
+        }
+    }
+    if (bis && r.it >= n32) {
+        if (fp64) to_emit(L, r, true, r.lo, r.hi, j == 0);
+        else { const float t = (r.lo + r.hi) * 0.5f; to_emit(L, r, false, t, t, j == 0); }
+        return;
+    }
+    if (r.phase != PH_MARCH || !(wants & 1ull)) return;
+    const int jc = cross ? __ffsll((long long)cross) - 1 : G;
+    if (want && nonskip && j <= jc) {
+        cn.samples++;
+        cn.march++;
+        cn.cand += cand;
+    }
+    if (cross) {
+        r.k += jc;
+        start_bracket(L, r, n32, j == 0);
+    } else {
+        r.k += G;
+    }
 }
 
 struct DensPlain {
@@ -615,6 +753,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     int qpos = 64;
     bool exhausted = false;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    // lane groups at the end of the launch (the one-lane march otherwise)
+    int G = 1;
+    const bool group_ok = L.s.skip_mode == PM_SKIP_BOUND || L.s.skip_mode == PM_SKIP_NONE;
 
     while (true) {
         // refill idle lanes from the wave's tile queue
@@ -659,9 +800,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             if (nwant > 0 && (2 * nwant >= nbusy || exhausted || nwant == nbusy))
                 wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
         }
-        if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
-            if (use_lds) ray_step(L, r, DensLds<CAP, UNROLL>{ &L.g, buf }, cn);
-            else ray_step(L, r, DensPlain{ &L.g, false }, cn);
+        if (!LEARNED && exhausted && group_ok) regroup(r, G, region);
+        if (G == 1) {
+            if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
+                if (use_lds) ray_step(L, r, DensLds<CAP, UNROLL>{ &L.g, buf }, cn);
+                else ray_step(L, r, DensPlain{ &L.g, false }, cn);
+            }
+        } else {
+            const int j = (int)(lane & (uint32_t)(G - 1));
+            float qx = 0.f, qy = 0.f, qz = 0.f, rho = 0.f;
+            bool nonskip = false, want = false;
+            uint32_t cand = 0;
+            if (r.phase != PH_DONE) want = group_prepare(L, r, G, j, qx, qy, qz, nonskip);
+            if (want) rho = use_lds ? pm_density32_lds<CAP, UNROLL>(L.g, qx, qy, qz, buf, cand)
+                                    : pm_density32(L.g, qx, qy, qz, false, cand);
+            group_finish(L, r, G, j, want, nonskip, rho, cand, cn);
         }
     }
     flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
@@ -973,9 +1126,11 @@ __device__ __forceinline__ void finish_tile(const MarchLaunch &L, uint32_t tile,
 }
 
 // Single launch: the tiles the march visited (its active-tile list, or all);
-// batch: the tiles of the chunks, one chunk per workgroup at a time.
+// batch: the tiles of the chunks, one chunk per workgroup at a time.  Four
+// waves per SIMD (registers capped at 128) hide more of the walks' latency
+// than three would at 150 registers.
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_finish(MarchLaunch L0, const MarchLaunch *__restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_finish(MarchLaunch L0, const MarchLaunch *__restrict__ jobs,
                                                 const BatchChunk *__restrict__ chunks, uint32_t nchunks)
 {
     __shared__ float lds[4 * kLdsCap * 64];
