@@ -588,6 +588,7 @@ struct DensLds {
 };
 
 static constexpr int kLdsCap = 32;
+static constexpr uint32_t kOwnTiles = 32;   // tiles a persistent wave finishes itself
 
 // Band b of the T tiles is [T b / 8, T (b+1) / 8).
 __device__ __forceinline__ uint32_t band_lo(uint32_t T, uint32_t b) { return (uint32_t)(((uint64_t)T * b) >> 3); }
@@ -718,6 +719,96 @@ __device__ __noinline__ void wave_propose(const MarchLaunch &L, RayState &r, Cou
     }
 }
 
+// ---- finish ----
+// Per hit record: the FP64 bisection step when the settings ask for it, the
+// normal (the six central differences in one fused walk, or the sum of gradW)
+// and the pixel's outputs.  Every lane of a wave does the same kind of work
+// and none of the march's state is held, so this runs packed and short; the
+// march kernels keep only the phases that decide where the hit is.
+__device__ __forceinline__ void finish_hit(const MarchLaunch &L, uint32_t px, uint32_t py, uint4 rec, float *buf,
+                                           Counts &cn)
+{
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
+    const uint32_t idx = py * L.W + px;
+    const float lo = __uint_as_float(rec.y), hi = __uint_as_float(rec.z);
+    float d[3];
+    pm_ray_dir(L, px, py, d);
+    const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
+    float xs[3], th;
+    if (rec.x & 2u) {
+        // the last bisection step in double (RayMarcher.cpp:316-326 in FP64)
+        const double tmd = ((double)lo + (double)hi) * 0.5;
+        const double x = (double)o0 + tmd * (double)d[0];
+        const double y = (double)o1 + tmd * (double)d[1];
+        const double z = (double)o2 + tmd * (double)d[2];
+        const double rho = pm_density64(g, x, y, z);
+        cn.samples++;
+        double dlo = lo, dhi = hi;
+        if (rho >= (double)s.iso) dhi = tmd; else dlo = tmd;
+        const double tt = (dlo + dhi) * 0.5;
+        xs[0] = (float)((double)o0 + tt * (double)d[0]);
+        xs[1] = (float)((double)o1 + tt * (double)d[1]);
+        xs[2] = (float)((double)o2 + tt * (double)d[2]);
+        th = (float)tt;
+    } else {
+        th = lo;
+        xs[0] = o0 + th * d[0]; xs[1] = o1 + th * d[1]; xs[2] = o2 + th * d[2];
+    }
+    float nx, ny, nz;
+    if (s.normal_mode == PM_NORMAL_GRADW) {
+        float sx, sy, sz;
+        uint32_t cand = 0;
+        pm_sum_gradw(g, xs[0], xs[1], xs[2], sx, sy, sz, cand);
+        pm_normalize(sx, sy, sz, nx, ny, nz);
+    } else {
+        const bool comp = s.compensated != 0;
+        const float e = s.normal_eps;
+        float gv[3];
+        const Cdiff6 o = pm_cdiff6<kLdsCap>(g, xs[0], xs[1], xs[2], e, buf, comp);
+        if (o.cand != 0xffffffffu) {
+            cn.samples += 6;
+            cn.cand += o.cand;
+            gv[0] = o.v[0] - o.v[1]; gv[1] = o.v[2] - o.v[3]; gv[2] = o.v[4] - o.v[5];
+        } else {
+            // the fused walk's cells leave the grid: +x, -x, +y, -y, +z, -z one at a time
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                float q[3] = { xs[0], xs[1], xs[2] };
+                q[k >> 1] = (k & 1) ? xs[k >> 1] - e : xs[k >> 1] + e;
+                uint32_t cand = 0;
+                const float rho = comp ? pm_density32_lds<kLdsCap, 4>(g, q[0], q[1], q[2], buf, cand)
+                                       : pm_density32(g, q[0], q[1], q[2], false, cand);
+                cn.samples++;
+                cn.cand += cand;
+                if (k & 1) gv[k >> 1] -= rho; else gv[k >> 1] = rho;
+            }
+        }
+        // RayMarcher.cpp:331-338 with central differences
+        const float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
+        nx = ny = nz = 0.0f;
+        if (dd > 0.0f) {
+            const float inv = 1.0f / sqrtf(dd);
+            nx = -gv[0] * inv; ny = -gv[1] * inv; nz = -gv[2] * inv;
+        }
+    }
+    L.pos[idx] = make_float4(xs[0], xs[1], xs[2], 1.0f);
+    L.nrm[idx] = make_float4(nx, ny, nz, 1.0f);
+    if (L.t) L.t[idx] = th;
+    cn.hits++;
+}
+
+// One 8x8 tile per wave, one pixel per lane: the lanes whose pixel holds a
+// hit record finish it.
+__device__ __forceinline__ void finish_tile(const MarchLaunch &L, uint32_t tile, float *buf, Counts &cn)
+{
+    const uint32_t lane = threadIdx.x & 63, tiles_x = (L.W + 7) / 8;
+    const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
+    if (px >= L.W || py >= L.H) return;
+    const uint4 rec = L.hit_rec[py * L.W + px];
+    if (rec.x & 1u) finish_hit(L, px, py, rec, buf, cn);
+}
+
 // Schedule 0: persistent waves.  The image is cut into 8x8 tiles and the tiles
 // into 8 contiguous bands, one per XCD (workgroups are dispatched round-robin
 // over the XCDs, so blockIdx % 8 names the XCD); a wave takes tiles from its
@@ -739,10 +830,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     // floats per wave: the compaction buffer, or X[64][32] + H[32][64] bf16 + Y[64]
     constexpr int kRegion = LEARNED ? 2112 : CAP * 64;
     __shared__ float lds[4 * kRegion];
+    __shared__ uint32_t own[4][kOwnTiles];       // tiles this wave marched (finished by it at the end)
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float *region = lds + wave * kRegion;
     float *buf = region + lane;
     const uint32_t tiles_x = (L.W + 7) / 8, tiles_y = (L.H + 7) / 8, T = tiles_x * tiles_y;
+    uint32_t nown = 0;
     const uint32_t xcd = blockIdx.x & 7u;
     const bool use_lds = L.s.compensated != 0;
 
@@ -778,6 +871,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
                 t = __shfl(t, 0, 64);
                 steal = __shfl(steal, 0, 64);
                 if (t == 0xffffffffu) { exhausted = true; break; }
+                if (!LEARNED && L.tile_buf && nown < kOwnTiles) {
+                    if (lane == 0) own[wave][nown] = t;
+                    nown++;
+                }
                 tile = t;
                 qpos = 0;
             }
@@ -815,6 +912,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             if (want) rho = use_lds ? pm_density32_lds<CAP, UNROLL>(L.g, qx, qy, qz, buf, cand)
                                     : pm_density32(L.g, qx, qy, qz, false, cand);
             group_finish(L, r, G, j, want, nonskip, rho, cand, cn);
+        }
+    }
+    // The tiles this wave marched have all their rays done: their FP64 steps,
+    // normals and outputs are computed here, while other waves still march
+    // (k_finish then skips them).
+    if (!LEARNED && nown > 0) {
+        wave_lds_sync();
+        for (uint32_t i = 0; i < nown; i++) {
+            const uint32_t t = own[wave][i];
+            finish_tile(L, t, buf, cn);
+            if (lane == 0) L.tile_buf[t] = 2u;
         }
     }
     flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
@@ -1035,96 +1143,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1
     }
 }
 
-// ---- finish ----
-// Per hit record: the FP64 bisection step when the settings ask for it, the
-// normal (the six central differences in one fused walk, or the sum of gradW)
-// and the pixel's outputs.  Every lane of a wave does the same kind of work
-// and none of the march's state is held, so this runs packed and short; the
-// march kernels keep only the phases that decide where the hit is.
-__device__ __forceinline__ void finish_hit(const MarchLaunch &L, uint32_t px, uint32_t py, uint4 rec, float *buf,
-                                           Counts &cn)
-{
-    const pm_settings &s = L.s;
-    const GridDev &g = L.g;
-    const uint32_t idx = py * L.W + px;
-    const float lo = __uint_as_float(rec.y), hi = __uint_as_float(rec.z);
-    float d[3];
-    pm_ray_dir(L, px, py, d);
-    const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
-    float xs[3], th;
-    if (rec.x & 2u) {
-        // the last bisection step in double (RayMarcher.cpp:316-326 in FP64)
-        const double tmd = ((double)lo + (double)hi) * 0.5;
-        const double x = (double)o0 + tmd * (double)d[0];
-        const double y = (double)o1 + tmd * (double)d[1];
-        const double z = (double)o2 + tmd * (double)d[2];
-        const double rho = pm_density64(g, x, y, z);
-        cn.samples++;
-        double dlo = lo, dhi = hi;
-        if (rho >= (double)s.iso) dhi = tmd; else dlo = tmd;
-        const double tt = (dlo + dhi) * 0.5;
-        xs[0] = (float)((double)o0 + tt * (double)d[0]);
-        xs[1] = (float)((double)o1 + tt * (double)d[1]);
-        xs[2] = (float)((double)o2 + tt * (double)d[2]);
-        th = (float)tt;
-    } else {
-        th = lo;
-        xs[0] = o0 + th * d[0]; xs[1] = o1 + th * d[1]; xs[2] = o2 + th * d[2];
-    }
-    float nx, ny, nz;
-    if (s.normal_mode == PM_NORMAL_GRADW) {
-        float sx, sy, sz;
-        uint32_t cand = 0;
-        pm_sum_gradw(g, xs[0], xs[1], xs[2], sx, sy, sz, cand);
-        pm_normalize(sx, sy, sz, nx, ny, nz);
-    } else {
-        const bool comp = s.compensated != 0;
-        const float e = s.normal_eps;
-        float gv[3];
-        const Cdiff6 o = pm_cdiff6<kLdsCap>(g, xs[0], xs[1], xs[2], e, buf, comp);
-        if (o.cand != 0xffffffffu) {
-            cn.samples += 6;
-            cn.cand += o.cand;
-            gv[0] = o.v[0] - o.v[1]; gv[1] = o.v[2] - o.v[3]; gv[2] = o.v[4] - o.v[5];
-        } else {
-            // the fused walk's cells leave the grid: +x, -x, +y, -y, +z, -z one at a time
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                float q[3] = { xs[0], xs[1], xs[2] };
-                q[k >> 1] = (k & 1) ? xs[k >> 1] - e : xs[k >> 1] + e;
-                uint32_t cand = 0;
-                const float rho = comp ? pm_density32_lds<kLdsCap, 4>(g, q[0], q[1], q[2], buf, cand)
-                                       : pm_density32(g, q[0], q[1], q[2], false, cand);
-                cn.samples++;
-                cn.cand += cand;
-                if (k & 1) gv[k >> 1] -= rho; else gv[k >> 1] = rho;
-            }
-        }
-        // RayMarcher.cpp:331-338 with central differences
-        const float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
-        nx = ny = nz = 0.0f;
-        if (dd > 0.0f) {
-            const float inv = 1.0f / sqrtf(dd);
-            nx = -gv[0] * inv; ny = -gv[1] * inv; nz = -gv[2] * inv;
-        }
-    }
-    L.pos[idx] = make_float4(xs[0], xs[1], xs[2], 1.0f);
-    L.nrm[idx] = make_float4(nx, ny, nz, 1.0f);
-    if (L.t) L.t[idx] = th;
-    cn.hits++;
-}
-
-// One 8x8 tile per wave, one pixel per lane: the lanes whose pixel holds a
-// hit record finish it.
-__device__ __forceinline__ void finish_tile(const MarchLaunch &L, uint32_t tile, float *buf, Counts &cn)
-{
-    const uint32_t lane = threadIdx.x & 63, tiles_x = (L.W + 7) / 8;
-    const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
-    if (px >= L.W || py >= L.H) return;
-    const uint4 rec = L.hit_rec[py * L.W + px];
-    if (rec.x & 1u) finish_hit(L, px, py, rec, buf, cn);
-}
-
 // Single launch: the tiles the march visited (its active-tile list, or all);
 // batch: the tiles of the chunks, one chunk per workgroup at a time.  Four
 // waves per SIMD (registers capped at 128) hide more of the walks' latency
@@ -1146,8 +1164,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     } else {
         const uint32_t T = ((L0.W + 7) / 8) * ((L0.H + 7) / 8);
         const uint32_t TA = L0.tile_buf ? L0.tile_buf[2 * (T + 1) + T] : T;
-        for (uint32_t i = blockIdx.x * 4 + wave; i < TA; i += gridDim.x * 4)
-            finish_tile(L0, L0.tile_buf ? L0.tile_buf[2 * (T + 1) + i] : i, buf, cn);
+        for (uint32_t i = blockIdx.x * 4 + wave; i < TA; i += gridDim.x * 4) {
+            const uint32_t tile = L0.tile_buf ? L0.tile_buf[2 * (T + 1) + i] : i;
+            if (L0.tile_buf && L0.tile_buf[tile] == 2u) continue;     // finished by the march
+            finish_tile(L0, tile, buf, cn);
+        }
     }
     flush_counters(L0.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
 }
