@@ -55,10 +55,13 @@ def cpu_model() -> str:
 
 def alg_bytes(st: dict) -> float:
     """Bytes the march must read and write, from the launch's own counters: 12 B
-    (x, y, z) per stencil candidate, 200 B of row bounds per density sample (25
+    (x, y, z) per stencil candidate, 200 B of row bounds per stencil walk (25
     rows of the fine stencil, two u32 each) and 36 B of output per ray
-    (position and normal RGBA32F, t)."""
-    return 12.0 * st["cand"] + 200.0 * st["samples"] + 36.0 * st["rays"]
+    (position and normal RGBA32F, t).  A hit's six central-difference samples
+    share one walk (the candidate counter counts that walk's candidates once),
+    so there are samples - 5 hits walks."""
+    walks = st["samples"] - 5 * st["hits"]
+    return 12.0 * st["cand"] + 200.0 * walks + 36.0 * st["rays"]
 
 
 def run_gpu(m, frame, settings, cam, steps, warmup):
