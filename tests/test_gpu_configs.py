@@ -95,3 +95,34 @@ def test_error_vs_fp64_oracle(marcher, name):
     m = compare(pos[::step], nrm[::step], t[::step], p64[::step], n64[::step], t64[::step])
     print(name, "vs FP64:", m, "extent", extent(sc.xyz), flush=True)
     assert_within(m, extent(sc.xyz))
+
+
+def test_wave_tile_overflow_matches_finish_kernel(marcher):
+    """A 5120x2880 view from inside the particles' AABB: every one of the
+    230,400 tiles is active and most rays cross empty space quickly, so waves of
+    the persistent march take far more than the 32 tiles they finish themselves;
+    the rest go to the finish kernel.  The image equals the schedule without the
+    tile pre-pass (every tile finished by the finish kernel) and the oracle on a
+    row sample."""
+    rng = np.random.default_rng(21)
+    blobs = [rng.normal(cen, 0.03, (5000, 3)) for cen in ((0.3, 0.5, 0.7), (0.6, 0.4, 0.8), (0.5, 0.6, 0.5))]
+    xyz = np.concatenate(blobs + [rng.uniform(0.0, 1.0, (5000, 3))]).astype(np.float32)
+    sc = scenes.tiny_scene(xyz, 0.02, eye=(0.5, 0.5, 0.05), target=(0.5, 0.5, 1.0), width=5120, height=2880)
+    c = sc.cameras[0]
+    print(f"tile overflow n={sc.n} {c.width}x{c.height}", flush=True)
+    marcher.set_frame(80, sc.xyz, sc.h)
+    st = sc.settings(skip_mode=4)
+    a = marcher.march(80, st, c.inv_pv, c.eye, c.width, c.height)
+    b = marcher.march(80, dict(st, variant=7), c.inv_pv, c.eye, c.width, c.height)
+    print(a[3], flush=True)
+    assert a[3]["rays_in_aabb"] == c.width * c.height
+    assert a[3]["hits"] > 100_000
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    for k in ("hits", "samples", "cand", "march_samples"):
+        assert a[3][k] == b[3][k], k
+    G = O.Grid(sc.xyz, sc.h)
+    step = 360
+    op, on, ot, _ = O.march(G, O.make_settings(**{k: v for k, v in st.items() if k in KEYS}), c.inv_pv,
+                            c.eye, c.width, c.height, row_step=step)
+    _rows_equal(a[:3], (op, on, ot), step)
