@@ -1087,6 +1087,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1
     int qpos = 64;
     bool exhausted = false;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    // lane groups while a chunk drains (its last rays, before the next job)
+    int G = 1;
+    const bool group_ok = B.base.s.skip_mode == PM_SKIP_BOUND || B.base.s.skip_mode == PM_SKIP_NONE;
 
     while (true) {
         while (!exhausted) {
@@ -1107,6 +1110,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1
                     tile0 = ch.tile0;
                     ntiles = ch.ntiles;
                     tile_i = 0;
+                    G = 1;                                        // every lane its own ray again
                     continue;
                 }
             }
@@ -1131,9 +1135,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1
             if (nwant > 0 && (2 * nwant >= nbusy || exhausted || nwant == nbusy || (qpos >= 64 && tile_i >= ntiles)))
                 wave_propose(*Lp, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
         }
-        if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
-            if (use_lds) ray_step(*Lp, r, DensLds<kLdsCap, 4>{ &Lp->g, buf }, cn);
-            else ray_step(*Lp, r, DensPlain{ &Lp->g, false }, cn);
+        if (!LEARNED && group_ok && (exhausted || (qpos >= 64 && tile_i >= ntiles))) regroup(r, G, region);
+        if (G == 1) {
+            if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
+                if (use_lds) ray_step(*Lp, r, DensLds<kLdsCap, 4>{ &Lp->g, buf }, cn);
+                else ray_step(*Lp, r, DensPlain{ &Lp->g, false }, cn);
+            }
+        } else {
+            const int j = (int)(lane & (uint32_t)(G - 1));
+            float qx = 0.f, qy = 0.f, qz = 0.f, rho = 0.f;
+            bool nonskip = false, want = false;
+            uint32_t cand = 0;
+            if (r.phase != PH_DONE) want = group_prepare(*Lp, r, G, j, qx, qy, qz, nonskip);
+            if (want) rho = use_lds ? pm_density32_lds<kLdsCap, 4>(Lp->g, qx, qy, qz, buf, cand)
+                                    : pm_density32(Lp->g, qx, qy, qz, false, cand);
+            group_finish(*Lp, r, G, j, want, nonskip, rho, cand, cn);
         }
     }
     flush_counters(B.base.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
