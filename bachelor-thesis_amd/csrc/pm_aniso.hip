@@ -1,0 +1,603 @@
+// pm_aniso.hip -- the anisotropic march on CDNA4: PerPixel_Anisotropic
+// (RayMarcher.cpp:346-423), the reference's default mode.
+//
+// Per sample point q:
+//   1. the extended neighbours |x_j - q|^2 < hx^2, hx = ParticleRadiusExt
+//      (GetNeighborsExt, Dataset.cpp:282-290), at most 8192 (RayMarcher.cpp:14,
+//      381), walked on the search grid (cells of h) over the cells within two
+//      cells of q: rows dz outer, dy, x, by input index inside a cell;
+//   2. WPCA (RayMarcher.cpp:114-254): weights 1 - (|r|/hx)^3 (CubicKernel,
+//      Kernel.cpp:110-120), weighted mean and covariance C in two passes over
+//      that walk, a cyclic Jacobi eigen-decomposition of C, the clamp of [YT13]
+//      eq. 15 (:232-242) and G = h^-1 R diag(1/Sigma) R^T (:246-247);
+//   3. rho = sum over the extended neighbours within h of
+//      AnisotropicKernel::W(G, det G, r) (Kernel.cpp:63-83), walked over the
+//      cells within one cell of q (the same particles in the same order; the
+//      extended walk when the list was capped).
+// One ray per lane, one 8x8 tile per wave; every lane runs its ray to the end
+// (march, bisection, FP64 step, normal, outputs).  All float arithmetic is in
+// the order of the CPU oracle (oracle/pm_aniso.inc), -ffp-contract=off.
+#include "pm_internal.h"
+#include "pm_ray.h"
+
+namespace pm {
+
+static constexpr uint32_t kMaxNeighbors = 8192;       // RayMarcher.cpp:14
+static constexpr float kLimExt = 4.00390625f;         // 2 cells: 4 (1 + 2^-10)
+static constexpr float kLimNear = 1.0009765625f;      // 1 cell: 1 (1 + 2^-10)
+static constexpr int kJacobiSweeps = 5;
+
+// ---- search-grid walk ----
+struct SWalk {
+    int32_t c[3];   // q's cell, relative to the grid origin
+    float f[3];     // q's offset inside its cell, in cells
+};
+
+// False when every cell within R cells of q lies outside the grid.
+__device__ __forceinline__ bool an_centre(const GridDev &g, const float q[3], int R, SWalk &w)
+{
+    const int32_t org[3] = { g.ox, g.oy, g.oz }, dim[3] = { g.nx, g.ny, g.nz };
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float u = g.inv_h * q[a];
+        const int32_t k = q[a] >= 0.0f ? (int32_t)u : (int32_t)u - 1;
+        w.f[a] = u - (float)k;
+        w.c[a] = k - org[a];
+        if (w.c[a] < -R || w.c[a] > dim[a] - 1 + R) ok = false;
+    }
+    return ok;
+}
+
+// Row (dy, dz): false when it is dropped (farther than R cells, or outside the
+// grid); else the stored range [b, e) of its kept cells, clamped to the grid.
+__device__ __forceinline__ bool an_row(const GridDev &g, const SWalk &w, int R, float lim, int dy, int dz, uint32_t &b,
+                                       uint32_t &e)
+{
+    const float ty = pm_fine_off(dy, w.f[1]), tz = pm_fine_off(dz, w.f[2]);
+    const float r2 = ty * ty + tz * tz;
+    if (!(r2 < lim)) return false;
+    const int32_t y = w.c[1] + dy, z = w.c[2] + dz;
+    if (y < 0 || y >= g.ny || z < 0 || z >= g.nz) return false;
+    int lo = 0, hi = 0;
+    for (int d = -R; d < 0; d++) {
+        const float t = pm_fine_off(d, w.f[0]);
+        if (t * t + r2 < lim) { lo = d; break; }
+    }
+    for (int d = R; d > 0; d--) {
+        const float t = pm_fine_off(d, w.f[0]);
+        if (t * t + r2 < lim) { hi = d; break; }
+    }
+    int32_t xl = w.c[0] + lo, xh = w.c[0] + hi;
+    if (xl < 0) xl = 0;
+    if (xh > g.nx - 1) xh = g.nx - 1;
+    if (xl > xh) return false;
+    const uint32_t row = (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+    b = g.cell_start[row + (uint32_t)xl];
+    e = g.cell_start[row + (uint32_t)xh + 1];
+    return true;
+}
+
+// ---- WPCA ----
+template <class P>
+__device__ __forceinline__ typename P::T an_wcubic(typename P::T r2, const P &a)
+{
+    using T = typename P::T;
+    const T d = sqrt(r2);
+    if (d >= a.hx) return (T)0;
+    const T k = d * a.hxinv;
+    return (T)1 - k * k * k;
+}
+
+// One Jacobi rotation zeroing A[p][q] (r the third index).
+template <class T, int p, int q, int r>
+__device__ __forceinline__ void an_jrot(T (&A)[3][3], T (&V)[3][3])
+{
+    const T apq = A[p][q];
+    if (apq == (T)0) return;
+    const T theta = (A[q][q] - A[p][p]) / ((T)2 * apq);
+    T t = (T)1 / (fabs(theta) + sqrt(theta * theta + (T)1));
+    if (theta < (T)0) t = -t;
+    const T c = (T)1 / sqrt(t * t + (T)1);
+    const T s = t * c;
+    const T tp = t * apq;
+    A[p][p] = A[p][p] - tp;
+    A[q][q] = A[q][q] + tp;
+    A[p][q] = A[q][p] = (T)0;
+    const T arp = A[r][p], arq = A[r][q];
+    const T nrp = c * arp - s * arq, nrq = s * arp + c * arq;
+    A[r][p] = A[p][r] = nrp;
+    A[r][q] = A[q][r] = nrq;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const T vkp = V[k][p], vkq = V[k][q];
+        V[k][p] = c * vkp - s * vkq;
+        V[k][q] = s * vkp + c * vkq;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T an_max(T x, T y) { return (x < y) ? y : x; }   // std::max
+
+// C (lower triangle c00, c10, c11, c20, c21, c22) of N neighbours -> G (row
+// major, symmetric) and det G.
+template <class P>
+__device__ __forceinline__ void an_gmat(const typename P::T c[6], uint32_t N, const P &a, typename P::T G[9],
+                                        typename P::T &detG)
+{
+    using T = typename P::T;
+    T A[3][3] = { { c[0], c[1], c[3] }, { c[1], c[2], c[4] }, { c[3], c[4], c[5] } };
+    T V[3][3] = { { (T)1, (T)0, (T)0 }, { (T)0, (T)1, (T)0 }, { (T)0, (T)0, (T)1 } };
+    _Pragma("unroll 1") for (int sw = 0; sw < kJacobiSweeps; sw++) {
+        an_jrot<T, 0, 1, 2>(A, V);
+        an_jrot<T, 0, 2, 1>(A, V);
+        an_jrot<T, 1, 2, 0>(A, V);
+    }
+    T lam[3] = { A[0][0], A[1][1], A[2][2] };
+    if (N <= (uint32_t)a.n_eps) {
+        lam[0] = lam[1] = lam[2] = a.k_n;
+    } else {
+        const T mn = an_max(lam[0], an_max(lam[1], lam[2])) / a.k_r;
+#pragma unroll
+        for (int k = 0; k < 3; k++) lam[k] = an_max(lam[k], mn);
+#pragma unroll
+        for (int k = 0; k < 3; k++) lam[k] = lam[k] * a.k_s;
+    }
+    const T d[3] = { (T)1 / lam[0], (T)1 / lam[1], (T)1 / lam[2] };
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int s = r; s < 3; s++) {
+            const T t0 = ((a.hinv * V[r][0]) * d[0]) * V[s][0];
+            const T t1 = ((a.hinv * V[r][1]) * d[1]) * V[s][1];
+            const T t2 = ((a.hinv * V[r][2]) * d[2]) * V[s][2];
+            G[3 * r + s] = G[3 * s + r] = (t0 + t1) + t2;
+        }
+    // glm::determinant (mat3), m[c][r] = G[r][c]
+    const T A0 = G[0] * (G[4] * G[8] - G[5] * G[7]);
+    const T A1 = G[1] * (G[3] * G[8] - G[5] * G[6]);
+    const T A2 = G[2] * (G[3] * G[7] - G[4] * G[6]);
+    detG = (A0 - A1) + A2;
+}
+
+// The two WPCA passes: C in c; returns the extended-neighbour count (before
+// the cap) and adds both walks' candidates to cand.
+template <class P>
+__device__ __forceinline__ uint32_t an_cov(const GridDev &g, const P &a, const float qf[3], const typename P::T q[3],
+                                           typename P::T c[6], uint32_t &cand)
+{
+    using T = typename P::T;
+    SWalk w;
+    if (!an_centre(g, qf, 2, w)) return 0;
+    uint32_t nx = 0, cnd = 0;
+    T ws = (T)0, m0 = (T)0, m1 = (T)0, m2 = (T)0;
+    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++)
+        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
+            uint32_t b, e;
+            if (!an_row(g, w, 2, kLimExt, dy, dz, b, e)) continue;
+            cnd += e - b;
+            for (uint32_t i = b; i < e; i++) {
+                const float4 p = g.sorted[i];
+                const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
+                const T r2 = (rx * rx + ry * ry) + rz * rz;
+                if (r2 < a.R2x) {
+                    if (nx < kMaxNeighbors) {
+                        const T wt = an_wcubic(r2, a);
+                        ws += wt;
+                        m0 += wt * (T)p.x;
+                        m1 += wt * (T)p.y;
+                        m2 += wt * (T)p.z;
+                    }
+                    nx++;
+                }
+            }
+        }
+    if (nx == 0) { cand += cnd; return 0; }
+    const uint32_t N = nx < kMaxNeighbors ? nx : kMaxNeighbors;
+    const T inv = (T)1 / ws;
+    m0 = m0 * inv; m1 = m1 * inv; m2 = m2 * inv;
+    T c0 = (T)0, c1 = (T)0, c2 = (T)0, c3 = (T)0, c4 = (T)0, c5 = (T)0;
+    uint32_t n2 = 0;
+    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++)
+        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
+            uint32_t b, e;
+            if (!an_row(g, w, 2, kLimExt, dy, dz, b, e)) continue;
+            cnd += e - b;
+            for (uint32_t i = b; i < e; i++) {
+                const float4 p = g.sorted[i];
+                const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
+                const T r2 = (rx * rx + ry * ry) + rz * rz;
+                if (r2 < a.R2x && n2 < N) {
+                    const T wt = an_wcubic(r2, a);
+                    const T x0 = (T)p.x - m0, x1 = (T)p.y - m1, x2 = (T)p.z - m2;
+                    const T w0 = wt * x0, w1 = wt * x1, w2 = wt * x2;
+                    c0 += w0 * x0;
+                    c1 += w1 * x0;
+                    c2 += w1 * x1;
+                    c3 += w2 * x0;
+                    c4 += w2 * x1;
+                    c5 += w2 * x2;
+                    n2++;
+                }
+            }
+        }
+    cand += cnd;
+    c[0] = c0 * inv; c[1] = c1 * inv; c[2] = c2 * inv;
+    c[3] = c3 * inv; c[4] = c4 * inv; c[5] = c5 * inv;
+    return nx;
+}
+
+// AnisotropicKernel::W (Kernel.cpp:63-83), r' = G r as glm's mat3 * vec3.
+template <class P>
+__device__ __forceinline__ typename P::T an_W(const typename P::T G[9], typename P::T detG, typename P::T rx,
+                                              typename P::T ry, typename P::T rz, const P &a)
+{
+    using T = typename P::T;
+    const T g0 = (G[0] * rx + G[1] * ry) + G[2] * rz;
+    const T g1 = (G[3] * rx + G[4] * ry) + G[5] * rz;
+    const T g2 = (G[6] * rx + G[7] * ry) + G[8] * rz;
+    const T q2 = (g0 * g0 + g1 * g1) + g2 * g2;
+    if (q2 >= a.h2) return (T)0;
+    const T q = sqrt(q2) * a.hinv;
+    const T sd = a.sig * detG;
+    if (q >= (T)0.5) {
+        const T qq = (T)1 - q;
+        return sd * ((T)2 * qq * qq * qq);
+    }
+    return sd * ((T)6 * (q * q * q - q * q) + (T)1);
+}
+
+// AnisotropicKernel::gradW (Kernel.cpp:85-107): gradQ = normalize(r')/(|r'| h).
+__device__ __forceinline__ void an_gradW(const float G[9], float detG, float rx, float ry, float rz, const AnParF &a,
+                                         float &o0, float &o1, float &o2)
+{
+    const float g0 = (G[0] * rx + G[1] * ry) + G[2] * rz;
+    const float g1 = (G[3] * rx + G[4] * ry) + G[5] * rz;
+    const float g2 = (G[6] * rx + G[7] * ry) + G[8] * rz;
+    const float rn = (g0 * g0 + g1 * g1) + g2 * g2;
+    o0 = o1 = o2 = 0.0f;
+    if (rn >= a.h2) return;
+    const float rl = sqrtf(rn);
+    const float q = rl * a.hinv;
+    const float inv = 1.0f / sqrtf(rn);
+    const float den = rl * a.h;
+    const float q0 = (g0 * inv) / den, q1 = (g1 * inv) / den, q2 = (g2 * inv) / den;
+    if (q >= 0.5f) {
+        const float qq = 1.0f - q;
+        const float f = 6.0f * qq * qq;
+        const float ns = -a.sig * detG;
+        o0 = (ns * q0) * f; o1 = (ns * q1) * f; o2 = (ns * q2) * f;
+        return;
+    }
+    const float f = 6.0f * (3.0f * q * q - 2.0f * q);
+    const float sd = a.sig * detG;
+    o0 = (sd * q0) * f; o1 = (sd * q1) * f; o2 = (sd * q2) * f;
+}
+
+template <class T>
+__device__ __forceinline__ void an_neu_add(T &s, T &c, T w)
+{
+    const T t = s + w;
+    if (fabs(s) >= fabs(w)) c += (s - t) + w;
+    else c += (w - t) + s;
+    s = t;
+}
+
+// Density at q (compensated: Neumaier sum), or with grad != NULL the sum of
+// gradW over the same neighbours (FP32).  qf is the FP32 point that picks the
+// cells.
+template <class P>
+__device__ __forceinline__ typename P::T an_sum(const GridDev &g, const P &a, const float qf[3],
+                                                const typename P::T q[3], bool comp, uint32_t &cand,
+                                                float *grad = nullptr)
+{
+    using T = typename P::T;
+    T c[6], G[9], detG;
+    if (grad) grad[0] = grad[1] = grad[2] = 0.0f;
+    const uint32_t nx = an_cov(g, a, qf, q, c, cand);
+    if (nx == 0) return (T)0;
+    an_gmat(c, nx < kMaxNeighbors ? nx : kMaxNeighbors, a, G, detG);
+    const bool capped = nx > kMaxNeighbors;
+    const int R = capped ? 2 : 1;
+    const float lim = capped ? kLimExt : kLimNear;
+    SWalk w;
+    an_centre(g, qf, R, w);
+    T s = (T)0, cc = (T)0;
+    uint32_t n = 0, cnd = 0;
+    _Pragma("unroll 1") for (int dz = -R; dz <= R; dz++)
+        _Pragma("unroll 1") for (int dy = -R; dy <= R; dy++) {
+            uint32_t b, e;
+            if (!an_row(g, w, R, lim, dy, dz, b, e)) continue;
+            cnd += e - b;
+            for (uint32_t i = b; i < e; i++) {
+                const float4 p = g.sorted[i];
+                const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
+                const T r2 = (rx * rx + ry * ry) + rz * rz;
+                if (capped) {
+                    if (!(r2 < a.R2x)) continue;
+                    if (n++ >= kMaxNeighbors) continue;
+                }
+                if (!(r2 < a.h2)) continue;
+                if constexpr (sizeof(T) == 4) {
+                    if (grad) {
+                        float o0, o1, o2;
+                        an_gradW(G, detG, rx, ry, rz, a, o0, o1, o2);
+                        grad[0] += o0; grad[1] += o1; grad[2] += o2;
+                        continue;
+                    }
+                }
+                const T wv = an_W(G, detG, rx, ry, rz, a);
+                if (comp) an_neu_add(s, cc, wv);
+                else s += wv;
+            }
+        }
+    cand += cnd;
+    return comp ? s + cc : s;
+}
+
+__device__ __forceinline__ float an_density32(const MarchLaunch &L, float x, float y, float z, bool comp,
+                                              uint32_t &cand)
+{
+    const float q[3] = { x, y, z };
+    return an_sum(L.g, L.an.f, q, q, comp, cand);
+}
+
+__device__ __forceinline__ double an_density64(const GridDev &g, const AnParD &a, double x, double y, double z)
+{
+    const double q[3] = { x, y, z };
+    const float qf[3] = { (float)x, (float)y, (float)z };
+    uint32_t cand = 0;
+    return an_sum(g, a, qf, q, false, cand);
+}
+
+// Normal at the hit point p: normalize(sum gradW) (RayMarcher.cpp:402-409) or
+// normalize(-grad rho) by central differences (six samples, each with its own
+// G), counted as density evaluations.
+__device__ __forceinline__ void an_normal(const MarchLaunch &L, const float p[3], Counts &cn, float &nx, float &ny,
+                                          float &nz)
+{
+    const pm_settings &s = L.s;
+    if (s.normal_mode == PM_NORMAL_GRADW) {
+        float gs[3];
+        uint32_t cand = 0;
+        an_sum(L.g, L.an.f, p, p, false, cand, gs);
+        const float dd = (gs[0] * gs[0] + gs[1] * gs[1]) + gs[2] * gs[2];
+        const float inv = 1.0f / sqrtf(dd);
+        nx = gs[0] * inv; ny = gs[1] * inv; nz = gs[2] * inv;
+        return;
+    }
+    const bool comp = s.compensated != 0;
+    float gv[3];
+    _Pragma("unroll 1") for (int a = 0; a < 3; a++) {
+        float xp[3] = { p[0], p[1], p[2] }, xm[3] = { p[0], p[1], p[2] };
+        xp[a] = p[a] + s.normal_eps;
+        xm[a] = p[a] - s.normal_eps;
+        uint32_t cand = 0;
+        const float rp = an_density32(L, xp[0], xp[1], xp[2], comp, cand);
+        const float rm = an_density32(L, xm[0], xm[1], xm[2], comp, cand);
+        cn.samples += 2;
+        cn.cand += cand;
+        gv[a] = rp - rm;
+    }
+    const float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
+    nx = ny = nz = 0.0f;
+    if (dd > 0.0f) {
+        const float inv = 1.0f / sqrtf(dd);
+        nx = -gv[0] * inv; ny = -gv[1] * inv; nz = -gv[2] * inv;
+    }
+}
+
+// The configured march of one pixel with the anisotropic density: AABB entry,
+// fixed step, first sample at or above iso, bisection (last step in FP64),
+// normal, outputs.
+__device__ __forceinline__ void an_pixel_config(const MarchLaunch &L, uint32_t px, uint32_t py, Counts &cn)
+{
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
+    const bool comp = s.compensated != 0;
+    const uint32_t idx = py * L.W + px;
+    cn.rays++;
+    float d[3];
+    pm_ray_dir(L, px, py, d);
+    const float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
+    RayState r{};
+    if (!pm_slab(o, d, g.bmin, g.bmax, r.tn, r.tf)) {
+        write_miss(L, idx);
+        return;
+    }
+    cn.in_aabb++;
+    r.d0 = d[0]; r.d1 = d[1]; r.d2 = d[2];
+    r.k = 0;
+    const float step = s.step;
+    int hitk = -1;
+    for (;;) {
+        if (s.max_steps > 0 && r.k >= s.max_steps) break;
+        const float t = r.tn + (float)r.k * step;
+        if (t > r.tf) break;
+        const float x = o[0] + t * d[0], y = o[1] + t * d[1], z = o[2] + t * d[2];
+        if (s.skip_mode == PM_SKIP_EMPTY) {
+            const int kn = pm_skip_empty(g, o[0], o[1], o[2], r, x, y, z, step);
+            if (kn != r.k) { r.k = kn; continue; }
+        }
+        uint32_t cand = 0;
+        const float rho = an_density32(L, x, y, z, comp, cand);
+        cn.samples++; cn.march++; cn.cand += cand;
+        if (rho >= s.iso) { hitk = r.k; break; }
+        r.k++;
+    }
+    if (hitk < 0) {
+        write_miss(L, idx);
+        return;
+    }
+    cn.hits++;
+    float xs[3], th;
+    if (hitk == 0) {
+        th = r.tn;
+        xs[0] = o[0] + th * d[0]; xs[1] = o[1] + th * d[1]; xs[2] = o[2] + th * d[2];
+    } else {
+        float lo = r.tn + (float)(hitk - 1) * step, hi = r.tn + (float)hitk * step;
+        const bool fp64 = s.final_fp64 && s.bisect_iters > 0;
+        const int n32 = fp64 ? s.bisect_iters - 1 : s.bisect_iters;
+        for (int it = 0; it < n32; it++) {
+            const float tm = (lo + hi) * 0.5f;
+            uint32_t cand = 0;
+            const float rho = an_density32(L, o[0] + tm * d[0], o[1] + tm * d[1], o[2] + tm * d[2], comp, cand);
+            cn.samples++; cn.cand += cand;
+            if (rho >= s.iso) hi = tm; else lo = tm;
+        }
+        if (fp64) {
+            double dlo = lo, dhi = hi;
+            const double tm = (dlo + dhi) * 0.5;
+            const double rho = an_density64(g, L.an.d, (double)o[0] + tm * (double)d[0],
+                                            (double)o[1] + tm * (double)d[1], (double)o[2] + tm * (double)d[2]);
+            cn.samples++;
+            if (rho >= (double)s.iso) dhi = tm; else dlo = tm;
+            const double tt = (dlo + dhi) * 0.5;
+            xs[0] = (float)((double)o[0] + tt * (double)d[0]);
+            xs[1] = (float)((double)o[1] + tt * (double)d[1]);
+            xs[2] = (float)((double)o[2] + tt * (double)d[2]);
+            th = (float)tt;
+        } else {
+            th = (lo + hi) * 0.5f;
+            xs[0] = o[0] + th * d[0]; xs[1] = o[1] + th * d[1]; xs[2] = o[2] + th * d[2];
+        }
+    }
+    float nx, ny, nz;
+    an_normal(L, xs, cn, nx, ny, nz);
+    L.pos[idx] = make_float4(xs[0], xs[1], xs[2], 1.0f);
+    L.nrm[idx] = make_float4(nx, ny, nz, 1.0f);
+    if (L.t) L.t[idx] = th;
+}
+
+// PerPixel_Anisotropic itself: start on the depth-buffer surface, fixed step
+// vector, MaxSteps samples (DensityGrid skips not counted), first sample with
+// rho >= iso, normal = normalize(sum gradW) with that sample's G.
+__device__ __forceinline__ void an_pixel_reference(const MarchLaunch &L, uint32_t px, uint32_t py, Counts &cn)
+{
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
+    const bool comp = s.compensated != 0;
+    const uint32_t idx = py * L.W + px;
+    cn.rays++;
+    const float z = L.depth[idx];
+    float4 P = make_float4(0.f, 0.f, 0.f, 0.f), N = make_float4(0.f, 0.f, 0.f, 0.f);
+    float tout = -1.0f;
+    if (z != 1.0f) {                                                       // :354
+        cn.in_aabb++;
+        const float cx = (float)px * (2.0f / (float)L.W) - 1.0f;
+        const float cy = (float)py * (2.0f / (float)L.H) - 1.0f;
+        float p0, p1, p2, d0, d1, d2;
+        pm_unproject(L.M, cx, cy, z, p0, p1, p2);                          // :356-362
+        pm_normalize(p0 - L.cam[0], p1 - L.cam[1], p2 - L.cam[2], d0, d1, d2);
+        const float st0 = s.step * d0, st1 = s.step * d1, st2 = s.step * d2;
+        for (int i = 0; i < s.max_steps; i++) {
+            p0 += st0; p1 += st1; p2 += st2;                               // :367
+            if (s.skip_mode == PM_SKIP_DG_BOX || s.skip_mode == PM_SKIP_DG_CONS) {   // :370-376
+                float nmin[3], nmax[3];
+                while (pm_dg_query(g, p0, p1, p2, nmin, nmax)) {
+                    const float pin[3] = { p0, p1, p2 }, stv[3] = { st0, st1, st2 };
+                    float q[3];
+                    pm_intersect_aabb(pin, stv, nmin, nmax, q);
+                    p0 = q[0] + st0; p1 = q[1] + st1; p2 = q[2] + st2;
+                }
+            }
+            uint32_t cand = 0;
+            const float rho = an_density32(L, p0, p1, p2, comp, cand);
+            cn.samples++; cn.march++; cn.cand += cand;
+            if (rho >= s.iso) {                                            // :402
+                const float p[3] = { p0, p1, p2 };
+                float nx, ny, nz;
+                an_normal(L, p, cn, nx, ny, nz);
+                P = make_float4(p0, p1, p2, 1.0f);
+                N = make_float4(nx, ny, nz, 1.0f);
+                const float v0 = p0 - L.cam[0], v1 = p1 - L.cam[1], v2 = p2 - L.cam[2];
+                tout = sqrtf((v0 * v0 + v1 * v1) + v2 * v2);
+                cn.hits++;
+                break;
+            }
+        }
+    }
+    L.pos[idx] = P;
+    L.nrm[idx] = N;
+    if (L.t) L.t[idx] = tout;
+}
+
+__global__ __launch_bounds__(256) void k_march_aniso(MarchLaunch L)
+{
+    uint32_t px, py;
+    if (!pm_tile(L.W, L.H, px, py)) return;
+    Counts cn;
+    if (px < L.W && py < L.H) {
+        if (L.s.mode == PM_MODE_REFERENCE) an_pixel_reference(L, px, py, cn);
+        else an_pixel_config(L, px, py, cn);
+    }
+    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+}
+
+hipError_t launch_march_aniso(const MarchLaunch &L, hipStream_t st)
+{
+    const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
+    hipLaunchKernelGGL(k_march_aniso, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
+    return hipGetLastError();
+}
+
+// ---- point queries used by the parity tests ----
+
+__global__ __launch_bounds__(256) void k_density_aniso(GridDev g, AnParF a, const float *__restrict__ q, uint32_t nq,
+                                                       int comp, float *__restrict__ rho)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float p[3] = { q[3 * i], q[3 * i + 1], q[3 * i + 2] };
+    uint32_t cand = 0;
+    rho[i] = an_sum(g, a, p, p, comp != 0, cand);
+}
+
+__global__ __launch_bounds__(256) void k_density_aniso64(GridDev g, AnParD a, const double *__restrict__ q,
+                                                         uint32_t nq, double *__restrict__ rho)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    rho[i] = an_density64(g, a, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+}
+
+__global__ __launch_bounds__(256) void k_aniso_matrix(GridDev g, AnParF a, const float *__restrict__ q, uint32_t nq,
+                                                      float *__restrict__ G, float *__restrict__ det,
+                                                      uint32_t *__restrict__ next)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float p[3] = { q[3 * i], q[3 * i + 1], q[3 * i + 2] };
+    float c[6], Gm[9], dt = 0.0f;
+    uint32_t cand = 0;
+    const uint32_t nx = an_cov(g, a, p, p, c, cand);
+    if (nx != 0) an_gmat(c, nx < kMaxNeighbors ? nx : kMaxNeighbors, a, Gm, dt);
+    else
+        for (int k = 0; k < 9; k++) Gm[k] = 0.0f;
+    for (int k = 0; k < 9; k++) G[9 * i + k] = Gm[k];
+    det[i] = dt;
+    next[i] = nx;
+}
+
+hipError_t launch_density_aniso(const GridDev &g, const AnParF &a, const float *q, uint32_t nq, int compensated,
+                                float *rho, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_density_aniso, dim3((nq + 255) / 256), dim3(256), 0, st, g, a, q, nq, compensated, rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_aniso64(const GridDev &g, const AnParD &a, const double *q, uint32_t nq, double *rho,
+                                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_density_aniso64, dim3((nq + 255) / 256), dim3(256), 0, st, g, a, q, nq, rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_aniso_matrix(const GridDev &g, const AnParF &a, const float *q, uint32_t nq, float *G, float *det,
+                               uint32_t *next, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_aniso_matrix, dim3((nq + 255) / 256), dim3(256), 0, st, g, a, q, nq, G, det, next);
+    return hipGetLastError();
+}
+
+}  // namespace pm

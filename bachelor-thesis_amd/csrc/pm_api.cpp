@@ -458,6 +458,15 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
     if (s->learned && !(s->learned_alpha >= 0.0f && s->learned_alpha <= 4.0f)) return fail(c, PM_ERR_ARG, "learned_alpha out of [0,4]");
     if (s->mode == PM_MODE_REFERENCE && (s->max_steps < 0 || s->max_steps > (1 << 24)))
         return fail(c, PM_ERR_ARG, "max_steps out of range");
+    if (s->anisotropic) {
+        if (!(s->k_n > 0.0f && s->k_r > 0.0f && s->k_s > 0.0f) || !std::isfinite(s->k_n) || !std::isfinite(s->k_r) ||
+            !std::isfinite(s->k_s))
+            return fail(c, PM_ERR_ARG, "anisotropic: k_n, k_r and k_s must be finite and > 0");
+        if (s->ext_mult != 0.0f && !(s->ext_mult >= 1.0f && s->ext_mult <= 16.0f))
+            return fail(c, PM_ERR_ARG, "anisotropic: ext_mult must be 0 (= 2) or in [1, 16]");
+        if (s->learned) return fail(c, PM_ERR_ARG, "anisotropic: no learned step");
+        if (s->skip_mode == PM_SKIP_BOUND) return fail(c, PM_ERR_ARG, "anisotropic: the bound skip holds for the isotropic kernel only");
+    }
     if (s->mode == PM_MODE_CONFIG) {
         double diag = 0.0;
         for (int a = 0; a < 3; a++) {
@@ -468,6 +477,25 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
             return fail(c, PM_ERR_ARG, "step too small for the frame extent (> 2^24 samples per ray)");
     }
     return PM_OK;
+}
+
+// Anisotropic parameters of a frame (h = ParticleRadius) and settings, as
+// Dataset.cpp:169-177 and Kernel.cpp:55-61 derive them, in FP32 and FP64.
+static AnisoDev aniso_params(const Frame *f, const pm_settings *s)
+{
+    AnisoDev a{};
+    const float mult = s->ext_mult != 0.0f ? s->ext_mult : 2.0f;
+    AnParF &F = a.f;
+    F.h = f->info.h; F.h2 = F.h * F.h; F.hinv = 1.0f / F.h;
+    F.hx = mult * F.h; F.R2x = F.hx * F.hx; F.hxinv = 1.0f / F.hx;
+    F.sig = 8.0f / PM_PI_F;
+    F.k_n = s->k_n; F.k_r = s->k_r; F.k_s = s->k_s; F.n_eps = s->n_eps;
+    AnParD &D = a.d;
+    D.h = (double)f->info.h; D.h2 = D.h * D.h; D.hinv = 1.0 / D.h;
+    D.hx = (double)mult * D.h; D.R2x = D.hx * D.hx; D.hxinv = 1.0 / D.hx;
+    D.sig = 8.0 / PM_PI_D;
+    D.k_n = s->k_n; D.k_r = s->k_r; D.k_s = s->k_s; D.n_eps = s->n_eps;
+    return a;
 }
 
 int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float inv_pv[16], const float cam[3],
@@ -512,12 +540,14 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     L.t = (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
     L.band_ctr = (uint32_t *)c->d_band.p;
-    if (s->mode == PM_MODE_CONFIG) {
+    if (s->mode == PM_MODE_CONFIG && !s->anisotropic) {
         HIPCHK(c, c->d_hits.ensure(sizeof(uint4) * npx));
         L.hit_rec = (uint4 *)c->d_hits.p;
     }
+    if (s->anisotropic) L.an = aniso_params(f, s);
     // tile order and miss pre-pass for the persistent schedules (variant 7: without)
-    const bool persistent = s->learned || (s->variant != 1 && s->variant != 4 && s->variant != 5);
+    const bool persistent =
+        !s->anisotropic && (s->learned || (s->variant != 1 && s->variant != 4 && s->variant != 5));
     if (s->mode == PM_MODE_CONFIG && persistent && s->variant != 7) {
         uint64_t tiles = (uint64_t)((W + 7) / 8) * ((H + 7) / 8);
         HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (3 * (tiles + 1) + scan_tmp_size((uint32_t)(tiles + 1)))));
@@ -685,6 +715,7 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
     if (!views || n_views == 0 || n_views > (1u << 20)) return fail(c, PM_ERR_ARG, "pm_batch_set: need 1..2^20 views");
     if (!s) return fail(c, PM_ERR_ARG, "settings is NULL");
     if (s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "pm_batch_set: the batch runs the configured march");
+    if (s->anisotropic) return fail(c, PM_ERR_ARG, "pm_batch_set: the batch runs the isotropic march");
     HIPCHK(c, hipSetDevice(c->device));
     c->b_ready = false;
     if (c->b_exec) { hipGraphExecDestroy(c->b_exec); c->b_exec = nullptr; }
@@ -935,6 +966,76 @@ int pm_density64(pm_ctx *c, int32_t frame, const double *q, uint64_t nq, double 
     return PM_OK;
 }
 
+static int aniso_query_settings(pm_ctx *c, const pm_settings *s)
+{
+    if (!s) return fail(c, PM_ERR_ARG, "settings is NULL");
+    const pm_settings &t = *s;
+    if (!(t.k_n > 0.0f && t.k_r > 0.0f && t.k_s > 0.0f) || !std::isfinite(t.k_n) || !std::isfinite(t.k_r) ||
+        !std::isfinite(t.k_s))
+        return fail(c, PM_ERR_ARG, "anisotropic: k_n, k_r and k_s must be finite and > 0");
+    if (t.ext_mult != 0.0f && !(t.ext_mult >= 1.0f && t.ext_mult <= 16.0f))
+        return fail(c, PM_ERR_ARG, "anisotropic: ext_mult must be 0 (= 2) or in [1, 16]");
+    return PM_OK;
+}
+
+int pm_density_aniso(pm_ctx *c, int32_t frame, const pm_settings *s, const float *q, uint64_t nq, float *rho)
+{
+    Frame *f;
+    int rc = run_query(c, frame, nq, &f);
+    if (rc) return rc;
+    if (!q || !rho) return fail(c, PM_ERR_ARG, "q and rho are required");
+    if ((rc = aniso_query_settings(c, s))) return rc;
+    const AnisoDev a = aniso_params(f, s);
+    HIPCHK(c, c->d_q.ensure(12 * nq));
+    HIPCHK(c, c->d_out.ensure(4 * nq));
+    HIPCHK(c, hipMemcpyAsync(c->d_q.p, q, 12 * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_density_aniso(f->g, a.f, (const float *)c->d_q.p, (uint32_t)nq, s->compensated,
+                                   (float *)c->d_out.p, c->stream));
+    HIPCHK(c, hipMemcpyAsync(rho, c->d_out.p, 4 * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
+int pm_density_aniso64(pm_ctx *c, int32_t frame, const pm_settings *s, const double *q, uint64_t nq, double *rho)
+{
+    Frame *f;
+    int rc = run_query(c, frame, nq, &f);
+    if (rc) return rc;
+    if (!q || !rho) return fail(c, PM_ERR_ARG, "q and rho are required");
+    if ((rc = aniso_query_settings(c, s))) return rc;
+    const AnisoDev a = aniso_params(f, s);
+    HIPCHK(c, c->d_q.ensure(24 * nq));
+    HIPCHK(c, c->d_out.ensure(8 * nq));
+    HIPCHK(c, hipMemcpyAsync(c->d_q.p, q, 24 * nq, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_density_aniso64(f->g, a.d, (const double *)c->d_q.p, (uint32_t)nq, (double *)c->d_out.p,
+                                     c->stream));
+    HIPCHK(c, hipMemcpyAsync(rho, c->d_out.p, 8 * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
+int pm_aniso_matrix(pm_ctx *c, int32_t frame, const pm_settings *s, const float *q, uint64_t nq, float *G, float *det,
+                    uint32_t *n_ext)
+{
+    Frame *f;
+    int rc = run_query(c, frame, nq, &f);
+    if (rc) return rc;
+    if (!q || !G || !det || !n_ext) return fail(c, PM_ERR_ARG, "q, G, det and n_ext are required");
+    if ((rc = aniso_query_settings(c, s))) return rc;
+    const AnisoDev a = aniso_params(f, s);
+    HIPCHK(c, c->d_q.ensure(12 * nq));
+    HIPCHK(c, c->d_out.ensure(48 * nq));
+    HIPCHK(c, hipMemcpyAsync(c->d_q.p, q, 12 * nq, hipMemcpyHostToDevice, c->stream));
+    float *dG = (float *)c->d_out.p, *dd = dG + 9 * nq;
+    uint32_t *dn = (uint32_t *)(dd + nq);
+    HIPCHK(c, launch_aniso_matrix(f->g, a.f, (const float *)c->d_q.p, (uint32_t)nq, dG, dd, dn, c->stream));
+    HIPCHK(c, hipMemcpyAsync(G, dG, 36 * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(det, dd, 4 * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(n_ext, dn, 4 * nq, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
 // ---------------------------------------------------------------- learned --
 
 int pm_mlp_set(pm_ctx *c, const float *params)
@@ -1083,7 +1184,8 @@ int pm_trace_append(pm_ctx *c, int32_t frame, const pm_settings *s, const float 
 {
     if (!c || !s || !inv_pv || !cam || ray_stride == 0 || k_stride == 0) return PM_ERR_ARG;
     if (c->ds_cap == 0) return fail(c, PM_ERR_STATE, "pm_trace_append: call pm_dataset_clear first");
-    if (s->mode != PM_MODE_CONFIG || s->learned) return fail(c, PM_ERR_ARG, "pm_trace_append: traces come from the plain configured march");
+    if (s->mode != PM_MODE_CONFIG || s->learned || s->anisotropic)
+        return fail(c, PM_ERR_ARG, "pm_trace_append: traces come from the plain configured march");
     int rc = pm_march_async(c, frame, s, inv_pv, cam, W, H, nullptr, nullptr, nullptr, nullptr);
     if (rc) return rc;
     rc = pm_wait(c);

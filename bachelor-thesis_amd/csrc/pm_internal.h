@@ -23,6 +23,24 @@ struct CoarseDev {
     int32_t nx, ny, nz;
 };
 
+// Anisotropic density parameters (PerPixel_Anisotropic, RayMarcher.cpp:346-423):
+// h = ParticleRadius, hx = ParticleRadiusExt, sig = 8/pi (Kernel.cpp:57-61),
+// the clamp knobs k_n, k_r, k_s, N_eps (RayMarcher.h:22-25); FP32 and FP64.
+struct AnParF {
+    using T = float;
+    float h, h2, hinv, hx, R2x, hxinv, sig, k_n, k_r, k_s;
+    int32_t n_eps;
+};
+struct AnParD {
+    using T = double;
+    double h, h2, hinv, hx, R2x, hxinv, sig, k_n, k_r, k_s;
+    int32_t n_eps;
+};
+struct AnisoDev {
+    AnParF f;
+    AnParD d;
+};
+
 struct MarchLaunch {
     GridDev g;
     pm_settings s;
@@ -49,6 +67,7 @@ struct MarchLaunch {
     // per pixel: the hit handed from the march to k_finish (1 | fp64 << 1,
     // lo, hi, 0), zeros for a miss; written for every pixel of a marched tile
     uint4 *hit_rec;
+    AnisoDev an;               // s.anisotropic
 };
 
 // Batched launch: many (view, frame) jobs in one persistent kernel; every job
@@ -102,6 +121,14 @@ hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t len, uint
 hipError_t launch_grid_build(const GridBuildArgs &a, hipStream_t st);
 
 hipError_t launch_march(const MarchLaunch &L, hipStream_t st);
+// The anisotropic march (configured or reference mode), one kernel.
+hipError_t launch_march_aniso(const MarchLaunch &L, hipStream_t st);
+hipError_t launch_density_aniso(const GridDev &g, const AnParF &a, const float *q, uint32_t nq, int compensated,
+                                float *rho, hipStream_t st);
+hipError_t launch_density_aniso64(const GridDev &g, const AnParD &a, const double *q, uint32_t nq, double *rho,
+                                  hipStream_t st);
+hipError_t launch_aniso_matrix(const GridDev &g, const AnParF &a, const float *q, uint32_t nq, float *G, float *det,
+                               uint32_t *next, hipStream_t st);
 hipError_t launch_density(const GridDev &g, const float *q, uint32_t nq, int compensated, float *rho,
                           hipStream_t st);
 hipError_t launch_density64(const GridDev &g, const double *q, uint32_t nq, double *rho, hipStream_t st);

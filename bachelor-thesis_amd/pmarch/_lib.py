@@ -30,7 +30,9 @@ class PmSettings(C.Structure):
         ("bisect_iters", C.c_int32), ("final_fp64", C.c_int32), ("normal_eps", C.c_float),
         ("normal_mode", C.c_int32), ("skip_mode", C.c_int32), ("compensated", C.c_int32),
         ("learned", C.c_int32), ("variant", C.c_int32), ("learned_alpha", C.c_float),
-        ("learned_min_skip", C.c_int32), ("learned_max_skip", C.c_int32), ("reserved", C.c_int32 * 1),
+        ("learned_min_skip", C.c_int32), ("learned_max_skip", C.c_int32),
+        ("anisotropic", C.c_int32), ("k_n", C.c_float), ("k_r", C.c_float), ("k_s", C.c_float),
+        ("n_eps", C.c_int32), ("ext_mult", C.c_float), ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -95,6 +97,9 @@ def load():
     L.pm_neighbor_counts.argtypes = [vp, i32, vp, u64, vp, vp]
     L.pm_density.argtypes = [vp, i32, vp, u64, C.c_int, vp]
     L.pm_density64.argtypes = [vp, i32, vp, u64, vp]
+    L.pm_density_aniso.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp]
+    L.pm_density_aniso64.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp]
+    L.pm_aniso_matrix.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp, vp, vp]
     L.pm_stream_copy_gbps.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
     L.pm_get_skip_grids.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.pm_mlp_set.argtypes = [vp, vp]
@@ -342,3 +347,32 @@ class Marcher:
         rho = np.zeros(q.shape[0], np.float64)
         self._check(self.L.pm_density64(self.ctx, frame, _ptr(q), q.shape[0], _ptr(rho)), "pm_density64")
         return rho
+
+    def density_aniso(self, frame: int, settings: dict, q):
+        """Anisotropic density (RayMarcher.cpp:378-400) at query points, FP32."""
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+        rho = np.zeros(q.shape[0], np.float32)
+        s = settings_from(settings)
+        self._check(self.L.pm_density_aniso(self.ctx, frame, C.byref(s), _ptr(q), q.shape[0], _ptr(rho)),
+                    "pm_density_aniso")
+        return rho
+
+    def density_aniso64(self, frame: int, settings: dict, q):
+        q = np.ascontiguousarray(q, np.float64).reshape(-1, 3)
+        rho = np.zeros(q.shape[0], np.float64)
+        s = settings_from(settings)
+        self._check(self.L.pm_density_aniso64(self.ctx, frame, C.byref(s), _ptr(q), q.shape[0], _ptr(rho)),
+                    "pm_density_aniso64")
+        return rho
+
+    def aniso_matrix(self, frame: int, settings: dict, q):
+        """WPCA at query points: (G [n,3,3], det G [n], extended-neighbour counts [n])."""
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+        n = q.shape[0]
+        G = np.zeros((n, 9), np.float32)
+        det = np.zeros(n, np.float32)
+        cnt = np.zeros(n, np.uint32)
+        s = settings_from(settings)
+        self._check(self.L.pm_aniso_matrix(self.ctx, frame, C.byref(s), _ptr(q), n, _ptr(G), _ptr(det), _ptr(cnt)),
+                    "pm_aniso_matrix")
+        return G.reshape(n, 3, 3), det, cnt

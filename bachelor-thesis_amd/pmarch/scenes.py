@@ -104,6 +104,34 @@ def dam_break(seed: int = 0, nx: int = 80, ny: int = 160, nz: int = 80, spacing:
                  notes=f"{nx}x{ny}x{nz} lattice spacing {spacing}, N(0,{jitter_sigma}) jitter, seed {seed}", **prm)
 
 
+ANISO_SETTINGS = dict(anisotropic=1, k_n=0.5, k_r=2.0, k_s=2000.0, n_eps=1, ext_mult=2.0)
+"""The reference's defaults (AdvancedRenderer.cpp:18-28, config.yml particleRadiusMultiplier)."""
+
+
+def aniso_settings(sc: "Scene", iso: float = 1.0, **kw) -> dict:
+    """Configured-march settings of the anisotropic density for a scene at the
+    reference's tuning (h = 0.1, IsoDensity 1.0): the scene's step and normal
+    offset, the reference's k_n, k_r, k_s, N_eps and radius multiplier."""
+    d = sc.settings(iso=float(np.float32(iso)), **ANISO_SETTINGS)
+    d.update(kw)
+    return d
+
+
+def dam_break_aniso(seed: int = 0, scale: float = 10.0, width: int = 1920, height: int = 1080, **kw) -> Scene:
+    """The dam-break column scaled by 10 about the origin: 1,024,000 particles at
+    spacing 5 cm, h = 0.1 -- the particle radius the reference's anisotropic
+    march is tuned for (config.yml particleRadius 0.1; AdvancedRenderer.cpp:18-28
+    k_s = 2000 against covariances of order h^2: G ~ I/h in the bulk, which
+    makes the kernel scale-dependent).  Same camera geometry, scaled."""
+    base = dam_break(seed, width=width, height=height, **kw)
+    xyz = (base.xyz.astype(np.float64) * scale).astype(np.float32)
+    prm = _defaults(float(np.float32(base.h * scale)))
+    eye = orbit_eye(tuple(scale * c for c in COLUMN_CENTRE), 1.3 * scale, 30.0, 45.0)
+    cams = [make_camera(eye, tuple(scale * c for c in COLUMN_CENTRE), width, height)]
+    return Scene("dam_break_aniso", xyz, cameras=cams,
+                 notes=f"dam_break(seed {seed}) scaled by {scale}: h = {prm['h']}", **prm)
+
+
 def displaced_frames(base: Scene, frames: int, seed: int, rel: float = 0.01, spacing: float = 0.005):
     """Frame f = frame f-1 + an independent uniform displacement of norm <= rel*spacing
     per particle (seeded), i.e. a slowly moving particle set."""

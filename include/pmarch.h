@@ -55,8 +55,9 @@ extern "C" {
                                  march without the skip                                */
 
 /* March settings.  The first four fields carry VisualizationSettings'
- * MaxSteps / StepSize / IsoDensity (RayMarcher.h:16-18); the rest select the
- * configured march.  Layout is fixed: 16 x 4 bytes. */
+ * MaxSteps / StepSize / IsoDensity (RayMarcher.h:16-18), the anisotropic
+ * fields its EnableAnisotropy / k_n / k_r / k_s / N_eps (RayMarcher.h:20-25);
+ * the rest select the configured march.  Layout is fixed: 24 x 4 bytes. */
 typedef struct pm_settings {
     int32_t mode;          /* PM_MODE_*                                              */
     int32_t max_steps;     /* reference: MaxSteps; configured: cap on march samples, 0 = none */
@@ -80,7 +81,18 @@ typedef struct pm_settings {
     float   learned_alpha; /* learned: fraction of the predicted distance skipped (0 = 0.7) */
     int32_t learned_min_skip; /* learned: shorter proposals end the proposals (0 = 4)  */
     int32_t learned_max_skip; /* learned: longest jump in steps (0 = 24)              */
-    int32_t reserved[1];
+    /* PerPixel_Anisotropic (RayMarcher.cpp:346-423): per sample, WPCA over the
+       particles within ext_mult * h (RayMarcher.cpp:114-254), then the
+       anisotropic kernel (Kernel.cpp:63-107) over those within h.  Either
+       mode; not with the learned step, the bound skip or the batch.          */
+    int32_t anisotropic;   /* EnableAnisotropy                                        */
+    float   k_n;           /* k_n: Sigma for N <= N_eps (> 0)                         */
+    float   k_r;           /* k_r: eigenvalues clamped to max / k_r (> 0)             */
+    float   k_s;           /* k_s: scale of the clamped eigenvalues (> 0)             */
+    int32_t n_eps;         /* N_eps                                                   */
+    float   ext_mult;      /* ParticleRadiusExt / ParticleRadius (Dataset.cpp:169-177),
+                              >= 1; 0 = 2 (config.yml particleRadiusMultiplier)       */
+    int32_t reserved[3];
 } pm_settings;
 
 /* Per-launch counters, written by the march kernel itself. */
@@ -196,6 +208,14 @@ int pm_neighbor_counts(pm_ctx *ctx, int32_t frame, const float *q, uint64_t nq,
 int pm_density(pm_ctx *ctx, int32_t frame, const float *q, uint64_t nq, int compensated,
                float *rho);
 int pm_density64(pm_ctx *ctx, int32_t frame, const double *q, uint64_t nq, double *rho);
+/* The anisotropic density at query points (RayMarcher.cpp:378-400) with the
+ * anisotropic fields (and compensated) of s: FP32 and FP64. */
+int pm_density_aniso(pm_ctx *ctx, int32_t frame, const pm_settings *s, const float *q, uint64_t nq, float *rho);
+int pm_density_aniso64(pm_ctx *ctx, int32_t frame, const pm_settings *s, const double *q, uint64_t nq, double *rho);
+/* WPCA at query points (RayMarcher.cpp:114-254): G (9 floats per point, row
+ * major), det G and the number of extended neighbours (0: G and det are 0). */
+int pm_aniso_matrix(pm_ctx *ctx, int32_t frame, const pm_settings *s, const float *q, uint64_t nq, float *G,
+                    float *det, uint32_t *n_ext);
 
 /* ---- learned step proposal (no counterpart in the reference; it replaces the
  * role of the DensityGrid skip, Dataset.cpp:94-164, RayMarcher.cpp:284-304) ----

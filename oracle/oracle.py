@@ -34,6 +34,12 @@ class Settings(C.Structure):
         ("normal_mode", C.c_int32),
         ("skip_mode", C.c_int32),
         ("compensated", C.c_int32),
+        ("anisotropic", C.c_int32),
+        ("k_n", C.c_float),
+        ("k_r", C.c_float),
+        ("k_s", C.c_float),
+        ("n_eps", C.c_int32),
+        ("ext_mult", C.c_float),
     ]
 
 
@@ -78,6 +84,12 @@ class GridStruct(C.Structure):
 
 
 _lib = None
+
+
+def _default_threads() -> int:
+    """Worker threads of the oracle's marches: the machine's cores less one,
+    at most 16 (a GPU box's CPU share)."""
+    return max(1, min(16, (os.cpu_count() or 2) - 1))
 
 
 def lib():
@@ -133,6 +145,24 @@ def lib():
                                       fp(np.float32, flags="C"), fp(np.float32, flags="C"), C.c_uint32,
                                       C.c_uint32, C.c_uint32, C.c_int, fp(np.float32, flags="C"),
                                       fp(np.float32, flags="C"), C.c_void_p, C.POINTER(Counters)]
+        L.or_density_aniso_f.restype = C.c_float
+        L.or_density_aniso_f.argtypes = [C.POINTER(GridStruct), C.POINTER(Settings), fp(np.float32, flags="C"),
+                                         C.c_int, C.POINTER(C.c_uint32)]
+        L.or_density_aniso_d.restype = C.c_double
+        L.or_density_aniso_d.argtypes = [C.POINTER(GridStruct), C.POINTER(Settings), fp(np.float64, flags="C")]
+        L.or_gradW_aniso_f.argtypes = [C.POINTER(GridStruct), C.POINTER(Settings), fp(np.float32, flags="C"),
+                                       fp(np.float32, flags="C")]
+        L.or_aniso_G_f.restype = C.c_uint32
+        L.or_aniso_G_f.argtypes = [C.POINTER(GridStruct), C.POINTER(Settings), fp(np.float32, flags="C"),
+                                   fp(np.float32, flags="C"), C.POINTER(C.c_float)]
+        L.or_aniso_G_d.restype = C.c_uint32
+        L.or_aniso_G_d.argtypes = [C.POINTER(GridStruct), C.POINTER(Settings), fp(np.float64, flags="C"),
+                                   fp(np.float64, flags="C"), C.POINTER(C.c_double)]
+        L.or_eig3_f.argtypes = [fp(np.float32, flags="C")] * 3
+        L.or_eig3_d.argtypes = [fp(np.float64, flags="C")] * 3
+        L.or_neighbors_ext.restype = C.c_uint32
+        L.or_neighbors_ext.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), C.c_float,
+                                       fp(np.float32, flags="C"), C.c_uint32, C.POINTER(C.c_uint32)]
         L.or_ray.argtypes = [fp(np.float32, flags="C"), fp(np.float32, flags="C"), C.c_uint32, C.c_uint32,
                              C.c_uint32, fp(np.float32, flags="C"), fp(np.float32, flags="C")]
         L.or_slab.argtypes = [fp(np.float32, flags="C")] * 5
@@ -271,11 +301,69 @@ class Grid:
                                  cand.ctypes.data)
         return (rho, cand) if with_cand else rho
 
+    def density_aniso(self, settings, q, compensated=None, with_cand=False):
+        """Anisotropic density (RayMarcher.cpp:378-400) at each row of q, FP32."""
+        q = np.ascontiguousarray(np.asarray(q, np.float32).reshape(-1, 3))
+        comp = settings.compensated if compensated is None else int(compensated)
+        rho = np.zeros(q.shape[0], np.float32)
+        cand = np.zeros(q.shape[0], np.uint32)
+        c = C.c_uint32(0)
+        for i in range(q.shape[0]):
+            rho[i] = lib().or_density_aniso_f(C.byref(self.g), C.byref(settings), q[i], comp, C.byref(c))
+            cand[i] = c.value
+        return (rho, cand) if with_cand else rho
+
+    def density_aniso64(self, settings, q):
+        q = np.ascontiguousarray(np.asarray(q, np.float64).reshape(-1, 3))
+        return np.array([lib().or_density_aniso_d(C.byref(self.g), C.byref(settings), q[i])
+                         for i in range(q.shape[0])], np.float64)
+
+    def gradW_aniso(self, settings, q) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        lib().or_gradW_aniso_f(C.byref(self.g), C.byref(settings), np.ascontiguousarray(q, np.float32), out)
+        return out
+
+    def aniso_G(self, settings, q, fp64=False):
+        """(number of extended neighbours, G (3x3), det G) of WPCA at q."""
+        if fp64:
+            G = np.zeros(9, np.float64)
+            det = C.c_double(0.0)
+            n = lib().or_aniso_G_d(C.byref(self.g), C.byref(settings), np.ascontiguousarray(q, np.float64), G,
+                                   C.byref(det))
+        else:
+            G = np.zeros(9, np.float32)
+            det = C.c_float(0.0)
+            n = lib().or_aniso_G_f(C.byref(self.g), C.byref(settings), np.ascontiguousarray(q, np.float32), G,
+                                   C.byref(det))
+        return int(n), G.reshape(3, 3), float(det.value)
+
+    def neighbors_ext(self, q, hx) -> tuple[np.ndarray, int]:
+        """Positions of the particles with |x - q|^2 < hx^2 in the walk order, and the candidates visited."""
+        q = np.ascontiguousarray(q, np.float32)
+        cap = 1 << 16
+        out = np.zeros(3 * cap, np.float32)
+        cand = C.c_uint32(0)
+        n = lib().or_neighbors_ext(C.byref(self.g), q, float(np.float32(hx)), out, cap, C.byref(cand))
+        assert n <= cap
+        return out[:3 * n].reshape(-1, 3).copy(), int(cand.value)
+
     def density64(self, q):
         q = np.ascontiguousarray(np.asarray(q, np.float64).reshape(-1, 3))
         rho = np.zeros(q.shape[0], np.float64)
         lib().or_density_batch_d(C.byref(self.g), q.reshape(-1), q.shape[0], rho)
         return rho
+
+
+def eig3(c6, fp64=False):
+    """Jacobi eigen-decomposition of the symmetric 3x3 matrix with lower
+    triangle c6 = (c00, c10, c11, c20, c21, c22): (eigenvalues, V) with
+    eigenvector k in column k of V."""
+    dt = np.float64 if fp64 else np.float32
+    c = np.ascontiguousarray(c6, dt)
+    lam = np.zeros(3, dt)
+    V = np.zeros(9, dt)
+    (lib().or_eig3_d if fp64 else lib().or_eig3_f)(c, lam, V)
+    return lam, V.reshape(3, 3)
 
 
 def neighbors_brute(xyz, h, q) -> np.ndarray:
@@ -310,7 +398,8 @@ def slab(o, d, bmin, bmax):
 def make_settings(**kw) -> Settings:
     s = Settings()
     defaults = dict(mode=0, max_steps=0, step=0.0, iso=0.0, bisect_iters=8, final_fp64=1,
-                    normal_eps=0.0, normal_mode=0, skip_mode=0, compensated=1)
+                    normal_eps=0.0, normal_mode=0, skip_mode=0, compensated=1,
+                    anisotropic=0, k_n=0.5, k_r=2.0, k_s=2000.0, n_eps=1, ext_mult=2.0)
     defaults.update(kw)
     for k, v in defaults.items():
         setattr(s, k, v)
@@ -321,7 +410,7 @@ def march(grid: Grid, settings: Settings, inv_pv, cam, W: int, H: int, depth=Non
           threads=None, want_t=True):
     """Oracle march, FP32 (configured or reference mode, selected by settings.mode)."""
     if threads is None:
-        threads = max(1, (os.cpu_count() or 2) - 1)
+        threads = _default_threads()
     inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
     cam = np.ascontiguousarray(cam, np.float32).reshape(3)
     pos = np.zeros((H, W, 4), np.float32)
@@ -344,7 +433,7 @@ def march(grid: Grid, settings: Settings, inv_pv, cam, W: int, H: int, depth=Non
 def march64(grid: Grid, settings: Settings, inv_pv, cam, W: int, H: int, row_step=1, threads=None):
     """FP64 configured march: the ground truth."""
     if threads is None:
-        threads = max(1, (os.cpu_count() or 2) - 1)
+        threads = _default_threads()
     inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
     cam = np.ascontiguousarray(cam, np.float32).reshape(3)
     pos = np.zeros((H, W, 4), np.float64)
@@ -373,7 +462,7 @@ class HashedSearch:
 
     def march(self, grid: Grid, settings: Settings, inv_pv, cam, W, H, row_step=1, threads=None):
         if threads is None:
-            threads = max(1, (os.cpu_count() or 2) - 1)
+            threads = _default_threads()
         inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
         cam = np.ascontiguousarray(cam, np.float32).reshape(3)
         pos = np.zeros((H, W, 4), np.float32)

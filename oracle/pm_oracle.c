@@ -883,6 +883,188 @@ int or_slab(const float o[3], const float d[3], const float bmin[3], const float
 }
 
 /* ------------------------------------------------------------------------- */
+/* Search-grid walk (cells of h) for the anisotropic density.                 */
+/* ------------------------------------------------------------------------- */
+
+#define OR_MAX_NEIGHBORS 8192   /* RayMarcher.cpp:14 */
+#define OR_NEAR_LIM 1.0009765625f   /* 1 (1 + 2^-10): rows within h = 1 cell      */
+#define OR_JACOBI_SWEEPS 5
+
+/* The walk over the cells within R cells of q (R = 2 for the extended radius
+ * 2h, R = 1 for h): q's cell (CompactNSearch key) and its offsets f inside the
+ * cell, in cell units; rows (dy, dz), dz outer, each row the cells whose
+ * distance to q is below R (1 + 2^-10) -- the same rule as the march grid's
+ * walk, with cells of h -- clamped to the grid (cells outside hold no
+ * particle).  0 when every cell of the walk is outside the grid. */
+typedef struct { int32_t c[3]; float f[3]; } or_sw;
+
+static int sw_centre(const or_grid *g, const float q[3], int R, or_sw *w)
+{
+    int ok = 1;
+    for (int a = 0; a < 3; a++) {
+        float u = g->inv_h * q[a];
+        int32_t k = q[a] >= 0.0f ? (int32_t)u : (int32_t)u - 1;
+        w->f[a] = u - (float)k;
+        w->c[a] = k - g->origin[a];
+        if (w->c[a] < -R || w->c[a] > g->dims[a] - 1 + R) ok = 0;
+    }
+    return ok;
+}
+
+static int sw_row(const or_grid *g, const or_sw *w, int R, float lim, int dy, int dz, uint32_t *b, uint32_t *e)
+{
+    float ty = fine_off(dy, w->f[1]), tz = fine_off(dz, w->f[2]);
+    float r2 = ty * ty + tz * tz;
+    if (!(r2 < lim)) return 0;
+    int32_t y = w->c[1] + dy, z = w->c[2] + dz;
+    if (y < 0 || y >= g->dims[1] || z < 0 || z >= g->dims[2]) return 0;
+    int lo = 0, hi = 0;
+    for (int d = -R; d < 0; d++) {
+        float t = fine_off(d, w->f[0]);
+        if (t * t + r2 < lim) { lo = d; break; }
+    }
+    for (int d = R; d > 0; d--) {
+        float t = fine_off(d, w->f[0]);
+        if (t * t + r2 < lim) { hi = d; break; }
+    }
+    int32_t xl = w->c[0] + lo, xh = w->c[0] + hi;
+    if (xl < 0) xl = 0;
+    if (xh > g->dims[0] - 1) xh = g->dims[0] - 1;
+    if (xl > xh) return 0;
+    *b = g->cell_start[cell_lin(g, xl, y, z)];
+    *e = g->cell_start[cell_lin(g, xh, y, z) + 1];
+    return 1;
+}
+
+typedef struct { float h, h2, hinv, hx, R2x, hxinv, sig, k_n, k_r, k_s; int32_t n_eps; } an_par_f;
+typedef struct { double h, h2, hinv, hx, R2x, hxinv, sig, k_n, k_r, k_s; int32_t n_eps; } an_par_d;
+
+/* ParticleRadius h, ParticleRadiusExt = multiplier * h (Dataset.cpp:169-177),
+ * AnisotropicKernel's sig = 8 / pi (Kernel.cpp:57-61), the clamp knobs of
+ * VisualizationSettings (RayMarcher.h:22-25). */
+static void an_par_make_f(const or_grid *g, const or_settings *s, an_par_f *a)
+{
+    a->h = g->h; a->h2 = a->h * a->h; a->hinv = 1.0f / a->h;
+    a->hx = s->ext_mult * a->h; a->R2x = a->hx * a->hx; a->hxinv = 1.0f / a->hx;
+    a->sig = 8.0f / OR_PI_F;
+    a->k_n = s->k_n; a->k_r = s->k_r; a->k_s = s->k_s; a->n_eps = s->n_eps;
+}
+
+static void an_par_make_d(const or_grid *g, const or_settings *s, an_par_d *a)
+{
+    a->h = (double)g->h; a->h2 = a->h * a->h; a->hinv = 1.0 / a->h;
+    a->hx = (double)s->ext_mult * a->h; a->R2x = a->hx * a->hx; a->hxinv = 1.0 / a->hx;
+    a->sig = 8.0 / OR_PI_D;
+    a->k_n = s->k_n; a->k_r = s->k_r; a->k_s = s->k_s; a->n_eps = s->n_eps;
+}
+
+#define REAL float
+#define SFX(x) x##_f
+#define SQRT sqrtf
+#define FABS fabsf
+#include "pm_aniso.inc"
+#undef REAL
+#undef SFX
+#undef SQRT
+#undef FABS
+#define REAL double
+#define SFX(x) x##_d
+#define SQRT sqrt
+#define FABS fabs
+#include "pm_aniso.inc"
+#undef REAL
+#undef SFX
+#undef SQRT
+#undef FABS
+
+float or_density_aniso_f(const or_grid *g, const or_settings *s, const float q[3], int compensated, uint32_t *ncand)
+{
+    an_par_f a;
+    an_par_make_f(g, s, &a);
+    uint32_t cand = 0;
+    float rho = an_sum_f(g, &a, q, q, compensated, &cand, NULL);
+    if (ncand) *ncand = cand;
+    return rho;
+}
+
+double or_density_aniso_d(const or_grid *g, const or_settings *s, const double q[3])
+{
+    an_par_d a;
+    an_par_make_d(g, s, &a);
+    uint32_t cand = 0;
+    const float qf[3] = { (float)q[0], (float)q[1], (float)q[2] };
+    return an_sum_d(g, &a, qf, q, 0, &cand, NULL);
+}
+
+void or_gradW_aniso_f(const or_grid *g, const or_settings *s, const float q[3], float out[3])
+{
+    an_par_f a;
+    an_par_make_f(g, s, &a);
+    uint32_t cand = 0;
+    an_sum_f(g, &a, q, q, 0, &cand, out);
+}
+
+uint32_t or_aniso_G_f(const or_grid *g, const or_settings *s, const float q[3], float G[9], float *detG)
+{
+    an_par_f a;
+    an_par_make_f(g, s, &a);
+    uint32_t cand = 0;
+    return an_wpca_f(g, &a, q, q, G, detG, &cand);
+}
+
+uint32_t or_aniso_G_d(const or_grid *g, const or_settings *s, const double q[3], double G[9], double *detG)
+{
+    an_par_d a;
+    an_par_make_d(g, s, &a);
+    uint32_t cand = 0;
+    const float qf[3] = { (float)q[0], (float)q[1], (float)q[2] };
+    return an_wpca_d(g, &a, qf, q, G, detG, &cand);
+}
+
+uint32_t or_aniso_C_f(const or_grid *g, const or_settings *s, const float q[3], float c[6])
+{
+    an_par_f a;
+    an_par_make_f(g, s, &a);
+    uint32_t cand = 0;
+    return an_cov_f(g, &a, q, q, c, &cand);
+}
+
+void or_aniso_G_from_C_f(const or_grid *g, const or_settings *s, const float c[6], uint32_t n, float G[9],
+                         float *detG)
+{
+    an_par_f a;
+    an_par_make_f(g, s, &a);
+    an_gmat_f(c, n, &a, G, detG);
+}
+
+void or_eig3_f(const float c[6], float lam[3], float V[9]) { an_eig3_f(c, lam, (float(*)[3])V); }
+void or_eig3_d(const double c[6], double lam[3], double V[9]) { an_eig3_d(c, lam, (double(*)[3])V); }
+
+uint32_t or_neighbors_ext(const or_grid *g, const float q[3], float hx, float *out, uint32_t cap, uint32_t *ncand)
+{
+    or_sw w;
+    const float R2x = hx * hx;
+    uint32_t n = 0, cand = 0;
+    if (sw_centre(g, q, 2, &w)) {
+        for (int dz = -2; dz <= 2; dz++)
+            for (int dy = -2; dy <= 2; dy++) {
+                uint32_t b, e;
+                if (!sw_row(g, &w, 2, OR_FINE_LIM, dy, dz, &b, &e)) continue;
+                cand += e - b;
+                for (uint32_t i = b; i < e; i++) {
+                    const float *p = g->sorted + 4 * (uint64_t)i;
+                    float rx = p[0] - q[0], ry = p[1] - q[1], rz = p[2] - q[2];
+                    if (!((rx * rx + ry * ry) + rz * rz < R2x)) continue;
+                    if (n < cap) { out[3 * n] = p[0]; out[3 * n + 1] = p[1]; out[3 * n + 2] = p[2]; }
+                    n++;
+                }
+            }
+    }
+    if (ncand) *ncand = cand;
+    return n;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Density providers: dense grid or hashed table, same canonical order.       */
 /* ------------------------------------------------------------------------- */
 
@@ -892,15 +1074,18 @@ typedef struct {
     uint32_t *nbuf;      /* thread-local neighbour buffer (hashed)  */
     float h2, hinv, sig;
     int compensated;
+    const or_settings *an;   /* non-NULL: anisotropic density (PerPixel_Anisotropic) */
 } dens_ctx;
-
-#define OR_MAX_NEIGHBORS 8192   /* RayMarcher.cpp:14 */
 
 static float dens_f(dens_ctx *dc, const float q[3], or_counters *cnt)
 {
     uint32_t cand = 0;
     float rho;
-    if (dc->hs) {
+    if (dc->an) {
+        /* RayMarcher.cpp:378-400: WPCA over the extended neighbours, then the
+         * anisotropic kernel over those within h. */
+        rho = or_density_aniso_f(dc->g, dc->an, q, dc->compensated, &cand);
+    } else if (dc->hs) {
         /* RayMarcher.cpp:309-325: neighbour list, then sum of W over it. */
         uint32_t nn = hash_neighbors(dc->hs, q, dc->nbuf, OR_MAX_NEIGHBORS, &cand);
         if (nn > OR_MAX_NEIGHBORS) nn = OR_MAX_NEIGHBORS;
@@ -929,6 +1114,12 @@ static void sum_gradW(dens_ctx *dc, const float q[3], float out[3], or_counters 
     or_fine fs;
     out[0] = out[1] = out[2] = 0.0f;
     uint32_t cand = 0;
+    if (dc->an) {
+        /* RayMarcher.cpp:405-409: normal += gradW(G, det G, r) with the
+         * sample's own G. */
+        or_gradW_aniso_f(g, dc->an, q, out);
+        return;
+    }
     if (or_fine_centre(g, q, &fs)) {
         for (int dz = -2; dz <= 2; dz++)
             for (int dy = -2; dy <= 2; dy++) {
@@ -1013,7 +1204,7 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
             double tm = (dlo + dhi) * 0.5;
             double x[3] = { (double)o[0] + tm * (double)d[0], (double)o[1] + tm * (double)d[1],
                             (double)o[2] + tm * (double)d[2] };
-            double rho = or_density_d(g, x);
+            double rho = dc->an ? or_density_aniso_d(g, dc->an, x) : or_density_d(g, x);
             cnt->samples++;
             if (rho >= (double)s->iso) dhi = tm; else dlo = tm;
             double tt = (dlo + dhi) * 0.5;
@@ -1031,7 +1222,7 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
          * reads them (or_cdiff_union); their densities are the single-sample
          * ones.  The hashed search keeps one count per sample. */
         uint32_t ucand;
-        const int fused = !dc->hs && or_cdiff_union(dc->g, xs, s->normal_eps, &ucand, NULL, 0, NULL);
+        const int fused = !dc->hs && !dc->an && or_cdiff_union(dc->g, xs, s->normal_eps, &ucand, NULL, 0, NULL);
         or_counters one;
         memset(&one, 0, sizeof one);
         or_counters *c6 = fused ? &one : cnt;
@@ -1161,6 +1352,7 @@ static void *march_worker(void *arg)
     dc.hinv = 1.0f / j->g->h;
     dc.sig = or_sigma_f(j->g->h);
     dc.compensated = j->s->compensated;
+    dc.an = j->s->anisotropic ? j->s : NULL;
     dc.nbuf = j->hs ? (uint32_t *)malloc(OR_MAX_NEIGHBORS * sizeof(uint32_t)) : NULL;
     or_counters c;
     memset(&c, 0, sizeof(c));
@@ -1189,6 +1381,8 @@ static int run_march(const or_grid *g, const or_hash *hs, const or_settings *s, 
     if (s->mode == OR_MODE_REFERENCE && !depth) return -1;
     if (!(s->step > 0.0f)) return -1;
     if (s->skip_mode == 4 && (!g->bound || g->bound_iso != s->iso)) return -2;   /* or_bound_build first */
+    /* the sub-box bound holds for the isotropic kernel only */
+    if (s->anisotropic && (s->skip_mode == 4 || !(s->ext_mult >= 1.0f))) return -1;
     if (row_step == 0) row_step = 1;
     march_job j;
     memset(&j, 0, sizeof(j));
@@ -1224,7 +1418,7 @@ int or_march_hashed(const or_hash *hs, const or_grid *g, const or_settings *s,
                     uint32_t row_step, int nthreads, float *pos, float *nrm, float *tout,
                     or_counters *cnt)
 {
-    if (!hs || s->mode != OR_MODE_CONFIG || s->normal_mode != 0) return -1;
+    if (!hs || s->mode != OR_MODE_CONFIG || s->normal_mode != 0 || s->anisotropic) return -1;
     return run_march(g, hs, s, invpv, cam, W, H, NULL, row_step, nthreads, pos, nrm, tout, cnt);
 }
 
@@ -1243,6 +1437,11 @@ typedef struct {
     _Atomic uint64_t next;
     uint64_t total;
 } march64_job;
+
+static double dens64(const or_grid *g, const or_settings *s, const double x[3])
+{
+    return s->anisotropic ? or_density_aniso_d(g, s, x) : or_density_d(g, x);
+}
 
 static void pixel_f64(const march64_job *j, uint32_t idx)
 {
@@ -1279,7 +1478,7 @@ static void pixel_f64(const march64_job *j, uint32_t idx)
         double t = tn + (double)k * step;
         if (t > tf) break;
         double x[3] = { o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2] };
-        if (or_density_d(g, x) >= iso) { hitk = k; break; }
+        if (dens64(g, s, x) >= iso) { hitk = k; break; }
     }
     if (hitk < 0) return;
     double th;
@@ -1289,7 +1488,7 @@ static void pixel_f64(const march64_job *j, uint32_t idx)
         for (int it = 0; it < s->bisect_iters; it++) {
             double tm = (lo + hi) * 0.5;
             double x[3] = { o[0] + tm * d[0], o[1] + tm * d[1], o[2] + tm * d[2] };
-            if (or_density_d(g, x) >= iso) hi = tm; else lo = tm;
+            if (dens64(g, s, x) >= iso) hi = tm; else lo = tm;
         }
         th = (lo + hi) * 0.5;
     }
@@ -1299,7 +1498,7 @@ static void pixel_f64(const march64_job *j, uint32_t idx)
         double xp[3] = { xs[0], xs[1], xs[2] }, xm[3] = { xs[0], xs[1], xs[2] };
         xp[a] += s->normal_eps;
         xm[a] -= s->normal_eps;
-        gv[a] = or_density_d(g, xp) - or_density_d(g, xm);
+        gv[a] = dens64(g, s, xp) - dens64(g, s, xm);
     }
     double gg = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
     double n[3] = { 0, 0, 0 };
@@ -1325,6 +1524,7 @@ int or_march_f64(const or_grid *g, const or_settings *s, const float invpv[16], 
                  double *pos, double *nrm, double *tout)
 {
     if (!g || !s || s->mode != OR_MODE_CONFIG || !(s->step > 0.0f) || W == 0 || H == 0) return -1;
+    if (s->anisotropic && !(s->ext_mult >= 1.0f)) return -1;
     if (row_step == 0) row_step = 1;
     march64_job j;
     memset(&j, 0, sizeof(j));

@@ -23,7 +23,8 @@
 extern "C" {
 #endif
 
-/* Same field order and meaning as pm_settings in include/pmarch.h. */
+/* pm_settings (include/pmarch.h): its first ten fields, then its anisotropic
+ * fields, same names and meaning. */
 typedef struct or_settings {
     int32_t mode;          /* 0 = configured march, 1 = reference march          */
     int32_t max_steps;     /* reference mode: samples that count (MaxSteps)       */
@@ -38,6 +39,11 @@ typedef struct or_settings {
                               flag (reference mode); 4 sub-box bound skip
                               (configured mode, needs or_bound_build)             */
     int32_t compensated;   /* 1 = Neumaier-compensated FP32 density sums          */
+    /* anisotropic density (PerPixel_Anisotropic, RayMarcher.cpp:346-423) */
+    int32_t anisotropic;   /* 1 = WPCA + AnisotropicKernel instead of CubicSpline  */
+    float   k_n, k_r, k_s; /* VisualizationSettings k_n, k_r, k_s (RayMarcher.h:22-24) */
+    int32_t n_eps;         /* VisualizationSettings N_eps                          */
+    float   ext_mult;      /* ParticleRadiusExt / ParticleRadius (>= 1)            */
 } or_settings;
 
 enum { OR_MODE_CONFIG = 0, OR_MODE_REFERENCE = 1 };
@@ -138,6 +144,31 @@ double or_density_d(const or_grid *g, const double q[3]);
 void   or_density_batch_f(const or_grid *g, const float *q, uint64_t nq, int compensated,
                           float *rho, uint32_t *ncand);
 void   or_density_batch_d(const or_grid *g, const double *q, uint64_t nq, double *rho);
+
+/* ---- anisotropic density (RayMarcher.cpp:114-254, 378-409; Kernel.cpp:55-125) ----
+ * The extended neighbours (|x_j - q|^2 < (ext_mult h)^2, at most 8192) are
+ * walked on the search grid (cells of h) over the cells within 2 cells of q,
+ * rows dz outer, dy, x, by input index inside a cell. */
+float  or_density_aniso_f(const or_grid *g, const or_settings *s, const float q[3], int compensated, uint32_t *ncand);
+double or_density_aniso_d(const or_grid *g, const or_settings *s, const double q[3]);
+void   or_gradW_aniso_f(const or_grid *g, const or_settings *s, const float q[3], float out[3]);
+/* G (row major) and det G of WPCA at q; returns the number of extended
+ * neighbours (0: G undefined). */
+uint32_t or_aniso_G_f(const or_grid *g, const or_settings *s, const float q[3], float G[9], float *detG);
+uint32_t or_aniso_G_d(const or_grid *g, const or_settings *s, const double q[3], double G[9], double *detG);
+/* The weighted covariance C of WPCA at q (lower triangle c00, c10, c11, c20,
+ * c21, c22); returns the number of extended neighbours.  G and det G from C
+ * and that number (capped at 8192). */
+uint32_t or_aniso_C_f(const or_grid *g, const or_settings *s, const float q[3], float c[6]);
+void or_aniso_G_from_C_f(const or_grid *g, const or_settings *s, const float c[6], uint32_t n, float G[9],
+                         float *detG);
+/* Cyclic Jacobi eigen-decomposition of the symmetric 3x3 matrix given by its
+ * lower triangle c = (c00, c10, c11, c20, c21, c22): eigenvalues lam[k],
+ * eigenvector k = column k of V (row major). */
+void or_eig3_f(const float c[6], float lam[3], float V[9]);
+void or_eig3_d(const double c[6], double lam[3], double V[9]);
+/* The extended neighbours of q in walk order (positions, 3 floats each). */
+uint32_t or_neighbors_ext(const or_grid *g, const float q[3], float hx, float *out, uint32_t cap, uint32_t *ncand);
 
 /* ---- ray generation (RayMarcher.cpp:268-274) and AABB slab test ---- */
 void or_ray(const float invpv[16], const float cam[3], uint32_t W, uint32_t H, uint32_t idx,

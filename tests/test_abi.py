@@ -24,13 +24,18 @@ def test_settings_layout_matches_header():
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     fields = re.findall(r"(int32_t|float)\s+(\w+)(\[\d+\])?;", body)
     assert [f[1] for f in fields] == [f[0] for f in _lib.PmSettings._fields_]
-    assert ctypes.sizeof(_lib.PmSettings) == 64
+    assert ctypes.sizeof(_lib.PmSettings) == 96
 
 
-def test_oracle_settings_prefix_matches():
+def test_oracle_settings_fields_match():
+    """The oracle's settings are pm_settings' first ten fields, then its
+    anisotropic fields, with the same names and types."""
     import oracle as O
-    ours = [f[0] for f in _lib.PmSettings._fields_][: len(O.Settings._fields_)]
-    assert ours == [f[0] for f in O.Settings._fields_]
+    ours = dict((f[0], f[1]) for f in _lib.PmSettings._fields_)
+    theirs = O.Settings._fields_
+    assert [f[0] for f in _lib.PmSettings._fields_][:10] == [f[0] for f in theirs[:10]]
+    for name, typ in theirs:
+        assert name in ours and ours[name] is typ, name
 
 
 def test_no_gpu_fails_loudly():
