@@ -1,8 +1,9 @@
 // RayMarcher.cpp -- the reference's marcher interface (RayMarcher.cpp:66-112)
 // over libpmarch.so.  Prepare() records the view and uploads the frame's
 // particles once (the dataset is immutable after load, Dataset.cpp:169-227);
-// Start() launches the isotropic reference march (PerPixel_Isotropic,
-// RayMarcher.cpp:256-344): depth-buffer start, MaxSteps fixed steps,
+// Start() launches the reference march the settings select (RayMarcher.cpp:
+// 102-112): PerPixel_Isotropic (:256-344) or, with EnableAnisotropy,
+// PerPixel_Anisotropic (:346-423) -- depth-buffer start, MaxSteps fixed steps,
 // DensityGrid skip, first sample with rho >= IsoDensity, normal =
 // normalize(sum gradW).
 #include "RayMarcher.h"
@@ -71,11 +72,6 @@ void RayMarcher::Prepare(const VisualizationSettings& settings,
 void RayMarcher::Start()
 {
     if (!m_Ctx) return;
-    if (m_Settings.EnableAnisotropy && !m_WarnedAnisotropy) {
-        std::fprintf(stderr, "RayMarcher: the anisotropic march is not available on the GPU; "
-                             "using the isotropic march\n");
-        m_WarnedAnisotropy = true;
-    }
     pm_settings s{};
     s.mode = PM_MODE_REFERENCE;
     s.max_steps = m_Settings.MaxSteps;
@@ -84,6 +80,14 @@ void RayMarcher::Start()
     s.normal_mode = PM_NORMAL_GRADW;
     s.skip_mode = PM_SKIP_DG_BOX;
     s.compensated = 0;
+    if (m_Settings.EnableAnisotropy) {                          // RayMarcher.cpp:104-106
+        s.anisotropic = 1;
+        s.k_n = m_Settings.k_n;
+        s.k_r = m_Settings.k_r;
+        s.k_s = m_Settings.k_s;
+        s.n_eps = m_Settings.N_eps;
+        s.ext_mult = m_Dataset->ParticleRadiusExt / m_Dataset->ParticleRadius;   // Dataset.cpp:169-177
+    }
     int rc = pm_march_async(m_Ctx, m_Settings.Frame, &s, m_InvProjectionView, m_CameraPosition, m_Width, m_Height,
                             m_Depth, (float*)m_Positions, (float*)m_Normals, nullptr);
     m_InFlight = rc == PM_OK;

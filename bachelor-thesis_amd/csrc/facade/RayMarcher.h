@@ -81,5 +81,4 @@ private:
     pm_ctx* m_Ctx = nullptr;
     std::vector<bool> m_FrameOnDevice;
     bool m_InFlight = false;
-    bool m_WarnedAnisotropy = false;
 };

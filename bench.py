@@ -23,6 +23,13 @@ same march and forms.
 cameras) over 16 consecutive frames of the 1M dam break (1 % displacement per
 frame), all 1,024 view-frames in one persistent launch replayed from a
 hipGraph; one step = one replay of the whole batch.
+
+--workload aniso_dam_1080p: the anisotropic march (PerPixel_Anisotropic, the
+reference's default mode: per sample WPCA over the particles within 2h, then
+the anisotropic kernel) on the dam-break column scaled to h = 0.1 (the
+particle radius its k_s = 2000 is tuned for), 1920x1080, IsoDensity 1.0,
+step h/8, 8 bisections (last in FP64), central-difference normals, empty-cell
+skip.
 """
 from __future__ import annotations
 
@@ -141,6 +148,91 @@ def traffic_for(path: str, workload: str):
     return None
 
 
+def alg_bytes_aniso(st: dict) -> float:
+    """Anisotropic march: 12 B per candidate of its walks (two passes over the
+    2h stencil for WPCA, one over the h stencil for the density: the candidate
+    counter counts all three), 8 B of row bounds per row (25 + 25 + 9 rows per
+    sample) and 36 B of output per ray."""
+    return 12.0 * st["cand"] + 8.0 * 59 * st["samples"] + 36.0 * st["rays"]
+
+
+def cpu_baseline_aniso(scene, cam, settings, budget_s: float):
+    """The oracle's anisotropic march (the same algorithm, grid walks, one pixel
+    per atomic fetch_add) on a row subsample of the same view."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    threads = max(1, min(16, os.cpu_count() or 2) - 1)
+    G = O.Grid(scene.xyz, scene.h)
+    st = O.make_settings(**{k: v for k, v in settings.items() if k in dict(O.Settings._fields_)})
+    W, H = cam.width, cam.height
+    probe = 64
+    t0 = time.perf_counter()
+    O.march(G, st, cam.inv_pv, cam.eye, W, H, row_step=probe, threads=threads, want_t=False)
+    tp = time.perf_counter() - t0
+    rows_probe = (H + probe - 1) // probe
+    rows = max(rows_probe, min(H, int(budget_s / max(tp / rows_probe, 1e-6))))
+    step = max(1, H // rows)
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _, _, _, cnt = O.march(G, st, cam.inv_pv, cam.eye, W, H, row_step=step, threads=threads, want_t=False)
+        runs.append(time.perf_counter() - t0)
+        if sum(runs) > budget_s:
+            break
+    dt = float(np.median(runs))
+    return {"value": round(cnt["rays"] / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{'every %dth row of ' % step if step > 1 else ''}the same {W}x{H} view "
+                      f"({cnt['rays']} rays), median of {len(runs)} runs of {dt:.2f} s, {threads} threads on "
+                      f"{os.cpu_count()} cpus ({cpu_model()})"}
+
+
+def bench_aniso(args):
+    """The anisotropic march on the dam-break column scaled to h = 0.1, 1080p."""
+    import pmarch
+    from pmarch import scenes
+
+    scene = scenes.dam_break_aniso(0)
+    cam = scene.cameras[0]
+    rays = cam.width * cam.height
+    m = pmarch.Marcher(0)
+    fi = m.set_frame(0, scene.xyz, scene.h)
+    st_in = scenes.aniso_settings(scene, skip_mode=1)
+    wall, kms, st = run_gpu(m, 0, st_in, cam, args.steps, args.warmup)
+    B = alg_bytes_aniso(st)
+    achieved = B / (kms * 1e-3) / 1e9
+    out = {
+        "metric": "Mrays/s @1080p/1M particles, plain vs learned-step march, at fixed depth error",
+        "value": round(args.steps * rays / wall / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "anisotropic march (WPCA over 2h, anisotropic kernel; k_n 0.5, k_r 2, k_s 2000, "
+                               "N_eps 1) on dam_break 1,024,000 particles scaled to h = 0.1, 1920x1080 view, "
+                               "configured march (AABB entry, step h/8, 8 bisections with the last in FP64, "
+                               "central-difference normals, empty-cell skip), IsoDensity 1.0",
+                   "particles": scene.n, "width": cam.width, "height": cam.height, "h": scene.h,
+                   "step": scene.step, "iso": 1.0, "path": "anisotropic"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "kernel_ms": round(kms, 4),
+        "alg_bytes_per_launch": B,
+        "march_counters": {k: st[k] for k in ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples")},
+        "grid_build_ms": round(float(fi.build_ms), 3),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_aniso(scene, cam, st_in, args.cpu_budget)
+    m.close()
+    print(json.dumps(out))
+
+
 def bench_batch(args):
     """64 views x 16 frames in one persistent launch from a hipGraph."""
     import pmarch
@@ -230,7 +322,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["dam_break_1080p", "clustered_4k", "batch"], default="dam_break_1080p")
+    ap.add_argument("--workload", choices=["dam_break_1080p", "clustered_4k", "batch", "aniso_dam_1080p"],
+                    default="dam_break_1080p")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
@@ -241,6 +334,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "aniso_dam_1080p":
+        bench_aniso(args)
         return
 
     import pmarch

@@ -3,7 +3,8 @@
 // images.  Input and output are little-endian binary files written/read by
 // tests/test_gpu_facade.py.
 //   in : "PMFD" u32 W H n, f32 h, f32 invPV[16], f32 cam[3], i32 max_steps,
-//        f32 step, f32 iso, f32 depth[W*H], f32 xyz[3n]
+//        f32 step, f32 iso, i32 anisotropy, f32 k_n k_r k_s, i32 N_eps,
+//        f32 h_ext, f32 depth[W*H], f32 xyz[3n]
 //   out: f32 positions[4*W*H], f32 normals[4*W*H], i32 polls
 #include <cstdio>
 #include <cstring>
@@ -22,10 +23,11 @@ int main(int argc, char** argv)
     if (!f) return 2;
     char magic[4];
     uint32_t W, H, n;
-    float h, inv[16], cam[3], step, iso;
-    int32_t max_steps;
+    float h, inv[16], cam[3], step, iso, kn, kr, ks, hext;
+    int32_t max_steps, aniso, neps;
     bool ok = rd(f, magic, 4) && !std::memcmp(magic, "PMFD", 4) && rd(f, &W, 1) && rd(f, &H, 1) && rd(f, &n, 1) &&
-              rd(f, &h, 1) && rd(f, inv, 16) && rd(f, cam, 3) && rd(f, &max_steps, 1) && rd(f, &step, 1) && rd(f, &iso, 1);
+              rd(f, &h, 1) && rd(f, inv, 16) && rd(f, cam, 3) && rd(f, &max_steps, 1) && rd(f, &step, 1) && rd(f, &iso, 1) &&
+              rd(f, &aniso, 1) && rd(f, &kn, 1) && rd(f, &kr, 1) && rd(f, &ks, 1) && rd(f, &neps, 1) && rd(f, &hext, 1);
     std::vector<float> depth((size_t)W * H), xyz(3 * (size_t)n);
     ok = ok && rd(f, depth.data(), depth.size()) && rd(f, xyz.data(), xyz.size());
     std::fclose(f);
@@ -33,6 +35,7 @@ int main(int argc, char** argv)
 
     Dataset ds;
     ds.ParticleRadius = h;
+    ds.ParticleRadiusExt = hext;
     ds.Frames.resize(1);
     ds.Frames[0].m_Particles.resize(n);
     for (uint32_t i = 0; i < n; i++)
@@ -50,7 +53,11 @@ int main(int argc, char** argv)
     vs.MaxSteps = max_steps;
     vs.StepSize = step;
     vs.IsoDensity = iso;
-    vs.EnableAnisotropy = false;
+    vs.EnableAnisotropy = aniso != 0;
+    vs.k_n = kn;
+    vs.k_r = kr;
+    vs.k_s = ks;
+    vs.N_eps = neps;
 
     std::vector<glm::vec4> pos((size_t)W * H), nrm((size_t)W * H);
     RayMarcher rm;

@@ -47,6 +47,7 @@ public:
 class Dataset {
 public:
     float ParticleRadius = 0.1f;
+    float ParticleRadiusExt = 0.2f;     // particleRadiusMultiplier * ParticleRadius
     std::vector<Frame> Frames;
 };
 
