@@ -14,9 +14,11 @@
 //      AnisotropicKernel::W(G, det G, r) (Kernel.cpp:63-83), walked over the
 //      cells within one cell of q (the same particles in the same order; the
 //      extended walk when the list was capped).
-// One ray per lane, one 8x8 tile per wave; every lane runs its ray to the end
-// (march, bisection, FP64 step, normal, outputs).  All float arithmetic is in
-// the order of the CPU oracle (oracle/pm_aniso.inc), -ffp-contract=off.
+// One ray per lane, one 8x8 tile per wave, each lane to the end of its ray
+// (march, bisection, FP64 step, normal, outputs); variant 2 of the configured
+// march runs persistent waves with ray refill and a finish pass over the hits
+// instead (below).  All float arithmetic is in the order of the CPU oracle
+// (oracle/pm_aniso.inc), -ffp-contract=off.
 #include "pm_internal.h"
 #include "pm_ray.h"
 
@@ -76,6 +78,43 @@ __device__ __forceinline__ bool an_row(const GridDev &g, const SWalk &w, int R, 
     b = g.cell_start[row + (uint32_t)xl];
     e = g.cell_start[row + (uint32_t)xh + 1];
     return true;
+}
+
+// Row r of the (2R+1)^2 rows in walk order (dz = r / (2R+1) - R outer, dy
+// inner): its stored range, empty when the row is dropped.
+template <int R>
+__device__ __forceinline__ void an_row_at(const GridDev &g, const SWalk &w, float lim, int r, uint32_t &b,
+                                          uint32_t &e)
+{
+    const int dz = r / (2 * R + 1) - R, dy = r - (2 * R + 1) * (r / (2 * R + 1)) - R;
+    if (!an_row(g, w, R, lim, dy, dz, b, e)) b = e = 0u;
+}
+
+// Every candidate of the walk, in order, through f(float4): the next row's
+// bounds are loaded while a row is read, and a row's candidates are loaded 4
+// at a time (the last group re-reads the row's first candidate in its unused
+// slots), so each lane keeps several gathers in flight.  Returns the number
+// of candidates.
+template <int R, class F>
+__device__ __forceinline__ uint32_t an_walk(const GridDev &g, const SWalk &w, float lim, F &&f)
+{
+    constexpr int NR = (2 * R + 1) * (2 * R + 1);
+    uint32_t cnd = 0, nb, ne;
+    an_row_at<R>(g, w, lim, 0, nb, ne);
+    _Pragma("unroll 1") for (int r = 0; r < NR; r++) {
+        const uint32_t b = nb, e = ne;
+        if (r + 1 < NR) an_row_at<R>(g, w, lim, r + 1, nb, ne);
+        cnd += e - b;
+        for (uint32_t i = b; i < e; i += 4) {
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = g.sorted[i + u < e ? i + u : i];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (i + u < e) f(v[u]);
+        }
+    }
+    return cnd;
 }
 
 // ---- WPCA ----
@@ -169,58 +208,44 @@ __device__ __forceinline__ uint32_t an_cov(const GridDev &g, const P &a, const f
     using T = typename P::T;
     SWalk w;
     if (!an_centre(g, qf, 2, w)) return 0;
-    uint32_t nx = 0, cnd = 0;
+    uint32_t nx = 0;
     T ws = (T)0, m0 = (T)0, m1 = (T)0, m2 = (T)0;
-    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++)
-        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
-            uint32_t b, e;
-            if (!an_row(g, w, 2, kLimExt, dy, dz, b, e)) continue;
-            cnd += e - b;
-            for (uint32_t i = b; i < e; i++) {
-                const float4 p = g.sorted[i];
-                const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
-                const T r2 = (rx * rx + ry * ry) + rz * rz;
-                if (r2 < a.R2x) {
-                    if (nx < kMaxNeighbors) {
-                        const T wt = an_wcubic(r2, a);
-                        ws += wt;
-                        m0 += wt * (T)p.x;
-                        m1 += wt * (T)p.y;
-                        m2 += wt * (T)p.z;
-                    }
-                    nx++;
-                }
+    uint32_t cnd = an_walk<2>(g, w, kLimExt, [&](const float4 p) {
+        const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
+        const T r2 = (rx * rx + ry * ry) + rz * rz;
+        if (r2 < a.R2x) {
+            if (nx < kMaxNeighbors) {
+                const T wt = an_wcubic(r2, a);
+                ws += wt;
+                m0 += wt * (T)p.x;
+                m1 += wt * (T)p.y;
+                m2 += wt * (T)p.z;
             }
+            nx++;
         }
+    });
     if (nx == 0) { cand += cnd; return 0; }
     const uint32_t N = nx < kMaxNeighbors ? nx : kMaxNeighbors;
     const T inv = (T)1 / ws;
     m0 = m0 * inv; m1 = m1 * inv; m2 = m2 * inv;
     T c0 = (T)0, c1 = (T)0, c2 = (T)0, c3 = (T)0, c4 = (T)0, c5 = (T)0;
     uint32_t n2 = 0;
-    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++)
-        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
-            uint32_t b, e;
-            if (!an_row(g, w, 2, kLimExt, dy, dz, b, e)) continue;
-            cnd += e - b;
-            for (uint32_t i = b; i < e; i++) {
-                const float4 p = g.sorted[i];
-                const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
-                const T r2 = (rx * rx + ry * ry) + rz * rz;
-                if (r2 < a.R2x && n2 < N) {
-                    const T wt = an_wcubic(r2, a);
-                    const T x0 = (T)p.x - m0, x1 = (T)p.y - m1, x2 = (T)p.z - m2;
-                    const T w0 = wt * x0, w1 = wt * x1, w2 = wt * x2;
-                    c0 += w0 * x0;
-                    c1 += w1 * x0;
-                    c2 += w1 * x1;
-                    c3 += w2 * x0;
-                    c4 += w2 * x1;
-                    c5 += w2 * x2;
-                    n2++;
-                }
-            }
+    cnd += an_walk<2>(g, w, kLimExt, [&](const float4 p) {
+        const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
+        const T r2 = (rx * rx + ry * ry) + rz * rz;
+        if (r2 < a.R2x && n2 < N) {
+            const T wt = an_wcubic(r2, a);
+            const T x0 = (T)p.x - m0, x1 = (T)p.y - m1, x2 = (T)p.z - m2;
+            const T w0 = wt * x0, w1 = wt * x1, w2 = wt * x2;
+            c0 += w0 * x0;
+            c1 += w1 * x0;
+            c2 += w1 * x1;
+            c3 += w2 * x0;
+            c4 += w2 * x1;
+            c5 += w2 * x2;
+            n2++;
         }
+    });
     cand += cnd;
     c[0] = c0 * inv; c[1] = c1 * inv; c[2] = c2 * inv;
     c[3] = c3 * inv; c[4] = c4 * inv; c[5] = c5 * inv;
@@ -298,39 +323,36 @@ __device__ __forceinline__ typename P::T an_sum(const GridDev &g, const P &a, co
     if (nx == 0) return (T)0;
     an_gmat(c, nx < kMaxNeighbors ? nx : kMaxNeighbors, a, G, detG);
     const bool capped = nx > kMaxNeighbors;
-    const int R = capped ? 2 : 1;
-    const float lim = capped ? kLimExt : kLimNear;
     SWalk w;
-    an_centre(g, qf, R, w);
     T s = (T)0, cc = (T)0;
-    uint32_t n = 0, cnd = 0;
-    _Pragma("unroll 1") for (int dz = -R; dz <= R; dz++)
-        _Pragma("unroll 1") for (int dy = -R; dy <= R; dy++) {
-            uint32_t b, e;
-            if (!an_row(g, w, R, lim, dy, dz, b, e)) continue;
-            cnd += e - b;
-            for (uint32_t i = b; i < e; i++) {
-                const float4 p = g.sorted[i];
-                const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
-                const T r2 = (rx * rx + ry * ry) + rz * rz;
-                if (capped) {
-                    if (!(r2 < a.R2x)) continue;
-                    if (n++ >= kMaxNeighbors) continue;
-                }
-                if (!(r2 < a.h2)) continue;
-                if constexpr (sizeof(T) == 4) {
-                    if (grad) {
-                        float o0, o1, o2;
-                        an_gradW(G, detG, rx, ry, rz, a, o0, o1, o2);
-                        grad[0] += o0; grad[1] += o1; grad[2] += o2;
-                        continue;
-                    }
-                }
-                const T wv = an_W(G, detG, rx, ry, rz, a);
-                if (comp) an_neu_add(s, cc, wv);
-                else s += wv;
+    uint32_t n = 0, cnd;
+    auto term = [&](const float4 p) {
+        const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
+        const T r2 = (rx * rx + ry * ry) + rz * rz;
+        if (capped) {
+            if (!(r2 < a.R2x)) return;
+            if (n++ >= kMaxNeighbors) return;
+        }
+        if (!(r2 < a.h2)) return;
+        if constexpr (sizeof(T) == 4) {
+            if (grad) {
+                float o0, o1, o2;
+                an_gradW(G, detG, rx, ry, rz, a, o0, o1, o2);
+                grad[0] += o0; grad[1] += o1; grad[2] += o2;
+                return;
             }
         }
+        const T wv = an_W(G, detG, rx, ry, rz, a);
+        if (comp) an_neu_add(s, cc, wv);
+        else s += wv;
+    };
+    if (capped) {
+        an_centre(g, qf, 2, w);
+        cnd = an_walk<2>(g, w, kLimExt, term);
+    } else {
+        an_centre(g, qf, 1, w);
+        cnd = an_walk<1>(g, w, kLimNear, term);
+    }
     cand += cnd;
     return comp ? s + cc : s;
 }
@@ -407,6 +429,7 @@ __device__ __forceinline__ void an_pixel_config(const MarchLaunch &L, uint32_t p
     }
     cn.in_aabb++;
     r.d0 = d[0]; r.d1 = d[1]; r.d2 = d[2];
+    r.id0 = 1.0f / d[0]; r.id1 = 1.0f / d[1]; r.id2 = 1.0f / d[2];
     r.k = 0;
     const float step = s.step;
     int hitk = -1;
@@ -415,8 +438,9 @@ __device__ __forceinline__ void an_pixel_config(const MarchLaunch &L, uint32_t p
         const float t = r.tn + (float)r.k * step;
         if (t > r.tf) break;
         const float x = o[0] + t * d[0], y = o[1] + t * d[1], z = o[2] + t * d[2];
-        if (s.skip_mode == PM_SKIP_EMPTY) {
-            const int kn = pm_skip_empty(g, o[0], o[1], o[2], r, x, y, z, step);
+        if (s.skip_mode == PM_SKIP_EMPTY || s.skip_mode == PM_SKIP_BOUND) {
+            const int kn = s.skip_mode == PM_SKIP_EMPTY ? pm_skip_empty(g, o[0], o[1], o[2], r, x, y, z, step)
+                                                        : pm_skip_bound(g, o[0], o[1], o[2], r, x, y, z, L.inv_step);
             if (kn != r.k) { r.k = kn; continue; }
         }
         uint32_t cand = 0;
@@ -534,10 +558,261 @@ __global__ __launch_bounds__(256) void k_march_aniso(MarchLaunch L)
     flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
 }
 
+// ---- configured march: persistent waves, then a finish pass over the hits ----
+//
+// The march kernel keeps every lane on the same work: each loop iteration
+// evaluates one FP32 density per busy lane (march sample or bisection step);
+// a lane whose ray ends takes the next ray of the wave's tile, and the wave
+// takes the next tile from one of 8 per-XCD band counters (stealing from the
+// other bands at the end).  A ray's march ends at its hit: (1 | fp64 << 1,
+// lo, hi, pixel) -- t itself, or the bracket of the FP64 bisection step -- is
+// staged in LDS and appended to a hit list 64 entries at a time (one atomic
+// per 64 hits).  k_finish_aniso then takes the list 64 hits per wave: the FP64
+// step, the six central-difference samples (or sum gradW) and the outputs.
+
+static constexpr uint32_t kListCtr = 16;   // band_ctr[16]: hit-list length
+
+__device__ __forceinline__ uint32_t an_band_lo(uint32_t T, uint32_t b) { return (uint32_t)(((uint64_t)T * b) >> 3); }
+
+__global__ __launch_bounds__(256) void k_march_aniso_persist(MarchLaunch L, uint32_t *__restrict__ band_ctr,
+                                                             uint4 *__restrict__ list)
+{
+    __shared__ uint4 stage[4][128];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
+    const bool comp = s.compensated != 0;
+    const bool fp64 = s.final_fp64 && s.bisect_iters > 0;
+    const int n32 = fp64 ? s.bisect_iters - 1 : s.bisect_iters;
+    const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
+    const uint32_t tiles_x = (L.W + 7) / 8, tiles_y = (L.H + 7) / 8, T = tiles_x * tiles_y;
+    const uint32_t xcd = blockIdx.x & 7u;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+
+    Counts cn;
+    RayState r{};
+    r.phase = PH_DONE;
+    uint32_t tile = 0, steal = 0;
+    int qpos = 64, qn = 0;
+    bool exhausted = false;
+
+    while (true) {
+        // refill idle lanes from the wave's tile
+        while (!exhausted) {
+            const unsigned long long needy = __ballot(r.phase == PH_DONE);
+            if (needy == 0) break;
+            if (qpos >= 64) {
+                uint32_t t = 0xffffffffu;
+                if (lane == 0) {
+                    while (steal < 8) {
+                        const uint32_t b = (xcd + steal) & 7u;
+                        const uint32_t lo = an_band_lo(T, b), hi = an_band_lo(T, b + 1);
+                        const uint32_t v = atomicAdd(&band_ctr[b], 1u);
+                        if (v < hi - lo) { t = lo + v; break; }
+                        steal++;
+                    }
+                }
+                t = __shfl(t, 0, 64);
+                steal = __shfl(steal, 0, 64);
+                if (t == 0xffffffffu) { exhausted = true; break; }
+                tile = t;
+                qpos = 0;
+            }
+            const int rank = __popcll(needy & lt_mask), take = __popcll(needy);
+            if (r.phase == PH_DONE && qpos + rank < 64) {
+                const uint32_t j = (uint32_t)(qpos + rank);
+                const uint32_t px = (tile % tiles_x) * 8 + (j & 7u), py = (tile / tiles_x) * 8 + (j >> 3);
+                if (px < L.W && py < L.H) {
+                    cn.rays++;
+                    r.idx = py * L.W + px;
+                    float d[3];
+                    pm_ray_dir(L, px, py, d);
+                    r.d0 = d[0]; r.d1 = d[1]; r.d2 = d[2];
+                    r.id0 = 1.0f / d[0]; r.id1 = 1.0f / d[1]; r.id2 = 1.0f / d[2];
+                    const float o[3] = { o0, o1, o2 };
+                    if (pm_slab(o, d, g.bmin, g.bmax, r.tn, r.tf)) {
+                        r.phase = PH_MARCH;
+                        r.k = 0;
+                        cn.in_aabb++;
+                    } else {
+                        write_miss(L, r.idx);
+                    }
+                }
+            }
+            qpos = qpos + take < 64 ? qpos + take : 64;
+        }
+        if (__ballot(r.phase != PH_DONE) == 0) break;
+
+        // this lane's next sample: march (after at most 8 empty-cell skips) or bisection
+        bool want = false;
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+        if (r.phase == PH_MARCH) {
+            for (int hop = 0;; hop++) {
+                const float t = r.tn + (float)r.k * s.step;
+                if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) {
+                    write_miss(L, r.idx);
+                    r.phase = PH_DONE;
+                    break;
+                }
+                qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
+                int kn;
+                if (s.skip_mode == PM_SKIP_BOUND) kn = pm_skip_bound(g, o0, o1, o2, r, qx, qy, qz, L.inv_step);
+                else if (s.skip_mode == PM_SKIP_EMPTY) kn = pm_skip_empty(g, o0, o1, o2, r, qx, qy, qz, s.step);
+                else { want = true; break; }
+                if (kn == r.k) { want = true; break; }
+                r.k = kn;
+                if (hop == 7) break;
+            }
+        } else if (r.phase == PH_BISECT) {
+            const float tm = (r.lo + r.hi) * 0.5f;
+            qx = o0 + tm * r.d0; qy = o1 + tm * r.d1; qz = o2 + tm * r.d2;
+            want = true;
+        }
+        float rho = 0.0f;
+        uint32_t cand = 0;
+        if (want) rho = an_density32(L, qx, qy, qz, comp, cand);
+
+        bool emit = false;
+        uint4 rec = make_uint4(0u, 0u, 0u, 0u);
+        if (want) {
+            cn.samples++;
+            cn.cand += cand;
+            if (r.phase == PH_MARCH) {
+                cn.march++;
+                if (rho >= s.iso) {
+                    cn.hits++;
+                    if (r.k == 0) {
+                        emit = true;
+                        rec = make_uint4(1u, __float_as_uint(r.tn), __float_as_uint(r.tn), r.idx);
+                    } else {
+                        r.lo = r.tn + (float)(r.k - 1) * s.step;
+                        r.hi = r.tn + (float)r.k * s.step;
+                        r.it = 0;
+                        if (n32 > 0) {
+                            r.phase = PH_BISECT;
+                        } else {
+                            emit = true;
+                            const float tt = (r.lo + r.hi) * 0.5f;
+                            rec = fp64 ? make_uint4(3u, __float_as_uint(r.lo), __float_as_uint(r.hi), r.idx)
+                                       : make_uint4(1u, __float_as_uint(tt), __float_as_uint(tt), r.idx);
+                        }
+                    }
+                } else {
+                    r.k++;
+                }
+            } else {
+                const float tm = (r.lo + r.hi) * 0.5f;
+                if (rho >= s.iso) r.hi = tm; else r.lo = tm;
+                r.it++;
+                if (r.it >= n32) {
+                    emit = true;
+                    const float tt = (r.lo + r.hi) * 0.5f;
+                    rec = fp64 ? make_uint4(3u, __float_as_uint(r.lo), __float_as_uint(r.hi), r.idx)
+                               : make_uint4(1u, __float_as_uint(tt), __float_as_uint(tt), r.idx);
+                }
+            }
+            if (emit) r.phase = PH_DONE;
+        }
+
+        // stage the hits; append 64 of them to the list at a time
+        const unsigned long long em = __ballot(emit);
+        if (em) {
+            if (emit) stage[wave][qn + __popcll(em & lt_mask)] = rec;
+            qn += __popcll(em);
+            if (qn >= 64) {
+                wave_lds_sync();
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&band_ctr[kListCtr], 64u);
+                base = __shfl(base, 0, 64);
+                list[base + lane] = stage[wave][lane];
+                const int rem = qn - 64;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if ((int)lane < rem) v = stage[wave][64 + lane];
+                wave_lds_sync();
+                if ((int)lane < rem) stage[wave][lane] = v;
+                qn = rem;
+            }
+        }
+    }
+    if (qn > 0) {
+        wave_lds_sync();
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&band_ctr[kListCtr], (uint32_t)qn);
+        base = __shfl(base, 0, 64);
+        if ((int)lane < qn) list[base + lane] = stage[wave][lane];
+    }
+    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+}
+
+// One hit of the list: the FP64 bisection step (flag 2), the normal and the outputs.
+__device__ __forceinline__ void an_finish_hit(const MarchLaunch &L, uint4 rec, Counts &cn)
+{
+    const pm_settings &s = L.s;
+    const uint32_t idx = rec.w;
+    const uint32_t px = idx % L.W, py = idx / L.W;
+    const float lo = __uint_as_float(rec.y), hi = __uint_as_float(rec.z);
+    float d[3];
+    pm_ray_dir(L, px, py, d);
+    const float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
+    float xs[3], th;
+    if (rec.x & 2u) {
+        double dlo = lo, dhi = hi;
+        const double tm = (dlo + dhi) * 0.5;
+        const double rho = an_density64(L.g, L.an.d, (double)o[0] + tm * (double)d[0],
+                                        (double)o[1] + tm * (double)d[1], (double)o[2] + tm * (double)d[2]);
+        cn.samples++;
+        if (rho >= (double)s.iso) dhi = tm; else dlo = tm;
+        const double tt = (dlo + dhi) * 0.5;
+        xs[0] = (float)((double)o[0] + tt * (double)d[0]);
+        xs[1] = (float)((double)o[1] + tt * (double)d[1]);
+        xs[2] = (float)((double)o[2] + tt * (double)d[2]);
+        th = (float)tt;
+    } else {
+        th = lo;
+        xs[0] = o[0] + th * d[0]; xs[1] = o[1] + th * d[1]; xs[2] = o[2] + th * d[2];
+    }
+    float nx, ny, nz;
+    an_normal(L, xs, cn, nx, ny, nz);
+    L.pos[idx] = make_float4(xs[0], xs[1], xs[2], 1.0f);
+    L.nrm[idx] = make_float4(nx, ny, nz, 1.0f);
+    if (L.t) L.t[idx] = th;
+}
+
+__global__ __launch_bounds__(256) void k_finish_aniso(MarchLaunch L, const uint32_t *__restrict__ band_ctr,
+                                                      const uint4 *__restrict__ list)
+{
+    const uint32_t n = band_ctr[kListCtr];
+    Counts cn;
+    for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += gridDim.x * 256u) {
+        const uint32_t i = base + (threadIdx.x & 63u);
+        if (i < n) an_finish_hit(L, list[i], cn);
+    }
+    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+}
+
+__global__ void k_zero_band(uint32_t *p)
+{
+    if (threadIdx.x < 32) p[threadIdx.x] = 0u;
+}
+
 hipError_t launch_march_aniso(const MarchLaunch &L, hipStream_t st)
 {
     const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
-    hipLaunchKernelGGL(k_march_aniso, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
+    // one ray per lane to its end (the default: neighbouring rays of a tile
+    // sample neighbouring points, so the wave's gathers share cache lines), or
+    // (variant 2) persistent waves with ray refill and the hit-list finish
+    if (L.s.mode == PM_MODE_REFERENCE || L.s.variant != 2 || !L.hit_rec) {
+        hipLaunchKernelGGL(k_march_aniso, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_zero_band, dim3(1), dim3(64), 0, st, L.band_ctr);
+    uint32_t blocks = 1024, need = (tiles + 3) / 4;
+    if (blocks > need) blocks = (need + 7) & ~7u;
+    hipLaunchKernelGGL(k_march_aniso_persist, dim3(blocks), dim3(256), 0, st, L, L.band_ctr, L.hit_rec);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finish_aniso, dim3(2048), dim3(256), 0, st, L, (const uint32_t *)L.band_ctr,
+                       (const uint4 *)L.hit_rec);
     return hipGetLastError();
 }
 

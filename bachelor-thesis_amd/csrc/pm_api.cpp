@@ -465,7 +465,6 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
         if (s->ext_mult != 0.0f && !(s->ext_mult >= 1.0f && s->ext_mult <= 16.0f))
             return fail(c, PM_ERR_ARG, "anisotropic: ext_mult must be 0 (= 2) or in [1, 16]");
         if (s->learned) return fail(c, PM_ERR_ARG, "anisotropic: no learned step");
-        if (s->skip_mode == PM_SKIP_BOUND) return fail(c, PM_ERR_ARG, "anisotropic: the bound skip holds for the isotropic kernel only");
     }
     if (s->mode == PM_MODE_CONFIG) {
         double diag = 0.0;
@@ -514,7 +513,9 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     HIPCHK(c, hipSetDevice(c->device));
 
     if (s->skip_mode == PM_SKIP_BOUND) {
-        rc = ensure_bound(c, f, s->iso);
+        // the anisotropic density is exactly 0 where no particle lies within h:
+        // the bound bits for PM_ZERO_ISO mark those sub-boxes
+        rc = ensure_bound(c, f, s->anisotropic ? PM_ZERO_ISO : s->iso);
         if (rc) return rc;
     }
     if (s->skip_mode == PM_SKIP_DG_BOX || s->skip_mode == PM_SKIP_DG_CONS) {
@@ -540,7 +541,7 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     L.t = (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
     L.band_ctr = (uint32_t *)c->d_band.p;
-    if (s->mode == PM_MODE_CONFIG && !s->anisotropic) {
+    if (s->mode == PM_MODE_CONFIG) {   // hit records; the anisotropic march's hit list
         HIPCHK(c, c->d_hits.ensure(sizeof(uint4) * npx));
         L.hit_rec = (uint4 *)c->d_hits.p;
     }

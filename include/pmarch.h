@@ -52,7 +52,10 @@ extern "C" {
 #define PM_SKIP_BOUND      4  /* configured mode: skip 4x4x4 sub-boxes of a cell whose
                                  density bound (sum of W at each particle's distance to
                                  the sub-box) stays below iso; the image is that of the
-                                 march without the skip                                */
+                                 march without the skip.  With the anisotropic density:
+                                 sub-boxes with no particle within h (the bound for
+                                 iso PM_ZERO_ISO), where that density is exactly 0     */
+#define PM_ZERO_ISO  7.88860905e-31f  /* 2^-100: a bound below it means no particle within h */
 
 /* March settings.  The first four fields carry VisualizationSettings'
  * MaxSteps / StepSize / IsoDensity (RayMarcher.h:16-18), the anisotropic
@@ -84,7 +87,7 @@ typedef struct pm_settings {
     /* PerPixel_Anisotropic (RayMarcher.cpp:346-423): per sample, WPCA over the
        particles within ext_mult * h (RayMarcher.cpp:114-254), then the
        anisotropic kernel (Kernel.cpp:63-107) over those within h.  Either
-       mode; not with the learned step, the bound skip or the batch.          */
+       mode; not with the learned step or the batch.                          */
     int32_t anisotropic;   /* EnableAnisotropy                                        */
     float   k_n;           /* k_n: Sigma for N <= N_eps (> 0)                         */
     float   k_r;           /* k_r: eigenvalues clamped to max / k_r (> 0)             */

@@ -19,6 +19,8 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 RESTING_LATTICE_SUM = 3.1415061532636885
+ZERO_ISO = float(np.float32(2.0 ** -100))
+"""A sub-box bound below this means no particle within h of the sub-box."""
 """sum_j W(x_j)/sigma over a perfect lattice of spacing h/2 (27 non-zero terms)."""
 
 
@@ -421,7 +423,8 @@ def march(grid: Grid, settings: Settings, inv_pv, cam, W: int, H: int, depth=Non
         depth = np.ascontiguousarray(depth, np.float32)
         dptr = depth.ctypes.data
     if settings.skip_mode == 4:
-        grid.bound(settings.iso)
+        # the anisotropic density skips the sub-boxes with no particle within h
+        grid.bound(ZERO_ISO if settings.anisotropic else settings.iso)
     cnt = Counters()
     rc = lib().or_march(C.byref(grid.g), C.byref(settings), inv_pv, cam, W, H, dptr, row_step, threads,
                         pos.reshape(-1), nrm.reshape(-1), t.ctypes.data if want_t else None, C.byref(cnt))

@@ -26,6 +26,7 @@
 
 #define OR_PI_F 3.14159265358979323846f
 #define OR_PI_D 3.14159265358979323846
+#define OR_ZERO_ISO 0x1p-100f   /* bound below it: no particle within h of the sub-box */
 
 /* ------------------------------------------------------------------------- */
 /* SPH cubic spline kernel -- Kernel.cpp:8-14 (constants), :16-32 (W),        */
@@ -1380,9 +1381,11 @@ static int run_march(const or_grid *g, const or_hash *hs, const or_settings *s, 
     if (!g || !s || !M || !cam || !pos || !nrm || W == 0 || H == 0) return -1;
     if (s->mode == OR_MODE_REFERENCE && !depth) return -1;
     if (!(s->step > 0.0f)) return -1;
-    if (s->skip_mode == 4 && (!g->bound || g->bound_iso != s->iso)) return -2;   /* or_bound_build first */
-    /* the sub-box bound holds for the isotropic kernel only */
-    if (s->anisotropic && (s->skip_mode == 4 || !(s->ext_mult >= 1.0f))) return -1;
+    /* or_bound_build first: for iso, or (anisotropic density) for OR_ZERO_ISO --
+     * sub-boxes with no particle within h, where that density is exactly 0 */
+    const float biso = s->anisotropic ? OR_ZERO_ISO : s->iso;
+    if (s->skip_mode == 4 && (!g->bound || g->bound_iso != biso)) return -2;
+    if (s->anisotropic && !(s->ext_mult >= 1.0f)) return -1;
     if (row_step == 0) row_step = 1;
     march_job j;
     memset(&j, 0, sizeof(j));

@@ -87,7 +87,7 @@ def _march_pair(m, frame, sc, st, cam, row_step, depth=None):
     return pos, stats
 
 
-@pytest.mark.parametrize("cam_i,normal_mode,skip", [(0, 0, 0), (1, 0, 1), (0, 1, 1)])
+@pytest.mark.parametrize("cam_i,normal_mode,skip", [(0, 0, 0), (1, 0, 1), (0, 1, 1), (0, 0, 4)])
 def test_march_aniso_config_bitexact(marcher, cam_i, normal_mode, skip):
     sc = scenes.lattice_cube(0)
     cam = sc.cameras[cam_i]
@@ -98,15 +98,33 @@ def test_march_aniso_config_bitexact(marcher, cam_i, normal_mode, skip):
     assert stats["hits"] > 50000
 
 
+def test_march_aniso_schedules_agree(marcher):
+    """One ray per lane to its end (variant 0) and the persistent march +
+    hit-list finish (variant 2) give the same images and counters."""
+    sc = scenes.lattice_cube(1)
+    cam = sc.cameras[0]
+    print("aniso config march lattice_cube variants 0/2", flush=True)
+    marcher.set_frame(0, sc.xyz, sc.h)
+    out = []
+    for v in (0, 2):
+        st = scenes.aniso_settings(sc, skip_mode=1, variant=v)
+        out.append(marcher.march(0, st, cam.inv_pv, cam.eye, cam.width, cam.height))
+    for a, b in zip(out[0][:3], out[1][:3]):
+        assert np.array_equal(a, b)
+    for k in ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples"):
+        assert out[0][3][k] == out[1][3][k], k
+
+
 def test_march_aniso_dam_bitexact(marcher):
     """The dam-break column scaled to h = 0.1, 480x270, whole image."""
     sc = scenes.dam_break_aniso(0, width=480, height=270)
     cam = sc.cameras[0]
     print("aniso config march dam_break_aniso 480x270", flush=True)
     marcher.set_frame(3, sc.xyz, sc.h)
-    st = scenes.aniso_settings(sc, skip_mode=1)
-    pos, stats = _march_pair(marcher, 3, sc, st, cam, 1)
-    assert stats["hits"] > 10000
+    for skip, variant in ((1, 0), (4, 0), (4, 2)):
+        st = scenes.aniso_settings(sc, skip_mode=skip, variant=variant)
+        pos, stats = _march_pair(marcher, 3, sc, st, cam, 1)
+        assert stats["hits"] > 10000
 
 
 def test_march_aniso_odd_size_and_misses(marcher):
@@ -137,7 +155,7 @@ def test_aniso_settings_rejected(marcher):
     sc = scenes.lattice_cube(0)
     marcher.set_frame(0, sc.xyz, sc.h)
     cam = sc.cameras[0]
-    for bad in (dict(skip_mode=4), dict(k_s=0.0), dict(k_r=-1.0), dict(ext_mult=0.5)):
+    for bad in (dict(learned=1), dict(k_s=0.0), dict(k_r=-1.0), dict(ext_mult=0.5)):
         st = scenes.aniso_settings(sc, **bad)
         with pytest.raises(Exception):
             marcher.march(0, st, cam.inv_pv, cam.eye, 64, 48)
