@@ -47,7 +47,8 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "bachelor-thesis_amd"))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC = ROOT / "profiles" / "march_dam_break_1080p_traffic.json"
+TRAFFIC = [ROOT / "profiles" / "march_dam_break_1080p_traffic.json",
+           ROOT / "profiles" / "march_aniso_dam_break_1080p_traffic.json"]
 
 
 def cpu_model() -> str:
@@ -141,10 +142,13 @@ def cpu_baseline(scene, cam, settings, budget_s: float, xyz=None):
 
 
 def traffic_for(path: str, workload: str):
-    if TRAFFIC.exists():
-        tr = json.loads(TRAFFIC.read_text())
-        if tr.get("path") == path and tr.get("workload", "dam_break_1080p") == workload:
-            return tr["hbm_bytes_per_launch"]
+    """HBM bytes per launch of the profiled form of a workload (FETCH_SIZE +
+    WRITE_SIZE of all its kernels, scaled against the copy kernel)."""
+    for f in TRAFFIC:
+        if f.exists():
+            tr = json.loads(f.read_text())
+            if tr.get("path") == path and tr.get("workload", "dam_break_1080p") == workload:
+                return tr["hbm_bytes_per_launch"]
     return None
 
 
@@ -222,11 +226,12 @@ def bench_aniso(args):
                    "particles": scene.n, "width": cam.width, "height": cam.height, "h": scene.h,
                    "step": scene.step, "iso": 1.0, "path": "anisotropic"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for("anisotropic", args.workload)},
         "kernel_ms": round(kms, 4),
         "alg_bytes_per_launch": B,
         "march_counters": {k: st[k] for k in ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples")},
         "grid_build_ms": round(float(fi.build_ms), 3),
+        "copy_gbps": round(m.stream_copy_gbps(1 << 28, 10), 1),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_aniso(scene, cam, st_in, args.cpu_budget)
