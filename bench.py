@@ -169,6 +169,8 @@ def cpu_baseline_aniso(scene, cam, settings, budget_s: float):
     threads = max(1, min(16, os.cpu_count() or 2) - 1)
     G = O.Grid(scene.xyz, scene.h)
     st = O.make_settings(**{k: v for k, v in settings.items() if k in dict(O.Settings._fields_)})
+    if st.skip_mode == 4:
+        G.bound(O.ZERO_ISO)        # the skip's bits, built once outside the timed runs
     W, H = cam.width, cam.height
     probe = 64
     t0 = time.perf_counter()
