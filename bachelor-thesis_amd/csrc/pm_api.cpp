@@ -66,6 +66,9 @@ struct Frame {
 struct pm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream_build = nullptr;   // frame builds of pm_set_frame_stream
+    int32_t pending_frame = 0;            // frame of the march in flight
+    bool pending_batch = false;           // the march in flight is a batch (all its frames)
     hipEvent_t ev_launch = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
     std::map<int32_t, Frame *> frames;
     std::string err;
@@ -163,6 +166,7 @@ int pm_create(int device, pm_ctx **out)
     pm_ctx *c = new pm_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream_build, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev_launch) != hipSuccess || hipEventCreate(&c->ev_k0) != hipSuccess ||
         hipEventCreate(&c->ev_k1) != hipSuccess || hipEventCreate(&c->ev_done) != hipSuccess ||
         c->d_counters.ensure(sizeof(unsigned long long) * kCtrWords) != hipSuccess ||
@@ -198,12 +202,16 @@ void pm_destroy(pm_ctx *c)
     if (c->ev_k1) hipEventDestroy(c->ev_k1);
     if (c->ev_done) hipEventDestroy(c->ev_done);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream_build) {
+        hipStreamSynchronize(c->stream_build);
+        hipStreamDestroy(c->stream_build);
+    }
     delete c;
 }
 
 const char *pm_last_error(const pm_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
-static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n, float h)
+static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n, float h, hipStream_t st)
 {
     Frame *f;
     auto it = c->frames.find(frame);
@@ -215,14 +223,14 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    HIPCHK(c, hipEventRecord(e0, st));
     uint32_t np = bounds_partials(n);
     HIPCHK(c, f->bounds_partial.ensure(sizeof(float) * 6 * np));
     HIPCHK(c, f->bounds.ensure(sizeof(float) * 6));
-    HIPCHK(c, launch_bounds_reduce(d_src, n, (float *)f->bounds_partial.p, np, (float *)f->bounds.p, c->stream));
+    HIPCHK(c, launch_bounds_reduce(d_src, n, (float *)f->bounds_partial.p, np, (float *)f->bounds.p, st));
     float b[6];
-    HIPCHK(c, hipMemcpyAsync(b, f->bounds.p, sizeof(b), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(b, f->bounds.p, sizeof(b), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
     for (int a = 0; a < 6; a++)
         if (!std::isfinite(b[a])) return fail(c, PM_ERR_ARG, "pm_set_frame: non-finite particle coordinate");
 
@@ -283,7 +291,7 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     a.sorted = (float4 *)f->sorted.p;
     a.cell_start = (uint32_t *)f->cell_start.p;
     a.scattered = (float4 *)f->scattered.p;
-    HIPCHK(c, launch_grid_build(a, c->stream));
+    HIPCHK(c, launch_grid_build(a, st));
 
     GridBuildArgs fa = a;   // the march grid: same build, cells of h/2
     fa.inv_h = inv_hf;
@@ -295,8 +303,8 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     fa.scan_tmp = (uint32_t *)f->scan_tmp.p;
     fa.sorted = (float4 *)f->fsorted.p;
     fa.cell_start = (uint32_t *)f->fcell_start.p;
-    HIPCHK(c, launch_grid_build(fa, c->stream));
-    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, launch_grid_build(fa, st));
+    HIPCHK(c, hipEventRecord(e1, st));
     HIPCHK(c, hipEventSynchronize(e1));
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
@@ -326,7 +334,7 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
 
     // skip structures: empty 27-neighbourhoods and the reference DensityGrid
     HIPCHK(c, f->empty27.ensure(nc));
-    HIPCHK(c, launch_empty27((const uint32_t *)f->cell_start.p, I.dims[0], I.dims[1], I.dims[2], (uint8_t *)f->empty27.p, c->stream));
+    HIPCHK(c, launch_empty27((const uint32_t *)f->cell_start.p, I.dims[0], I.dims[1], I.dims[2], (uint8_t *)f->empty27.p, st));
     g.empty27 = (const uint8_t *)f->empty27.p;
     uint64_t dgn = 1;
     for (int a2 = 0; a2 < 3; a2++) {
@@ -342,8 +350,8 @@ static int build_frame(pm_ctx *c, int32_t frame, const float *d_src, uint64_t n,
     f->cg.nx = (I.dims[0] + 3) / 4; f->cg.ny = (I.dims[1] + 3) / 4; f->cg.nz = (I.dims[2] + 3) / 4;
     HIPCHK(c, f->coarse.ensure(4ull * f->cg.nx * f->cg.ny * f->cg.nz));
     f->cg.count = (const uint32_t *)f->coarse.p;
-    HIPCHK(c, launch_coarse_counts(g, f->cg, (uint32_t *)f->coarse.p, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, launch_coarse_counts(g, f->cg, (uint32_t *)f->coarse.p, st));
+    HIPCHK(c, hipStreamSynchronize(st));
     return PM_OK;
 }
 
@@ -358,7 +366,35 @@ int pm_set_frame(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, float h
     Frame *f = it == c->frames.end() ? (c->frames[frame] = new Frame()) : it->second;
     HIPCHK(c, f->xyz.ensure(sizeof(float) * 3 * n));
     HIPCHK(c, hipMemcpyAsync(f->xyz.p, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    return build_frame(c, frame, (const float *)f->xyz.p, n, h);
+    return build_frame(c, frame, (const float *)f->xyz.p, n, h, c->stream);
+}
+
+int pm_set_frame_stream(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, float h)
+{
+    if (!c) return PM_ERR_ARG;
+    if (!xyz || n == 0 || n >= (1ull << 31) || !(h > 0.0f) || !std::isfinite(h))
+        return fail(c, PM_ERR_ARG, "pm_set_frame_stream: need xyz, 0 < n < 2^31 and finite h > 0");
+    if (c->pending && (c->pending_batch || c->pending_frame == frame))
+        return fail(c, PM_ERR_STATE, "pm_set_frame_stream: the march in flight reads frame %d", frame);
+    HIPCHK(c, hipSetDevice(c->device));
+    auto it = c->frames.find(frame);
+    Frame *f = it == c->frames.end() ? (c->frames[frame] = new Frame()) : it->second;
+    HIPCHK(c, f->xyz.ensure(sizeof(float) * 3 * n));
+    HIPCHK(c, hipMemcpyAsync(f->xyz.p, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream_build));
+    return build_frame(c, frame, (const float *)f->xyz.p, n, h, c->stream_build);
+}
+
+int pm_load_bgeo(pm_ctx *c, int32_t frame, const char *path, float h)
+{
+    if (!c) return PM_ERR_ARG;
+    uint64_t n = 0;
+    int rc = pm_bgeo_read(path, nullptr, 0, &n);
+    if (rc) return fail(c, rc, "pm_load_bgeo: cannot read a BGEO V5 file at %s", path ? path : "(null)");
+    if (n == 0) return fail(c, PM_ERR_ARG, "pm_load_bgeo: %s holds no points", path);
+    std::vector<float> xyz(3 * n);
+    rc = pm_bgeo_read(path, xyz.data(), n, &n);
+    if (rc) return fail(c, rc, "pm_load_bgeo: cannot read the points of %s", path);
+    return c->pending ? pm_set_frame_stream(c, frame, xyz.data(), n, h) : pm_set_frame(c, frame, xyz.data(), n, h);
 }
 
 int pm_set_frame_device(pm_ctx *c, int32_t frame, const void *d_xyz, uint64_t n, float h)
@@ -368,7 +404,7 @@ int pm_set_frame_device(pm_ctx *c, int32_t frame, const void *d_xyz, uint64_t n,
         return fail(c, PM_ERR_ARG, "pm_set_frame_device: need d_xyz, 0 < n < 2^31 and finite h > 0");
     if (c->pending) return fail(c, PM_ERR_STATE, "pm_set_frame_device: a march is in flight");
     HIPCHK(c, hipSetDevice(c->device));
-    return build_frame(c, frame, (const float *)d_xyz, n, h);
+    return build_frame(c, frame, (const float *)d_xyz, n, h, c->stream);
 }
 
 static Frame *get_frame(pm_ctx *c, int32_t frame)
@@ -594,6 +630,8 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     }
     HIPCHK(c, hipEventRecord(c->ev_done, c->stream));
     c->pending = true;
+    c->pending_frame = frame;
+    c->pending_batch = false;
     c->finished = false;
     return PM_OK;
 }
@@ -888,6 +926,7 @@ int pm_batch_run(pm_ctx *c)
     c->user_pos = c->user_nrm = c->user_t = nullptr;
     c->prepass = false;   // the batch counts its answered-once rays in the counters
     c->pending = true;
+    c->pending_batch = true;
     c->finished = false;
     return PM_OK;
 }

@@ -100,6 +100,9 @@ def load():
     L.pm_density_aniso.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp]
     L.pm_density_aniso64.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp]
     L.pm_aniso_matrix.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp, vp, vp]
+    L.pm_bgeo_read.argtypes = [C.c_char_p, vp, u64, C.POINTER(u64)]
+    L.pm_load_bgeo.argtypes = [vp, i32, C.c_char_p, f32]
+    L.pm_set_frame_stream.argtypes = [vp, i32, vp, u64, f32]
     L.pm_stream_copy_gbps.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
     L.pm_get_skip_grids.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.pm_mlp_set.argtypes = [vp, vp]
@@ -121,6 +124,22 @@ def load():
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def bgeo_read(path) -> np.ndarray:
+    """Positions (n, 3) float32 of a BGEO V5 file (plain or gzip) through
+    pm_bgeo_read -- host code of the library, no GPU needed."""
+    L = load()
+    n = C.c_uint64(0)
+    bpath = str(path).encode()
+    rc = L.pm_bgeo_read(bpath, None, 0, C.byref(n))
+    if rc != PM_OK:
+        raise PmError(f"pm_bgeo_read({path}) failed ({rc})")
+    xyz = np.zeros((n.value, 3), np.float32)
+    rc = L.pm_bgeo_read(bpath, _ptr(xyz), n.value, C.byref(n))
+    if rc != PM_OK:
+        raise PmError(f"pm_bgeo_read({path}) failed ({rc})")
+    return xyz
 
 
 def settings_from(d: dict) -> PmSettings:
@@ -165,6 +184,17 @@ class Marcher:
     def set_frame(self, frame: int, xyz: np.ndarray, h: float):
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
         self._check(self.L.pm_set_frame(self.ctx, frame, _ptr(xyz), xyz.shape[0], float(h)), "pm_set_frame")
+        return self.frame_info(frame)
+
+    def set_frame_stream(self, frame: int, xyz: np.ndarray, h: float):
+        """pm_set_frame_stream: builds the frame beside a march of another frame in flight."""
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        self._check(self.L.pm_set_frame_stream(self.ctx, frame, _ptr(xyz), xyz.shape[0], float(h)),
+                    "pm_set_frame_stream")
+        return self.frame_info(frame)
+
+    def load_bgeo(self, frame: int, path, h: float):
+        self._check(self.L.pm_load_bgeo(self.ctx, frame, str(path).encode(), float(h)), "pm_load_bgeo")
         return self.frame_info(frame)
 
     def frame_info(self, frame: int) -> PmFrameInfo:

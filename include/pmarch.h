@@ -38,6 +38,7 @@ extern "C" {
 #define PM_ERR_SIZE   (-4)
 #define PM_ERR_NODEV  (-5)
 #define PM_ERR_STATE  (-6)
+#define PM_ERR_IO     (-7)
 
 #define PM_MODE_CONFIG     0  /* AABB entry, fixed step, bisection, gradient normals   */
 #define PM_MODE_REFERENCE  1  /* PerPixel_Isotropic, RayMarcher.cpp:256-344            */
@@ -141,7 +142,25 @@ const char *pm_last_error(const pm_ctx *ctx);
 int pm_set_frame(pm_ctx *ctx, int32_t frame, const float *xyz, uint64_t n, float h);
 /* Same, from positions already in device memory (frame streaming). */
 int pm_set_frame_device(pm_ctx *ctx, int32_t frame, const void *d_xyz, uint64_t n, float h);
+/* Frame streaming (the reference's autoplay re-renders the next frame each
+ * second, AdvancedRenderer.cpp:283-290): pm_set_frame on a second stream,
+ * allowed while a march of another frame is in flight, so the next frame's
+ * upload and grid build run beside it; returns once the frame is built.
+ * PM_ERR_STATE when the march in flight reads this frame (or is a batch). */
+int pm_set_frame_stream(pm_ctx *ctx, int32_t frame, const float *xyz, uint64_t n, float h);
 int pm_frame_info_get(pm_ctx *ctx, int32_t frame, pm_frame_info *out);
+
+/* ---- dataset IO: Dataset::Dataset / ReadFile (Dataset.cpp:169-227, 292-306)
+ * through Partio::readBGEO (vendor/partio/src/io/BGEO.cpp:187-300) ----
+ * Positions (the first three floats of P) of a classic Houdini BGEO file,
+ * version 5, plain or gzip-compressed.  With xyz == NULL only *n is set;
+ * otherwise the n points are written (PM_ERR_SIZE when n > cap).  PM_ERR_IO
+ * for an unreadable file, another format or version, a string attribute (as
+ * Partio refuses it) or a truncated file.  Host code: needs no context or GPU. */
+int pm_bgeo_read(const char *path, float *xyz, uint64_t cap, uint64_t *n);
+/* One frame from a BGEO file: pm_bgeo_read, then pm_set_frame (or
+ * pm_set_frame_stream while a march is in flight). */
+int pm_load_bgeo(pm_ctx *ctx, int32_t frame, const char *path, float h);
 /* The frame's grid: cell_start[ncells+1], sorted[4n] (x,y,z,0), index[n]
  * (input index of each stored particle).  Any pointer may be NULL. */
 int pm_get_grid(pm_ctx *ctx, int32_t frame, uint32_t *cell_start, float *sorted, uint32_t *index);
