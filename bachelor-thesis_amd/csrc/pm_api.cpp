@@ -1076,6 +1076,50 @@ int pm_aniso_matrix(pm_ctx *c, int32_t frame, const pm_settings *s, const float 
     return PM_OK;
 }
 
+// ------------------------------------------------------------ depth smoothing --
+
+int pm_gauss_kernel(int32_t N, float *K)
+{
+    if (!K || N < 0 || N > 31) return PM_ERR_ARG;
+    // ComputeGaussKernel (GaussRenderPass.cpp:20-48): gauss(r) = powf(E, -0.5 r r),
+    // r = sqrtf(i^2 + j^2); the sum counts each off-axis entry 4 times and each
+    // on-axis entry twice (the full window), then every entry is divided by it
+    const float E = 2.7182818284590452353602874713527f;
+    float sum = 0.0f;
+    for (int i = 0; i <= N; i++)
+        for (int j = 0; j <= N; j++) {
+            const float r = sqrtf((float)i * (float)i + (float)j * (float)j);
+            const float y = powf(E, -0.5f * r * r);
+            K[j + i * (N + 1)] = y;
+            if (i == 0 && j == 0) sum += y;
+            else if (i != 0 && j != 0) sum += 4.0f * y;
+            else sum += 2.0f * y;
+        }
+    for (int k = 0; k < (N + 1) * (N + 1); k++) K[k] /= sum;
+    return PM_OK;
+}
+
+int pm_depth_smooth(pm_ctx *c, const float *depth, uint32_t W, uint32_t H, int32_t N, float *out)
+{
+    if (!c) return PM_ERR_ARG;
+    if (!depth || !out || N < 0 || N > 31) return fail(c, PM_ERR_ARG, "pm_depth_smooth: need images and 0 <= N <= 31");
+    if (W == 0 || H == 0 || (uint64_t)W * H > (1ull << 30)) return fail(c, PM_ERR_SIZE, "bad image size %ux%u", W, H);
+    if (c->pending) return fail(c, PM_ERR_STATE, "pm_depth_smooth: a march is in flight");
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint64_t npx = (uint64_t)W * H;
+    float K[32 * 32];
+    pm_gauss_kernel(N, K);
+    HIPCHK(c, c->d_q.ensure(4 * npx + sizeof(K)));
+    HIPCHK(c, c->d_out.ensure(4 * npx));
+    float *d_in = (float *)c->d_q.p, *d_K = d_in + npx;
+    HIPCHK(c, hipMemcpyAsync(d_in, depth, 4 * npx, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_K, K, sizeof(float) * (N + 1) * (N + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_depth_smooth(d_in, (float *)c->d_out.p, W, H, N, d_K, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->d_out.p, 4 * npx, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
 // ---------------------------------------------------------------- learned --
 
 int pm_mlp_set(pm_ctx *c, const float *params)

@@ -136,6 +136,9 @@ hipError_t launch_neighbor_counts(const GridDev &g, const float *q, uint32_t nq,
                                   uint32_t *ncand, hipStream_t st);
 
 hipError_t launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
+// Gaussian depth smoothing (GaussRenderPass): K = the (N + 1)^2 table of pm_gauss_kernel.
+hipError_t launch_depth_smooth(const float *in, float *out, uint32_t W, uint32_t H, int N, const float *K,
+                               hipStream_t st);
 hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz, uint8_t *out, hipStream_t st);
 hipError_t launch_densitygrid(const GridDev &g, int32_t dnx, int32_t dny, int32_t dnz, uint8_t *flag_box, uint8_t *flag_cons,
                               uint32_t *count_box, hipStream_t st);

@@ -101,6 +101,8 @@ def load():
     L.pm_density_aniso64.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp]
     L.pm_aniso_matrix.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp, vp, vp]
     L.pm_bgeo_read.argtypes = [C.c_char_p, vp, u64, C.POINTER(u64)]
+    L.pm_gauss_kernel.argtypes = [i32, vp]
+    L.pm_depth_smooth.argtypes = [vp, vp, u32, u32, i32, vp]
     L.pm_load_bgeo.argtypes = [vp, i32, C.c_char_p, f32]
     L.pm_set_frame_stream.argtypes = [vp, i32, vp, u64, f32]
     L.pm_stream_copy_gbps.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
@@ -140,6 +142,14 @@ def bgeo_read(path) -> np.ndarray:
     if rc != PM_OK:
         raise PmError(f"pm_bgeo_read({path}) failed ({rc})")
     return xyz
+
+
+def gauss_kernel(N: int) -> np.ndarray:
+    """pm_gauss_kernel: ComputeGaussKernel's (N+1)^2 table, [i][j] (host code)."""
+    K = np.zeros((N + 1) * (N + 1), np.float32)
+    if load().pm_gauss_kernel(int(N), _ptr(K)) != PM_OK:
+        raise PmError(f"pm_gauss_kernel({N}) failed")
+    return K.reshape(N + 1, N + 1)
 
 
 def settings_from(d: dict) -> PmSettings:
@@ -196,6 +206,14 @@ class Marcher:
     def load_bgeo(self, frame: int, path, h: float):
         self._check(self.L.pm_load_bgeo(self.ctx, frame, str(path).encode(), float(h)), "pm_load_bgeo")
         return self.frame_info(frame)
+
+    def depth_smooth(self, depth, N: int = 8):
+        """GaussRenderPass's smoothing of a row-major depth image on the GPU."""
+        depth = np.ascontiguousarray(depth, np.float32)
+        H, W = depth.shape
+        out = np.zeros_like(depth)
+        self._check(self.L.pm_depth_smooth(self.ctx, _ptr(depth), W, H, int(N), _ptr(out)), "pm_depth_smooth")
+        return out
 
     def frame_info(self, frame: int) -> PmFrameInfo:
         fi = PmFrameInfo()

@@ -239,6 +239,19 @@ int pm_density_aniso64(pm_ctx *ctx, int32_t frame, const pm_settings *s, const d
 int pm_aniso_matrix(pm_ctx *ctx, int32_t frame, const pm_settings *s, const float *q, uint64_t nq, float *G,
                     float *det, uint32_t *n_ext);
 
+/* ---- screen-space depth smoothing: GaussRenderPass (GaussRenderPass.cpp:8-48,
+ * 347-358; assets/shaders/advanced/gauss.frag) ----
+ * pm_gauss_kernel: ComputeGaussKernel's table, K[j + i (N + 1)] =
+ * exp(-r^2 / 2) (r = sqrt(i^2 + j^2), powf as the reference evaluates it)
+ * divided by its sum over the (2N + 1)^2 window; host code.
+ * pm_depth_smooth: out(x, y) = sum over i, j in [-N, N] (i horizontal, outer)
+ * of K(|i|, |j|) depth(x + i, y + j), coordinates clamped to the image (the
+ * depth texture's clamp-to-edge sampler, BilateralBuffer.cpp:246-248) at
+ * texel centres (Spread = 1).  Host images, W*H floats, row-major; 0 <= N <= 31
+ * (the reference's 32 x 32 table). */
+int pm_gauss_kernel(int32_t N, float *K);
+int pm_depth_smooth(pm_ctx *ctx, const float *depth, uint32_t W, uint32_t H, int32_t N, float *out);
+
 /* ---- learned step proposal (no counterpart in the reference; it replaces the
  * role of the DensityGrid skip, Dataset.cpp:94-164, RayMarcher.cpp:284-304) ----
  * A 24 -> 64 -> 64 -> 1 MLP, BF16 weights with FP32 accumulation on MFMA,

@@ -165,6 +165,9 @@ def lib():
         L.or_neighbors_ext.restype = C.c_uint32
         L.or_neighbors_ext.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), C.c_float,
                                        fp(np.float32, flags="C"), C.c_uint32, C.POINTER(C.c_uint32)]
+        L.or_gauss_kernel.argtypes = [C.c_int32, fp(np.float32, flags="C")]
+        L.or_depth_smooth.argtypes = [fp(np.float32, flags="C"), C.c_uint32, C.c_uint32, C.c_int32,
+                                      fp(np.float32, flags="C")]
         L.or_ray.argtypes = [fp(np.float32, flags="C"), fp(np.float32, flags="C"), C.c_uint32, C.c_uint32,
                              C.c_uint32, fp(np.float32, flags="C"), fp(np.float32, flags="C")]
         L.or_slab.argtypes = [fp(np.float32, flags="C")] * 5
@@ -366,6 +369,22 @@ def eig3(c6, fp64=False):
     V = np.zeros(9, dt)
     (lib().or_eig3_d if fp64 else lib().or_eig3_f)(c, lam, V)
     return lam, V.reshape(3, 3)
+
+
+def gauss_kernel(N: int) -> np.ndarray:
+    """ComputeGaussKernel (GaussRenderPass.cpp:20-48): the (N+1)^2 table, [i][j]."""
+    K = np.zeros((N + 1) * (N + 1), np.float32)
+    assert lib().or_gauss_kernel(N, K) == 0
+    return K.reshape(N + 1, N + 1)
+
+
+def depth_smooth(depth, N: int) -> np.ndarray:
+    """gauss.frag over a row-major depth image, clamp-to-edge, Spread = 1."""
+    depth = np.ascontiguousarray(depth, np.float32)
+    H, W = depth.shape
+    out = np.zeros_like(depth)
+    assert lib().or_depth_smooth(depth.reshape(-1), W, H, N, out.reshape(-1)) == 0
+    return out
 
 
 def neighbors_brute(xyz, h, q) -> np.ndarray:

@@ -1544,3 +1544,48 @@ int or_march_f64(const or_grid *g, const or_settings *s, const float invpv[16], 
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Screen-space depth smoothing -- GaussRenderPass: ComputeGaussKernel        */
+/* (GaussRenderPass.cpp:8-48) and gauss.frag's window sum, clamp-to-edge      */
+/* texel-centre taps (BilateralBuffer.cpp:246-248, Spread = 1).               */
+/* ------------------------------------------------------------------------- */
+
+int or_gauss_kernel(int32_t N, float *K)
+{
+    if (!K || N < 0 || N > 31) return -1;
+    const float E = 2.7182818284590452353602874713527f;     /* GaussRenderPass.cpp:10 */
+    float sum = 0.0f;
+    for (int i = 0; i <= N; i++)
+        for (int j = 0; j <= N; j++) {
+            float r = sqrtf((float)i * (float)i + (float)j * (float)j);   /* :17-20 */
+            float y = powf(E, -0.5f * r * r);                             /* :12-15 */
+            K[j + i * (N + 1)] = y;
+            if (i == 0 && j == 0) sum += y;
+            else if (i != 0 && j != 0) sum += 4.0f * y;
+            else sum += 2.0f * y;
+        }
+    for (int k = 0; k < (N + 1) * (N + 1); k++) K[k] /= sum;
+    return 0;
+}
+
+int or_depth_smooth(const float *depth, uint32_t W, uint32_t H, int32_t N, float *out)
+{
+    float K[32 * 32];
+    if (!depth || !out || W == 0 || H == 0 || or_gauss_kernel(N, K)) return -1;
+    for (uint32_t y = 0; y < H; y++)
+        for (uint32_t x = 0; x < W; x++) {
+            float s = 0.0f;
+            for (int i = -N; i <= N; i++) {                 /* gauss.frag: i outer (u), j inner (v) */
+                int gx = (int)x + i;
+                gx = gx < 0 ? 0 : (gx > (int)W - 1 ? (int)W - 1 : gx);
+                for (int j = -N; j <= N; j++) {
+                    int gy = (int)y + j;
+                    gy = gy < 0 ? 0 : (gy > (int)H - 1 ? (int)H - 1 : gy);
+                    s += K[(i < 0 ? -i : i) * (N + 1) + (j < 0 ? -j : j)] * depth[(size_t)gy * W + (size_t)gx];
+                }
+            }
+            out[(size_t)y * W + x] = s;
+        }
+    return 0;
+}

@@ -170,6 +170,10 @@ void or_eig3_d(const double c[6], double lam[3], double V[9]);
 /* The extended neighbours of q in walk order (positions, 3 floats each). */
 uint32_t or_neighbors_ext(const or_grid *g, const float q[3], float hx, float *out, uint32_t cap, uint32_t *ncand);
 
+/* ---- screen-space depth smoothing (GaussRenderPass.cpp:8-48, gauss.frag) ---- */
+int or_gauss_kernel(int32_t N, float *K);
+int or_depth_smooth(const float *depth, uint32_t W, uint32_t H, int32_t N, float *out);
+
 /* ---- ray generation (RayMarcher.cpp:268-274) and AABB slab test ---- */
 void or_ray(const float invpv[16], const float cam[3], uint32_t W, uint32_t H, uint32_t idx,
             float o[3], float d[3]);
