@@ -159,3 +159,32 @@ def test_aniso_settings_rejected(marcher):
         st = scenes.aniso_settings(sc, **bad)
         with pytest.raises(Exception):
             marcher.march(0, st, cam.inv_pv, cam.eye, 64, 48)
+
+
+@pytest.mark.parametrize("case", ["coincident", "mixed"])
+def test_aniso_degenerate_neighbourhoods(marcher, case):
+    """Singular covariances: three coincident particles (C = 0: the clamp gives
+    Sigma = 0 and G is NaN, so the density is never >= iso there) and a
+    coincident pair next to a collinear row (rank-deficient C, clamped)."""
+    if case == "coincident":
+        xyz = np.array([[0.5, 0.5, 0.5]] * 3 + [[2.0, 2.0, 2.0]], np.float32)
+    else:
+        xyz = np.array([[0.5, 0.5, 0.5]] * 2 + [[0.5, 0.5, 0.55]] + [[0.5 + 0.02 * k, 0.5, 0.7] for k in range(5)],
+                       np.float32)
+    sc = scenes.tiny_scene(xyz, 0.1, eye=(0.5, 0.5, -0.5), target=(0.5, 0.5, 0.55), width=48, height=48)
+    print(f"aniso degenerate {case}", flush=True)
+    marcher.set_frame(6, sc.xyz, sc.h)
+    G = O.Grid(sc.xyz, sc.h)
+    st = scenes.aniso_settings(sc)
+    s = O.make_settings(**st)
+    rng = np.random.default_rng(9)
+    q = (np.array([0.5, 0.5, 0.55]) + rng.uniform(-0.12, 0.12, (300, 3))).astype(np.float32)
+    q[:3] = [[0.5, 0.5, 0.5], [0.5, 0.5, 0.52], [0.54, 0.5, 0.7]]
+    Gg, dg, ng = marcher.aniso_matrix(6, st, q)
+    for i in range(len(q)):
+        n, Go, do = G.aniso_G(s, q[i])
+        assert ng[i] == n
+        if n:
+            assert np.array_equal(Gg[i], Go, equal_nan=True) and np.array_equal(dg[i], np.float32(do), equal_nan=True)
+    assert np.array_equal(marcher.density_aniso(6, st, q), G.density_aniso(s, q), equal_nan=True)
+    _march_pair(marcher, 6, sc, st, sc.cameras[0], 1)
