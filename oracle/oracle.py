@@ -90,7 +90,7 @@ _lib = None
 
 def _default_threads() -> int:
     """Worker threads of the oracle's marches: the machine's cores less one,
-    at most 16 (a GPU box's CPU share)."""
+    at most 16 (a shared machine's share of cores per GPU)."""
     return max(1, min(16, (os.cpu_count() or 2) - 1))
 
 
