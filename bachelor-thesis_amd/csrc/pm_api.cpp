@@ -154,6 +154,9 @@ static hipError_t zero_sync(pm_ctx *c, void *dst, size_t bytes)
     return hipStreamSynchronize(c->stream);
 }
 
+// Every point of a BGEO file, read as the data arrives (pm_bgeo.cpp).
+int pm_bgeo_read_all(const char *path, std::vector<float> &out);
+
 extern "C" {
 
 int pm_create(int device, pm_ctx **out)
@@ -387,13 +390,11 @@ int pm_set_frame_stream(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, 
 int pm_load_bgeo(pm_ctx *c, int32_t frame, const char *path, float h)
 {
     if (!c) return PM_ERR_ARG;
-    uint64_t n = 0;
-    int rc = pm_bgeo_read(path, nullptr, 0, &n);
+    std::vector<float> xyz;
+    const int rc = pm_bgeo_read_all(path, xyz);
     if (rc) return fail(c, rc, "pm_load_bgeo: cannot read a BGEO V5 file at %s", path ? path : "(null)");
+    const uint64_t n = xyz.size() / 3;
     if (n == 0) return fail(c, PM_ERR_ARG, "pm_load_bgeo: %s holds no points", path);
-    std::vector<float> xyz(3 * n);
-    rc = pm_bgeo_read(path, xyz.data(), n, &n);
-    if (rc) return fail(c, rc, "pm_load_bgeo: cannot read the points of %s", path);
     return c->pending ? pm_set_frame_stream(c, frame, xyz.data(), n, h) : pm_set_frame(c, frame, xyz.data(), n, h);
 }
 

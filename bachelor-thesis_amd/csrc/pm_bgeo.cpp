@@ -116,6 +116,31 @@ static int bgeo_header(GzIn &in, int32_t &npoints)
     return in.ok ? (int)words : PM_ERR_IO;
 }
 
+// Every point of the file, appended to out as it is read: memory grows with
+// the data actually present, so a header that claims more points than the
+// file holds fails at the end of the data, not at an allocation.
+int pm_bgeo_read_all(const char *path, std::vector<float> &out)
+{
+    if (!path) return PM_ERR_ARG;
+    GzIn in(path);
+    if (!in.ok) return PM_ERR_IO;
+    int32_t np = 0;
+    const int words = bgeo_header(in, np);
+    if (words < 0) return words;
+    std::vector<unsigned char> rec((size_t)words * 4);
+    out.clear();
+    try {
+        for (int32_t i = 0; i < np; i++) {
+            in.bytes(rec.data(), rec.size());
+            if (!in.ok) return PM_ERR_IO;
+            for (int a = 0; a < 3; a++) out.push_back(be_float(&rec[4 * a]));
+        }
+    } catch (const std::bad_alloc &) {
+        return PM_ERR_SIZE;
+    }
+    return PM_OK;
+}
+
 extern "C" int pm_bgeo_read(const char *path, float *xyz, uint64_t cap, uint64_t *n)
 {
     if (!path || !n) return PM_ERR_ARG;
