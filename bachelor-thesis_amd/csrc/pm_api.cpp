@@ -69,6 +69,7 @@ struct pm_ctx {
     hipStream_t stream_build = nullptr;   // frame builds of pm_set_frame_stream
     int32_t pending_frame = 0;            // frame of the march in flight
     bool pending_batch = false;           // the march in flight is a batch (all its frames)
+    bool outputs_on_device = false;       // the last march wrote the caller's device memory
     hipEvent_t ev_launch = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
     std::map<int32_t, Frame *> frames;
     std::string err;
@@ -534,9 +535,13 @@ static AnisoDev aniso_params(const Frame *f, const pm_settings *s)
     return a;
 }
 
-int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float inv_pv[16], const float cam[3],
-                   uint32_t W, uint32_t H, const float *depth_in, float *positions_out, float *normals_out,
-                   float *t_out)
+// The march of one view.  dev != nullptr: the outputs go straight into the
+// caller's device memory dev[0..2] (t may be null) and depth_in is device
+// memory; otherwise into the context's buffers, copied to the host pointers
+// (any may be null) at completion.
+static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const float inv_pv[16], const float cam[3],
+                        uint32_t W, uint32_t H, const float *depth_in, float *positions_out, float *normals_out,
+                        float *t_out, void *const *dev)
 {
     if (!c) return PM_ERR_ARG;
     if (c->pending) return fail(c, PM_ERR_STATE, "pm_march_async: previous march still in flight");
@@ -560,10 +565,13 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
         if (rc) return rc;
     }
     uint64_t npx = (uint64_t)W * H;
-    HIPCHK(c, c->d_pos.ensure(sizeof(float4) * npx));
-    HIPCHK(c, c->d_nrm.ensure(sizeof(float4) * npx));
-    HIPCHK(c, c->d_t.ensure(sizeof(float) * npx));
+    if (!dev) {
+        HIPCHK(c, c->d_pos.ensure(sizeof(float4) * npx));
+        HIPCHK(c, c->d_nrm.ensure(sizeof(float4) * npx));
+        HIPCHK(c, c->d_t.ensure(sizeof(float) * npx));
+    }
     c->npx = npx;
+    c->outputs_on_device = dev != nullptr;
 
     MarchLaunch L{};
     L.g = f->g;
@@ -573,9 +581,9 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     memcpy(L.M, inv_pv, sizeof(L.M));
     memcpy(L.cam, cam, sizeof(L.cam));
     L.W = W; L.H = H;
-    L.pos = (float4 *)c->d_pos.p;
-    L.nrm = (float4 *)c->d_nrm.p;
-    L.t = (float *)c->d_t.p;
+    L.pos = dev ? (float4 *)dev[0] : (float4 *)c->d_pos.p;
+    L.nrm = dev ? (float4 *)dev[1] : (float4 *)c->d_nrm.p;
+    L.t = dev ? (float *)dev[2] : (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
     L.band_ctr = (uint32_t *)c->d_band.p;
     if (s->mode == PM_MODE_CONFIG) {   // hit records; the anisotropic march's hit list
@@ -602,9 +610,13 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
 
     HIPCHK(c, hipEventRecord(c->ev_launch, c->stream));
     if (s->mode == PM_MODE_REFERENCE) {
-        HIPCHK(c, c->d_depth.ensure(sizeof(float) * npx));
-        HIPCHK(c, hipMemcpyAsync(c->d_depth.p, depth_in, sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
-        L.depth = (const float *)c->d_depth.p;
+        if (dev) {
+            L.depth = depth_in;
+        } else {
+            HIPCHK(c, c->d_depth.ensure(sizeof(float) * npx));
+            HIPCHK(c, hipMemcpyAsync(c->d_depth.p, depth_in, sizeof(float) * npx, hipMemcpyHostToDevice, c->stream));
+            L.depth = (const float *)c->d_depth.p;
+        }
     }
     HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, sizeof(unsigned long long) * kCtrWords, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
@@ -612,6 +624,7 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->h_counters, c->d_counters.p, sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, c->stream));
 
+    if (dev) positions_out = normals_out = t_out = nullptr;
     c->user_pos = positions_out;
     c->user_nrm = normals_out;
     c->user_t = t_out;
@@ -635,6 +648,23 @@ int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float i
     c->pending_batch = false;
     c->finished = false;
     return PM_OK;
+}
+
+int pm_march_async(pm_ctx *c, int32_t frame, const pm_settings *s, const float inv_pv[16], const float cam[3],
+                   uint32_t W, uint32_t H, const float *depth_in, float *positions_out, float *normals_out,
+                   float *t_out)
+{
+    return march_launch(c, frame, s, inv_pv, cam, W, H, depth_in, positions_out, normals_out, t_out, nullptr);
+}
+
+int pm_march_async_device(pm_ctx *c, int32_t frame, const pm_settings *s, const float inv_pv[16],
+                          const float cam[3], uint32_t W, uint32_t H, const float *d_depth, void *d_positions,
+                          void *d_normals, void *d_t)
+{
+    if (!c) return PM_ERR_ARG;
+    if (!d_positions || !d_normals) return fail(c, PM_ERR_ARG, "pm_march_async_device: need device positions and normals");
+    void *dev[3] = { d_positions, d_normals, d_t };
+    return march_launch(c, frame, s, inv_pv, cam, W, H, d_depth, nullptr, nullptr, nullptr, dev);
 }
 
 static int complete(pm_ctx *c)
@@ -691,6 +721,7 @@ int pm_download(pm_ctx *c, float *positions, float *normals, float *t)
     if (!c) return PM_ERR_ARG;
     if (c->pending) return fail(c, PM_ERR_STATE, "pm_download: march in flight; call pm_wait first");
     if (c->npx == 0) return fail(c, PM_ERR_STATE, "pm_download: nothing marched yet");
+    if (c->outputs_on_device) return fail(c, PM_ERR_STATE, "pm_download: the last march wrote the caller's device memory");
     HIPCHK(c, hipSetDevice(c->device));
     if (positions) HIPCHK(c, copy_sync(c, positions, c->d_pos.p, 16 * c->npx, hipMemcpyDeviceToHost));
     if (normals) HIPCHK(c, copy_sync(c, normals, c->d_nrm.p, 16 * c->npx, hipMemcpyDeviceToHost));

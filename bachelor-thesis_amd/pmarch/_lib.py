@@ -102,6 +102,7 @@ def load():
     L.pm_aniso_matrix.argtypes = [vp, i32, C.POINTER(PmSettings), vp, u64, vp, vp, vp]
     L.pm_bgeo_read.argtypes = [C.c_char_p, vp, u64, C.POINTER(u64)]
     L.pm_gauss_kernel.argtypes = [i32, vp]
+    L.pm_march_async_device.argtypes = [vp, i32, C.POINTER(PmSettings), vp, vp, u32, u32, vp, vp, vp, vp]
     L.pm_depth_smooth.argtypes = [vp, vp, u32, u32, i32, vp]
     L.pm_load_bgeo.argtypes = [vp, i32, C.c_char_p, f32]
     L.pm_set_frame_stream.argtypes = [vp, i32, vp, u64, f32]
@@ -244,6 +245,19 @@ class Marcher:
         self._check(self.L.pm_march_async(self.ctx, frame, C.byref(s), _ptr(inv_pv), _ptr(cam), W, H, _ptr(depth),
                                           _ptr(pos), _ptr(nrm), _ptr(t)), "pm_march_async")
         return pos, nrm, t
+
+    def march_device(self, frame: int, settings: dict, inv_pv, cam, W: int, H: int, d_pos: int, d_nrm: int,
+                     d_t: int = 0, d_depth: int = 0):
+        """pm_march_async_device + wait: the images go into the caller's device
+        memory (integer device addresses)."""
+        s = settings_from(settings)
+        inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
+        cam = np.ascontiguousarray(cam, np.float32).reshape(3)
+        self._keep = dict(s=s, inv_pv=inv_pv, cam=cam)
+        self._check(self.L.pm_march_async_device(self.ctx, frame, C.byref(s), _ptr(inv_pv), _ptr(cam), W, H,
+                                                 d_depth or None, d_pos, d_nrm, d_t or None), "pm_march_async_device")
+        self.wait()
+        return self.stats()
 
     def query(self) -> int:
         return self._check(self.L.pm_query(self.ctx), "pm_query")

@@ -188,6 +188,15 @@ int pm_get_bound_grid(pm_ctx *ctx, int32_t frame, float iso, uint64_t *bits);
 int pm_march_async(pm_ctx *ctx, int32_t frame, const pm_settings *s, const float inv_pv[16],
                    const float cam[3], uint32_t W, uint32_t H, const float *depth_in,
                    float *positions_out, float *normals_out, float *t_out);
+/* The same march with every image in device memory the caller owns -- e.g.
+ * buffers the renderer imported from its Vulkan images (the RGBA32F
+ * positions/normals of BilateralBuffer.cpp:14-24): the kernels write them
+ * directly, no host copy.  d_positions and d_normals hold W*H float4 each,
+ * d_t W*H floats or NULL; d_depth (reference mode) is device memory too.  They
+ * must stay valid until pm_query returns PM_OK; pm_download then refuses. */
+int pm_march_async_device(pm_ctx *ctx, int32_t frame, const pm_settings *s, const float inv_pv[16],
+                          const float cam[3], uint32_t W, uint32_t H, const float *d_depth, void *d_positions,
+                          void *d_normals, void *d_t);
 /* RayMarcher::IsDone: PM_OK when the march and its copies are complete,
  * PM_PENDING while running, negative on error. */
 int pm_query(pm_ctx *ctx);
