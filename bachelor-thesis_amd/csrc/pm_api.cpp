@@ -76,6 +76,7 @@ struct pm_ctx {
     // outputs of the last march
     DevBuf d_pos, d_nrm, d_t, d_depth, d_counters, d_q, d_out, d_band, d_tiles;
     DevBuf d_hits;                     // per-pixel hit records of one march
+    DevBuf d_gauss;                    // the smoothing tables for N = 0..31, written once
     uint64_t npx = 0;
     bool have_t = false;
     // pinned staging for asynchronous hand-off
@@ -191,6 +192,7 @@ void pm_destroy(pm_ctx *c)
     for (auto &kv : c->frames) { kv.second->release(); delete kv.second; }
     c->d_pos.release(); c->d_nrm.release(); c->d_t.release(); c->d_depth.release();
     c->d_counters.release(); c->d_q.release(); c->d_out.release(); c->d_band.release(); c->d_tiles.release();
+    c->d_gauss.release();
     c->d_hits.release();
     c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release(); c->mlp_W1.release(); c->mlp_W2.release();
     c->mlp_W2T.release(); c->mlp_partial.release(); c->mlp_out.release(); c->mlp_x.release(); c->mlp_idx.release();
@@ -535,6 +537,21 @@ static AnisoDev aniso_params(const Frame *f, const pm_settings *s)
     return a;
 }
 
+// True when [p, p + bytes) is device (or managed) memory of `device`: a kernel
+// handed a host address would fault.  The allocation's extent is checked where
+// HIP reports it (memory imported from other APIs may not report one).
+static bool device_range(int device, const void *p, size_t bytes)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) { (void)hipGetLastError(); return false; }
+    if ((a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) || a.device != device) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) { (void)hipGetLastError(); return true; }
+    return (const char *)p + bytes <= (const char *)base + size;
+}
+
 // The march of one view.  dev != nullptr: the outputs go straight into the
 // caller's device memory dev[0..2] (t may be null) and depth_in is device
 // memory; otherwise into the context's buffers, copied to the host pointers
@@ -565,6 +582,13 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
         if (rc) return rc;
     }
     uint64_t npx = (uint64_t)W * H;
+    if (dev) {
+        if (!device_range(c->device, dev[0], 16 * npx) || !device_range(c->device, dev[1], 16 * npx) ||
+            (dev[2] && !device_range(c->device, dev[2], 4 * npx)) ||
+            (depth_in && !device_range(c->device, depth_in, 4 * npx)))
+            return fail(c, PM_ERR_ARG, "pm_march_async_device: images must be device memory of GPU %d, W*H long",
+                        c->device);
+    }
     if (!dev) {
         HIPCHK(c, c->d_pos.ensure(sizeof(float4) * npx));
         HIPCHK(c, c->d_nrm.ensure(sizeof(float4) * npx));
@@ -1131,6 +1155,24 @@ int pm_gauss_kernel(int32_t N, float *K)
     return PM_OK;
 }
 
+// Offset of N's (N + 1)^2 table among the tables for 0..31 stored back to back.
+static inline size_t gauss_offset(int N) { return (size_t)N * (N + 1) * (2 * N + 1) / 6; }
+
+// The smoothing tables on the device, uploaded on first use and never written
+// again, so launches in flight on any stream may read them.
+static int ensure_gauss(pm_ctx *c, const float **d_K)
+{
+    if (!c->d_gauss.p) {
+        std::vector<float> all(gauss_offset(32));
+        for (int n = 0; n <= 31; n++) pm_gauss_kernel(n, all.data() + gauss_offset(n));
+        HIPCHK(c, c->d_gauss.ensure(sizeof(float) * all.size()));
+        hipError_t e = copy_sync(c, c->d_gauss.p, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { c->d_gauss.release(); HIPCHK(c, e); }
+    }
+    *d_K = (const float *)c->d_gauss.p;
+    return PM_OK;
+}
+
 int pm_depth_smooth(pm_ctx *c, const float *depth, uint32_t W, uint32_t H, int32_t N, float *out)
 {
     if (!c) return PM_ERR_ARG;
@@ -1139,16 +1181,36 @@ int pm_depth_smooth(pm_ctx *c, const float *depth, uint32_t W, uint32_t H, int32
     if (c->pending) return fail(c, PM_ERR_STATE, "pm_depth_smooth: a march is in flight");
     HIPCHK(c, hipSetDevice(c->device));
     const uint64_t npx = (uint64_t)W * H;
-    float K[32 * 32];
-    pm_gauss_kernel(N, K);
-    HIPCHK(c, c->d_q.ensure(4 * npx + sizeof(K)));
+    const float *d_K = nullptr;
+    int rc = ensure_gauss(c, &d_K);
+    if (rc) return rc;
+    HIPCHK(c, c->d_q.ensure(4 * npx));
     HIPCHK(c, c->d_out.ensure(4 * npx));
-    float *d_in = (float *)c->d_q.p, *d_K = d_in + npx;
+    float *d_in = (float *)c->d_q.p;
     HIPCHK(c, hipMemcpyAsync(d_in, depth, 4 * npx, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(d_K, K, sizeof(float) * (N + 1) * (N + 1), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_depth_smooth(d_in, (float *)c->d_out.p, W, H, N, d_K, c->stream));
+    HIPCHK(c, launch_depth_smooth(d_in, (float *)c->d_out.p, W, H, N, d_K + gauss_offset(N), c->stream));
     HIPCHK(c, hipMemcpyAsync(out, c->d_out.p, 4 * npx, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
+int pm_depth_smooth_device(pm_ctx *c, const float *d_depth, uint32_t W, uint32_t H, int32_t N, float *d_out,
+                           void *stream)
+{
+    if (!c) return PM_ERR_ARG;
+    if (N < 0 || N > 31) return fail(c, PM_ERR_ARG, "pm_depth_smooth_device: need 0 <= N <= 31");
+    if (W == 0 || H == 0 || (uint64_t)W * H > (1ull << 30)) return fail(c, PM_ERR_SIZE, "bad image size %ux%u", W, H);
+    const uint64_t nb = 4ull * W * H;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!device_range(c->device, d_depth, nb) || !device_range(c->device, d_out, nb))
+        return fail(c, PM_ERR_ARG, "pm_depth_smooth_device: images must be device memory of GPU %d, W*H floats",
+                    c->device);
+    const char *a = (const char *)d_depth, *b = (const char *)d_out;
+    if (a < b + nb && b < a + nb) return fail(c, PM_ERR_ARG, "pm_depth_smooth_device: input and output overlap");
+    const float *d_K = nullptr;
+    int rc = ensure_gauss(c, &d_K);
+    if (rc) return rc;
+    HIPCHK(c, launch_depth_smooth(d_depth, d_out, W, H, N, d_K + gauss_offset(N), (hipStream_t)stream));
     return PM_OK;
 }
 

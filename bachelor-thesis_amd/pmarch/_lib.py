@@ -104,6 +104,7 @@ def load():
     L.pm_gauss_kernel.argtypes = [i32, vp]
     L.pm_march_async_device.argtypes = [vp, i32, C.POINTER(PmSettings), vp, vp, u32, u32, vp, vp, vp, vp]
     L.pm_depth_smooth.argtypes = [vp, vp, u32, u32, i32, vp]
+    L.pm_depth_smooth_device.argtypes = [vp, vp, u32, u32, i32, vp, vp]
     L.pm_load_bgeo.argtypes = [vp, i32, C.c_char_p, f32]
     L.pm_set_frame_stream.argtypes = [vp, i32, vp, u64, f32]
     L.pm_stream_copy_gbps.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
@@ -215,6 +216,13 @@ class Marcher:
         out = np.zeros_like(depth)
         self._check(self.L.pm_depth_smooth(self.ctx, _ptr(depth), W, H, int(N), _ptr(out)), "pm_depth_smooth")
         return out
+
+    def depth_smooth_device(self, d_depth: int, d_out: int, W: int, H: int, N: int = 8, stream: int = 0):
+        """pm_depth_smooth_device: enqueue the smoothing of device memory
+        (integer device addresses) on `stream` (an integer hipStream_t handle,
+        0 for the null stream); returns without waiting."""
+        self._check(self.L.pm_depth_smooth_device(self.ctx, d_depth, W, H, int(N), d_out, stream or None),
+                    "pm_depth_smooth_device")
 
     def frame_info(self, frame: int) -> PmFrameInfo:
         fi = PmFrameInfo()

@@ -30,6 +30,10 @@ the anisotropic kernel) on the dam-break column scaled to h = 0.1 (the
 particle radius its k_s = 2000 is tuned for), 1920x1080, IsoDensity 1.0,
 step h/8, 8 bisections (last in FP64), central-difference normals, exact skip of
 the sub-boxes with no particle within h (the density is 0 there).
+
+--workload depth_smooth_1080p: the screen-space depth smoothing
+(GaussRenderPass, N = 8: a 17x17 window) of a 1920x1080 depth image resident
+in device memory; one step = one pass over the image.
 """
 from __future__ import annotations
 
@@ -48,7 +52,8 @@ sys.path.insert(0, str(ROOT / "bachelor-thesis_amd"))
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC = [ROOT / "profiles" / "march_dam_break_1080p_traffic.json",
-           ROOT / "profiles" / "march_aniso_dam_break_1080p_traffic.json"]
+           ROOT / "profiles" / "march_aniso_dam_break_1080p_traffic.json",
+           ROOT / "profiles" / "depth_smooth_1080p_traffic.json"]
 
 
 def cpu_model() -> str:
@@ -241,6 +246,92 @@ def bench_aniso(args):
     print(json.dumps(out))
 
 
+def smooth_depth_image(W: int, H: int, seed: int = 0) -> np.ndarray:
+    """A sprite-style depth image: 1.0 (nothing drawn) outside a disc of smooth
+    depth values with per-pixel noise; the pass has no data-dependent work, so
+    the content only has to be realistic, not exact."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:H, 0:W].astype(np.float32)
+    r = np.hypot((x - W / 2) / (0.4 * W), (y - H / 2) / (0.4 * H))
+    d = 0.6 + 0.2 * r + rng.uniform(-0.01, 0.01, (H, W))
+    return np.where(r < 1.0, d, 1.0).astype(np.float32)
+
+
+def cpu_baseline_smooth(depth: np.ndarray, N: int, budget_s: float):
+    """The oracle's smoothing (one thread, the same sums in the same order) on
+    the same image."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    H, W = depth.shape
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.depth_smooth(depth, N)
+        runs.append(time.perf_counter() - t0)
+        if sum(runs) > budget_s:
+            break
+    dt = float(np.median(runs))
+    return {"value": round(W * H / dt / 1e6, 4), "unit": "Mpixels/s", "cores": 1, "kind": "port",
+            "sample": f"the same {W}x{H} image, N = {N}, median of {len(runs)} runs of {dt:.2f} s, 1 thread "
+                      f"({cpu_model()})"}
+
+
+def bench_smooth(args):
+    """GaussRenderPass's depth smoothing, N = 8, on a 1080p image in HBM."""
+    import torch
+    import pmarch
+
+    W, H, N = 1920, 1080, 8
+    depth = smooth_depth_image(W, H)
+    m = pmarch.Marcher(0)
+    din = torch.from_numpy(depth).to("cuda:0")
+    dout = torch.empty_like(din)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    run = lambda: m.depth_smooth_device(din.data_ptr(), dout.data_ptr(), W, H, N, stream.cuda_stream)
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        run()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    B = 8.0 * W * H                    # one read and one write of the image
+    achieved = B / (kms * 1e-3) / 1e9
+    out = {
+        "metric": "depth smoothing Mpixels/s @1080p (GaussRenderPass, 17x17 window)",
+        "value": round(args.steps * W * H / wall / 1e6, 3),
+        "unit": "Mpixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "screen-space depth smoothing (GaussRenderPass: normalised Gaussian, N = 8, "
+                               "clamp-to-edge) of a 1920x1080 depth image in device memory",
+                   "width": W, "height": H, "N": N, "path": "smooth"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for("smooth", args.workload)},
+        "kernel_ms": round(kms, 5),
+        "alg_bytes_per_launch": B,
+        "copy_gbps": round(m.stream_copy_gbps(1 << 28, 10), 1),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_smooth(depth, N, args.cpu_budget)
+    m.close()
+    print(json.dumps(out))
+
+
 def bench_batch(args):
     """64 views x 16 frames in one persistent launch from a hipGraph."""
     import pmarch
@@ -330,7 +421,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["dam_break_1080p", "clustered_4k", "batch", "aniso_dam_1080p"],
+    ap.add_argument("--workload", choices=["dam_break_1080p", "clustered_4k", "batch", "aniso_dam_1080p",
+                                           "depth_smooth_1080p"],
                     default="dam_break_1080p")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -345,6 +437,9 @@ def main():
         return
     if args.workload == "aniso_dam_1080p":
         bench_aniso(args)
+        return
+    if args.workload == "depth_smooth_1080p":
+        bench_smooth(args)
         return
 
     import pmarch

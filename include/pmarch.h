@@ -254,12 +254,22 @@ int pm_aniso_matrix(pm_ctx *ctx, int32_t frame, const pm_settings *s, const floa
  * exp(-r^2 / 2) (r = sqrt(i^2 + j^2), powf as the reference evaluates it)
  * divided by its sum over the (2N + 1)^2 window; host code.
  * pm_depth_smooth: out(x, y) = sum over i, j in [-N, N] (i horizontal, outer)
- * of K(|i|, |j|) depth(x + i, y + j), coordinates clamped to the image (the
+ * of K(|i|, |j|) depth(x + i, y + j), each term added with one fused
+ * multiply-add (gauss.frag's `sum += k * t` may be fused by the shader
+ * compiler), coordinates clamped to the image (the
  * depth texture's clamp-to-edge sampler, BilateralBuffer.cpp:246-248) at
  * texel centres (Spread = 1).  Host images, W*H floats, row-major; 0 <= N <= 31
  * (the reference's 32 x 32 table). */
 int pm_gauss_kernel(int32_t N, float *K);
 int pm_depth_smooth(pm_ctx *ctx, const float *depth, uint32_t W, uint32_t H, int32_t N, float *out);
+/* pm_depth_smooth_device: the same pass on device memory (the sprite depth
+ * image the application already holds on the GPU, as GaussRenderPass reads its
+ * attachment): d_depth and d_out are W*H floats of the context's GPU, not
+ * overlapping; the kernel is enqueued on `stream` (a hipStream_t, NULL for the
+ * null stream) and the call returns without waiting.  The buffers must stay
+ * valid, and the context alive, until that stream has run it. */
+int pm_depth_smooth_device(pm_ctx *ctx, const float *d_depth, uint32_t W, uint32_t H, int32_t N, float *d_out,
+                           void *stream);
 
 /* ---- learned step proposal (no counterpart in the reference; it replaces the
  * role of the DensityGrid skip, Dataset.cpp:94-164, RayMarcher.cpp:284-304) ----

@@ -1548,7 +1548,9 @@ int or_march_f64(const or_grid *g, const or_settings *s, const float invpv[16], 
 /* ------------------------------------------------------------------------- */
 /* Screen-space depth smoothing -- GaussRenderPass: ComputeGaussKernel        */
 /* (GaussRenderPass.cpp:8-48) and gauss.frag's window sum, clamp-to-edge      */
-/* texel-centre taps (BilateralBuffer.cpp:246-248, Spread = 1).               */
+/* texel-centre taps (BilateralBuffer.cpp:246-248, Spread = 1); each term is  */
+/* accumulated with one fused multiply-add (`sum += k * t` in gauss.frag has  */
+/* no `precise` qualifier, so the shader compiler may fuse it).               */
 /* ------------------------------------------------------------------------- */
 
 int or_gauss_kernel(int32_t N, float *K)
@@ -1582,7 +1584,7 @@ int or_depth_smooth(const float *depth, uint32_t W, uint32_t H, int32_t N, float
                 for (int j = -N; j <= N; j++) {
                     int gy = (int)y + j;
                     gy = gy < 0 ? 0 : (gy > (int)H - 1 ? (int)H - 1 : gy);
-                    s += K[(i < 0 ? -i : i) * (N + 1) + (j < 0 ? -j : j)] * depth[(size_t)gy * W + (size_t)gx];
+                    s = fmaf(K[(i < 0 ? -i : i) * (N + 1) + (j < 0 ? -j : j)], depth[(size_t)gy * W + (size_t)gx], s);
                 }
             }
             out[(size_t)y * W + x] = s;

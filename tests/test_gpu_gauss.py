@@ -1,6 +1,7 @@
 """GPU parity of the screen-space depth smoothing (GaussRenderPass,
-gauss.frag): the HIP kernel's images equal the CPU oracle's bit for bit on
-ragged image sizes and window sizes 0, 1, 8 (the reference's default) and 31
+gauss.frag): the HIP kernels' images equal the CPU oracle's bit for bit on
+ragged image sizes (images smaller than one tile included) and window sizes
+0, 1, 7, 8 (the reference's default, which has a kernel of its own) and 31
 (its largest), with the border clamped."""
 import time
 
@@ -12,7 +13,8 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("W,H,N", [(203, 77, 8), (64, 48, 0), (64, 48, 1), (97, 131, 31), (640, 480, 8)])
+@pytest.mark.parametrize("W,H,N", [(203, 77, 8), (64, 48, 0), (64, 48, 1), (97, 131, 31), (640, 480, 8),
+                                   (5, 3, 8), (1, 1, 8), (33, 40, 7), (1920, 1080, 8)])
 def test_depth_smooth_bitexact(marcher, W, H, N):
     rng = np.random.default_rng(W * 7 + N)
     depth = rng.uniform(0.2, 1.0, (H, W)).astype(np.float32)
