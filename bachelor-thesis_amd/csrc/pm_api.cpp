@@ -70,6 +70,8 @@ struct pm_ctx {
     int32_t pending_frame = 0;            // frame of the march in flight
     bool pending_batch = false;           // the march in flight is a batch (all its frames)
     bool outputs_on_device = false;       // the last march wrote the caller's device memory
+    bool bound_used = false;              // a march has used the bound skip ...
+    float bound_iso_last = 0.0f;          // ... with this iso (its bits' key)
     hipEvent_t ev_launch = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
     std::map<int32_t, Frame *> frames;
     std::string err;
@@ -375,6 +377,8 @@ int pm_set_frame(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, float h
     return build_frame(c, frame, (const float *)f->xyz.p, n, h, c->stream);
 }
 
+static int ensure_bound(pm_ctx *c, Frame *f, float iso, hipStream_t st);
+
 int pm_set_frame_stream(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, float h)
 {
     if (!c) return PM_ERR_ARG;
@@ -387,7 +391,16 @@ int pm_set_frame_stream(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, 
     Frame *f = it == c->frames.end() ? (c->frames[frame] = new Frame()) : it->second;
     HIPCHK(c, f->xyz.ensure(sizeof(float) * 3 * n));
     HIPCHK(c, hipMemcpyAsync(f->xyz.p, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream_build));
-    return build_frame(c, frame, (const float *)f->xyz.p, n, h, c->stream_build);
+    int rc = build_frame(c, frame, (const float *)f->xyz.p, n, h, c->stream_build);
+    if (rc || !c->bound_used) return rc;
+    // Autoplay marches every frame with the same settings: the skip's bits for
+    // the iso of the last bound-skip march are built here too, beside the march
+    // in flight, instead of at the start of this frame's march.  A march with
+    // another iso rebuilds them (the bits are keyed by iso).
+    rc = ensure_bound(c, f, c->bound_iso_last, c->stream_build);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream_build));
+    return PM_OK;
 }
 
 int pm_load_bgeo(pm_ctx *c, int32_t frame, const char *path, float h)
@@ -467,12 +480,12 @@ static int ensure_densitygrid(pm_ctx *c, Frame *f)
 
 // Sub-box density bounds of a frame for one iso value, built on the context
 // stream when first needed (a batch holding the old bits is invalidated).
-static int ensure_bound(pm_ctx *c, Frame *f, float iso)
+static int ensure_bound(pm_ctx *c, Frame *f, float iso, hipStream_t st)
 {
     if (f->bound_ok && f->bound_iso == iso) return PM_OK;
     HIPCHK(c, f->bound.ensure(sizeof(uint64_t) * f->info.ncells));
     f->g.bound = (const uint64_t *)f->bound.p;
-    HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, c->stream));
+    HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, st));
     f->bound_ok = true;
     f->bound_iso = iso;
     c->frame_epoch++;
@@ -574,8 +587,11 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
     if (s->skip_mode == PM_SKIP_BOUND) {
         // the anisotropic density is exactly 0 where no particle lies within h:
         // the bound bits for PM_ZERO_ISO mark those sub-boxes
-        rc = ensure_bound(c, f, s->anisotropic ? PM_ZERO_ISO : s->iso);
+        const float biso = s->anisotropic ? PM_ZERO_ISO : s->iso;
+        rc = ensure_bound(c, f, biso, c->stream);
         if (rc) return rc;
+        c->bound_iso_last = biso;
+        c->bound_used = true;
     }
     if (s->skip_mode == PM_SKIP_DG_BOX || s->skip_mode == PM_SKIP_DG_CONS) {
         rc = ensure_densitygrid(c, f);
@@ -797,7 +813,7 @@ int pm_get_bound_grid(pm_ctx *c, int32_t frame, float iso, uint64_t *bits)
     Frame *f = get_frame(c, frame);
     if (!f) return fail(c, PM_ERR_FRAME, "frame %d not set", frame);
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_bound(c, f, iso);
+    int rc = ensure_bound(c, f, iso, c->stream);
     if (rc) return rc;
     HIPCHK(c, copy_sync(c, bits, f->bound.p, sizeof(uint64_t) * f->info.ncells, hipMemcpyDeviceToHost));
     return PM_OK;
@@ -830,7 +846,7 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
         int rc = validate_settings(c, s, f);
         if (rc) return rc;
         if (s->skip_mode == PM_SKIP_BOUND) {
-            rc = ensure_bound(c, f, s->iso);
+            rc = ensure_bound(c, f, s->iso, c->stream);
             if (rc) return rc;
         }
         c->b_off[v] = npx;

@@ -255,7 +255,9 @@ hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz
 // positions), U = sum over the cell's 27-cell stencil of W(dist(x_j, B)) bounds
 // the density at every point of the sub-box, since W falls with r and the
 // stencil holds every particle within h.  Bit l of bound[c] is set when
-// U < iso (1 - 2^-9); the margin covers the rounding of both sums.  Border
+// U < iso (1 - 2^-9); the margin covers the rounding of both sums.  U sums the
+// centre row (dz = dy = 0) first, then the other eight in (dz, dy) order,
+// particles in storage order.  Border
 // cells (no stencil, zero density) and cells with an empty 27-neighbourhood get
 // every bit.  One wave per cell: lane l evaluates sub-box l, all lanes read
 // the same particle, and the walk stops once every lane has reached the limit.
@@ -270,17 +272,12 @@ __device__ __forceinline__ float bound_term(const GridDev &g, float4 p, float lx
     return d2 < g.h2 ? pm_W_in(d2, g.inv_h, g.sig) : 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64_t *__restrict__ out)
+// The bits of cell c (one wave; returns the mask for lane 0 to store).
+__device__ __forceinline__ uint64_t bound_cell(const GridDev &g, float iso, uint32_t c, uint32_t lane)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny, nz = (uint32_t)g.nz;
-    if (c >= nx * ny * nz) return;   // whole wave
+    const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
     const int32_t x = (int32_t)(c % nx), y = (int32_t)((c / nx) % ny), z = (int32_t)(c / (nx * ny));
-    if (x < 1 || y < 1 || z < 1 || x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2 || g.empty27[c]) {
-        if (lane == 0) out[c] = ~0ull;
-        return;
-    }
+    if (x < 1 || y < 1 || z < 1 || x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2 || g.empty27[c]) return ~0ull;
     const int32_t kx = x + g.ox, ky = y + g.oy, kz = z + g.oz;
     int32_t km = abs(kx) > abs(ky) ? abs(kx) : abs(ky);
     km = (km > abs(kz) ? km : abs(kz)) + 1;
@@ -291,8 +288,10 @@ __global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64
     const float lz = ((float)kz + 0.25f * sz) * g.h - delta, hz = ((float)kz + 0.25f * (sz + 1.0f)) * g.h + delta;
     const float lim = iso * (1.0f - 0x1p-9f);
     float U = 0.0f;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++) {
+    // the centre row first: inside the fluid it alone takes every sub-box
+    // past the limit, and the walk stops there
+    for (int r = 0; r < 9; r++) {
+            const int dz = r == 0 ? 0 : (r - 1 + (r > 4)) / 3 - 1, dy = r == 0 ? 0 : (r - 1 + (r > 4)) % 3 - 1;
             const uint32_t row = (uint32_t)x - 1u + nx * ((uint32_t)(y + dy) + ny * (uint32_t)(z + dz));
             const uint32_t b = g.cell_start[row], e = g.cell_start[row + 3];
             // all lanes read the same particles (broadcast); 8 loads in flight
@@ -305,12 +304,18 @@ __global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64
                 for (int u = 0; u < 8; u++) U += bound_term(g, p[u], lx, hx, ly, hy, lz, hz);
             }
             for (; j < e; j++) U += bound_term(g, g.sorted[j], lx, hx, ly, hy, lz, hz);
-            if (__ballot(U < lim) == 0) {
-                if (lane == 0) out[c] = 0ull;
-                return;
-            }
+            if (__ballot(U < lim) == 0) return 0ull;
         }
-    const unsigned long long m = __ballot(U < lim);
+    return __ballot(U < lim);
+}
+
+// One wave per cell.
+__global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64_t *__restrict__ out)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= (uint32_t)g.nx * (uint32_t)g.ny * (uint32_t)g.nz) return;   // whole wave
+    const uint64_t m = bound_cell(g, iso, c, lane);
     if (lane == 0) out[c] = m;
 }
 

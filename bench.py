@@ -31,6 +31,13 @@ particle radius its k_s = 2000 is tuned for), 1920x1080, IsoDensity 1.0,
 step h/8, 8 bisections (last in FP64), central-difference normals, exact skip of
 the sub-boxes with no particle within h (the density is 0 there).
 
+--workload autoplay_dam_1080p: the reference's autoplay, a new frame for every
+march: 16 consecutive frames of the 1M dam break (1 % displacement per frame)
+played through two frame slots, each frame's upload and grid build on the
+second stream while the previous frame is marched (pm_set_frame_stream), and
+the same frames played serially (set, then march) for comparison; one step =
+one play of the 16 frames.
+
 --workload depth_smooth_1080p: the screen-space depth smoothing
 (GaussRenderPass, N = 8: a 17x17 window) of a 1920x1080 depth image resident
 in device memory; one step = one pass over the image.
@@ -246,6 +253,85 @@ def bench_aniso(args):
     print(json.dumps(out))
 
 
+def bench_autoplay(args):
+    """16 frames of the 1M dam break, each marched once at 1080p while the next
+    one streams in, against the serial order."""
+    import pmarch
+    from pmarch import scenes
+
+    base = scenes.dam_break(0)
+    nf = 16
+    frames = scenes.displaced_frames(base, nf, seed=1)
+    cam = base.cameras[0]
+    W, H = cam.width, cam.height
+    st = base.settings(skip_mode=4)
+    m = pmarch.Marcher(0)
+    kms = []
+
+    def march(slot):
+        m.march_async(slot, st, cam.inv_pv, cam.eye, W, H, outputs=False)
+
+    def serial():
+        for i in range(nf):
+            m.set_frame(i % 2, frames[i], base.h)
+            march(i % 2)
+            m.wait()
+
+    def streamed():
+        m.set_frame(0, frames[0], base.h)
+        for i in range(nf):
+            march(i % 2)
+            if i + 1 < nf:
+                m.set_frame_stream((i + 1) % 2, frames[i + 1], base.h)
+            m.wait()
+            kms.append(m.stats()["kernel_ms"])
+
+    def timed(play):
+        for _ in range(args.warmup):
+            play()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            play()
+        return time.perf_counter() - t0
+
+    wall_serial = timed(serial)
+    kms.clear()
+    wall = timed(streamed)
+    stats = m.stats()
+    km = float(np.mean(kms))
+    B = alg_bytes(stats)
+    achieved = B / (km * 1e-3) / 1e9
+    n = args.steps * nf
+    out = {
+        "metric": "Mrays/s @1080p/1M particles, a new frame for every march (autoplay)",
+        "value": round(n * W * H / wall / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "autoplay: 16 consecutive frames of the 1,024,000-particle dam break (1 % "
+                               "displacement per frame), each uploaded from host memory, built and marched once "
+                               "at 1920x1080 (configured march, exact sub-box bound skip); the next frame streams "
+                               "in on the second stream during each march",
+                   "particles": base.n, "frames": nf, "width": W, "height": H, "path": "autoplay"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for("plain", "dam_break_1080p")},
+        "ms_per_frame": round(wall / n * 1e3, 4),
+        "serial_ms_per_frame": round(wall_serial / n * 1e3, 4),
+        "overlap_speedup": round(wall_serial / wall, 3),
+        "march_kernel_ms": round(km, 4),
+        "march_counters": {k: stats[k] for k in ("rays", "rays_in_aabb", "hits", "samples", "cand")},
+    }
+    m.close()
+    print(json.dumps(out))
+
+
 def smooth_depth_image(W: int, H: int, seed: int = 0) -> np.ndarray:
     """A sprite-style depth image: 1.0 (nothing drawn) outside a disc of smooth
     depth values with per-pixel noise; the pass has no data-dependent work, so
@@ -422,7 +508,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["dam_break_1080p", "clustered_4k", "batch", "aniso_dam_1080p",
-                                           "depth_smooth_1080p"],
+                                           "depth_smooth_1080p", "autoplay_dam_1080p"],
                     default="dam_break_1080p")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -440,6 +526,9 @@ def main():
         return
     if args.workload == "depth_smooth_1080p":
         bench_smooth(args)
+        return
+    if args.workload == "autoplay_dam_1080p":
+        bench_autoplay(args)
         return
 
     import pmarch

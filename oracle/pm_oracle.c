@@ -697,8 +697,10 @@ static int64_t skip_empty(const or_grid *g, const float o[3], const float d[3], 
  * U = sum over the cell's 27-cell stencil of W(dist(x_j, B)) bounds the density
  * of every point inside (W falls with r; the stencil holds every particle
  * within h).  The bit is set when U < iso (1 - 2^-9).  Border cells and cells
- * with an empty 27-neighbourhood get every bit.  U is summed in stencil row
- * order (dz, dy outer; particles in storage order), as the GPU build does. */
+ * with an empty 27-neighbourhood get every bit.  U sums the centre row
+ * (dz = dy = 0) first, then the other eight rows in (dz, dy) order, particles
+ * in storage order, as the GPU build does (it stops a cell's walk once every
+ * sub-box is past the limit; adding terms >= 0 never lowers U again). */
 int or_bound_build(or_grid *g, float iso)
 {
     free(g->bound);
@@ -726,9 +728,11 @@ int or_bound_build(or_grid *g, float iso)
                         lo[a] = ((float)k[a] + 0.25f * sub[a]) * h - delta;
                         hi[a] = ((float)k[a] + 0.25f * (sub[a] + 1.0f)) * h + delta;
                     }
+                    static const int8_t order[9][2] = { { 0, 0 }, { -1, -1 }, { -1, 0 }, { -1, 1 }, { 0, -1 },
+                                                        { 0, 1 }, { 1, -1 }, { 1, 0 }, { 1, 1 } };
                     float U = 0.0f;
-                    for (int dz = -1; dz <= 1 && U < lim; dz++)
-                        for (int dy = -1; dy <= 1 && U < lim; dy++) {
+                    for (int r = 0; r < 9 && U < lim; r++) {
+                            const int dz = order[r][0], dy = order[r][1];
                             uint64_t row = cell_lin(g, x - 1, y + dy, z + dz);
                             for (uint32_t j = g->cell_start[row]; j < g->cell_start[row + 3]; j++) {
                                 const float *p = g->sorted + 4 * (uint64_t)j;

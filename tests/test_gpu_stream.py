@@ -68,7 +68,9 @@ def test_stream_frame_beside_march(marcher):
 
 def test_autoplay_double_buffer(marcher):
     """Frames 0..3 through two frame slots: march slot i % 2 while the next
-    frame streams into the other; every image equals the frame's own march."""
+    frame streams into the other (with the bound skip's bits for the last
+    march's iso built beside it too); every image and counter equals the
+    frame's own march, and a march with another iso (bits rebuilt) as well."""
     base = scenes.dam_break(1, nx=40, ny=80, nz=40, width=640, height=360)
     frames = scenes.displaced_frames(base, 4, seed=4)
     cam = base.cameras[0]
@@ -82,9 +84,18 @@ def test_autoplay_double_buffer(marcher):
         if i + 1 < 4:
             marcher.set_frame_stream(10 + ((i + 1) % 2), frames[i + 1], base.h)
         marcher.wait()
-        imgs.append((pos, nrm, t))
+        imgs.append((pos, nrm, t, marcher.stats()))
+    keys = ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples")
     for i in range(4):
         marcher.set_frame(12, frames[i], base.h)
         want = marcher.march(12, st, cam.inv_pv, cam.eye, cam.width, cam.height)
-        for x, y in zip(imgs[i], want[:3]):
+        for x, y in zip(imgs[i][:3], want[:3]):
             assert np.array_equal(x, y), i
+        assert all(imgs[i][3][k] == want[3][k] for k in keys), i
+    # slot 11 holds frame 3 with the bits built for st's iso; another iso rebuilds them
+    st2 = base.settings(skip_mode=4, iso=base.iso * 1.5)
+    got = marcher.march(11, st2, cam.inv_pv, cam.eye, cam.width, cam.height)
+    want = marcher.march(12, st2, cam.inv_pv, cam.eye, cam.width, cam.height)
+    for x, y in zip(got[:3], want[:3]):
+        assert np.array_equal(x, y)
+    assert all(got[3][k] == want[3][k] for k in keys)
