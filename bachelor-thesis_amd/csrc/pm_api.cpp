@@ -1230,6 +1230,36 @@ int pm_depth_smooth_device(pm_ctx *c, const float *d_depth, uint32_t W, uint32_t
     return PM_OK;
 }
 
+int pm_depth_sprites(pm_ctx *c, int32_t frame, const float view[16], const float proj[16], const float inv_pv[16],
+                     const float cam[3], float radius, uint32_t W, uint32_t H, float *depth)
+{
+    if (!c) return PM_ERR_ARG;
+    if (!view || !proj || !inv_pv || !cam || !depth) return fail(c, PM_ERR_ARG, "pm_depth_sprites: null argument");
+    if (!(radius > 0.0f) || !std::isfinite(radius)) return fail(c, PM_ERR_ARG, "pm_depth_sprites: radius must be finite and > 0");
+    if (W == 0 || H == 0 || (uint64_t)W * H > (1ull << 30)) return fail(c, PM_ERR_SIZE, "bad image size %ux%u", W, H);
+    if (c->pending) return fail(c, PM_ERR_STATE, "pm_depth_sprites: a march is in flight");
+    Frame *f = get_frame(c, frame);
+    if (!f) return fail(c, PM_ERR_FRAME, "frame %d not set", frame);
+    SpriteLaunch S{};
+    for (int k = 0; k < 16; k++) {
+        S.view[k] = view[k]; S.proj[k] = proj[k]; S.inv_pv[k] = inv_pv[k];
+        if (!std::isfinite(view[k]) || !std::isfinite(proj[k]) || !std::isfinite(inv_pv[k]))
+            return fail(c, PM_ERR_ARG, "pm_depth_sprites: non-finite matrix entry");
+    }
+    for (int k = 0; k < 3; k++) S.cam[k] = cam[k];
+    S.radius = radius;
+    S.W = W;
+    S.H = H;
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint64_t npx = (uint64_t)W * H;
+    HIPCHK(c, c->d_out.ensure(4 * npx));
+    HIPCHK(c, launch_depth_sprites(S, (const float4 *)f->sorted.p, (uint32_t)f->info.n, (uint32_t *)c->d_out.p,
+                                   c->stream));
+    HIPCHK(c, hipMemcpyAsync(depth, c->d_out.p, 4 * npx, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PM_OK;
+}
+
 // ---------------------------------------------------------------- learned --
 
 int pm_mlp_set(pm_ctx *c, const float *params)

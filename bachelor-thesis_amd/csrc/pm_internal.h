@@ -137,6 +137,14 @@ hipError_t launch_neighbor_counts(const GridDev &g, const float *q, uint32_t nq,
 
 hipError_t launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 // Gaussian depth smoothing (GaussRenderPass): K = the (N + 1)^2 table of pm_gauss_kernel.
+// The depth-sprite pass: View, Projection (column-major), the march's inverse
+// projection-view and camera for the pixel rays, the particle radius.
+struct SpriteLaunch {
+    float view[16], proj[16], inv_pv[16], cam[3];
+    float radius;
+    uint32_t W, H;
+};
+hipError_t launch_depth_sprites(const SpriteLaunch &S, const float4 *pts, uint32_t n, uint32_t *depth, hipStream_t st);
 hipError_t launch_depth_smooth(const float *in, float *out, uint32_t W, uint32_t H, int N, const float *K,
                                hipStream_t st);
 hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz, uint8_t *out, hipStream_t st);

@@ -105,6 +105,7 @@ def load():
     L.pm_march_async_device.argtypes = [vp, i32, C.POINTER(PmSettings), vp, vp, u32, u32, vp, vp, vp, vp]
     L.pm_depth_smooth.argtypes = [vp, vp, u32, u32, i32, vp]
     L.pm_depth_smooth_device.argtypes = [vp, vp, u32, u32, i32, vp, vp]
+    L.pm_depth_sprites.argtypes = [vp, i32, vp, vp, vp, vp, f32, u32, u32, vp]
     L.pm_load_bgeo.argtypes = [vp, i32, C.c_char_p, f32]
     L.pm_set_frame_stream.argtypes = [vp, i32, vp, u64, f32]
     L.pm_stream_copy_gbps.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
@@ -215,6 +216,15 @@ class Marcher:
         H, W = depth.shape
         out = np.zeros_like(depth)
         self._check(self.L.pm_depth_smooth(self.ctx, _ptr(depth), W, H, int(N), _ptr(out)), "pm_depth_smooth")
+        return out
+
+    def depth_sprites(self, frame: int, cam, radius: float):
+        """pm_depth_sprites: the sprite depth image (H x W, 1.0 where nothing is
+        drawn) of `frame` seen by `cam` (a pmarch.camera.Camera)."""
+        m = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (cam.view, cam.proj, cam.inv_pv, cam.eye)]
+        out = np.zeros((cam.height, cam.width), np.float32)
+        self._check(self.L.pm_depth_sprites(self.ctx, frame, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]), _ptr(m[3]),
+                                            float(radius), cam.width, cam.height, _ptr(out)), "pm_depth_sprites")
         return out
 
     def depth_smooth_device(self, d_depth: int, d_out: int, W: int, H: int, N: int = 8, stream: int = 0):

@@ -55,6 +55,8 @@ class Camera:
     fovy: float
     near: float
     far: float
+    view: np.ndarray = None  # float32 [16], column-major View (the depth pass's uniform)
+    proj: np.ndarray = None  # float32 [16], column-major Projection
 
     @property
     def inv_pv_colmajor(self) -> np.ndarray:
@@ -70,7 +72,8 @@ def make_camera(eye, target, width: int, height: int, fovy_deg: float = 60.0, ne
     PV = P @ V
     inv = np.linalg.inv(PV)
     inv_cm = np.ascontiguousarray(inv.T.reshape(16), dtype=np.float32)  # column-major
-    return Camera(np.asarray(eye, np.float32), inv_cm, PV, width, height, fovy_deg, near, far)
+    cm = lambda m: np.ascontiguousarray(m.T.reshape(16), dtype=np.float32)
+    return Camera(np.asarray(eye, np.float32), inv_cm, PV, width, height, fovy_deg, near, far, cm(V), cm(P))
 
 
 def orbit_eye(target, radius: float, elevation_deg: float, azimuth_deg: float) -> np.ndarray:

@@ -38,6 +38,11 @@ second stream while the previous frame is marched (pm_set_frame_stream), and
 the same frames played serially (set, then march) for comparison; one step =
 one play of the 16 frames.
 
+--workload sprites_dam_1080p: the depth-sprite pass (DepthRenderPass: every
+particle of the 1M dam break as a sphere sprite of radius h, depth test LESS)
+for the 1080p view, host image out; one step = one pass, wall clock including
+the image's copy to the host (the kernel alone is in the rocprof summary).
+
 --workload depth_smooth_1080p: the screen-space depth smoothing
 (GaussRenderPass, N = 8: a 17x17 window) of a 1920x1080 depth image resident
 in device memory; one step = one pass over the image.
@@ -332,6 +337,44 @@ def bench_autoplay(args):
     print(json.dumps(out))
 
 
+def bench_sprites(args):
+    """The depth-sprite pass of the 1M dam break at 1080p."""
+    import pmarch
+    from pmarch import scenes
+
+    sc = scenes.dam_break(0)
+    cam = sc.cameras[0]
+    m = pmarch.Marcher(0)
+    m.set_frame(0, sc.xyz, sc.h)
+    for _ in range(args.warmup):
+        d = m.depth_sprites(0, cam, sc.h)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d = m.depth_sprites(0, cam, sc.h)
+    wall = time.perf_counter() - t0
+    out = {
+        "metric": "depth sprites Mparticles/s @1080p/1M particles (DepthRenderPass), host image out",
+        "value": round(args.steps * sc.n / wall / 1e6, 3),
+        "unit": "Mparticles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "depth sprites (radius h, camera-facing quads, depth test LESS) of the "
+                               "1,024,000-particle dam break, 1920x1080 view, image copied to the host",
+                   "particles": sc.n, "width": cam.width, "height": cam.height, "path": "sprites"},
+        "roofline": None,
+        "pixels_drawn": int((d < 1.0).sum()),
+    }
+    m.close()
+    print(json.dumps(out))
+
+
 def smooth_depth_image(W: int, H: int, seed: int = 0) -> np.ndarray:
     """A sprite-style depth image: 1.0 (nothing drawn) outside a disc of smooth
     depth values with per-pixel noise; the pass has no data-dependent work, so
@@ -508,7 +551,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["dam_break_1080p", "clustered_4k", "batch", "aniso_dam_1080p",
-                                           "depth_smooth_1080p", "autoplay_dam_1080p"],
+                                           "depth_smooth_1080p", "autoplay_dam_1080p", "sprites_dam_1080p"],
                     default="dam_break_1080p")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -529,6 +572,9 @@ def main():
         return
     if args.workload == "autoplay_dam_1080p":
         bench_autoplay(args)
+        return
+    if args.workload == "sprites_dam_1080p":
+        bench_sprites(args)
         return
 
     import pmarch

@@ -271,6 +271,22 @@ int pm_depth_smooth(pm_ctx *ctx, const float *depth, uint32_t W, uint32_t H, int
 int pm_depth_smooth_device(pm_ctx *ctx, const float *d_depth, uint32_t W, uint32_t H, int32_t N, float *d_out,
                            void *stream);
 
+/* ---- depth sprites: the reference march's ray starts (DepthRenderPass.cpp:
+ * 243-250; sprite quads AdvancedRenderer.cpp:455-474; depth.vert, depth.frag;
+ * depth test LESS, clear 1.0) ----
+ * pm_depth_sprites draws every particle of `frame` as a camera-facing quad
+ * p + R (+-x +-y) (x, y: the first two rows of View) and writes, per pixel, the
+ * least depth.frag depth: at the pixel centre's ray through the quad's plane,
+ * UV = ((P - p).x, -(P - p).y) / R, discarded when |UV|^2 > 1, else the
+ * projected z of View P - R (0, 0, cos(pi/2 |UV|)).  Fragments whose plane
+ * depth is outside [0, 1] are clipped, depths are clamped to [0, 1], and a
+ * particle with a quad corner behind the camera is not drawn.  view and proj
+ * are View and Projection (column-major, as the uniforms hold them), inv_pv and
+ * cam the march's; depth is W*H floats (host), 1.0 where nothing is drawn:
+ * the depth_in of a reference-mode march. */
+int pm_depth_sprites(pm_ctx *ctx, int32_t frame, const float view[16], const float proj[16], const float inv_pv[16],
+                     const float cam[3], float radius, uint32_t W, uint32_t H, float *depth);
+
 /* ---- learned step proposal (no counterpart in the reference; it replaces the
  * role of the DensityGrid skip, Dataset.cpp:94-164, RayMarcher.cpp:284-304) ----
  * A 24 -> 64 -> 64 -> 1 MLP, BF16 weights with FP32 accumulation on MFMA,

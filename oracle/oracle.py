@@ -166,6 +166,10 @@ def lib():
         L.or_neighbors_ext.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), C.c_float,
                                        fp(np.float32, flags="C"), C.c_uint32, C.POINTER(C.c_uint32)]
         L.or_gauss_kernel.argtypes = [C.c_int32, fp(np.float32, flags="C")]
+        L.or_cos_half_pi.argtypes = [C.c_float]
+        L.or_cos_half_pi.restype = C.c_float
+        L.or_depth_sprites.argtypes = [fp(np.float32, flags="C"), C.c_uint64] + [fp(np.float32, flags="C")] * 4 + \
+            [C.c_float, C.c_uint32, C.c_uint32, fp(np.float32, flags="C")]
         L.or_depth_smooth.argtypes = [fp(np.float32, flags="C"), C.c_uint32, C.c_uint32, C.c_int32,
                                       fp(np.float32, flags="C")]
         L.or_ray.argtypes = [fp(np.float32, flags="C"), fp(np.float32, flags="C"), C.c_uint32, C.c_uint32,
@@ -376,6 +380,23 @@ def gauss_kernel(N: int) -> np.ndarray:
     K = np.zeros((N + 1) * (N + 1), np.float32)
     assert lib().or_gauss_kernel(N, K) == 0
     return K.reshape(N + 1, N + 1)
+
+
+def cos_half_pi(s) -> np.ndarray:
+    """or_cos_half_pi on every element: cos(pi/2 s) as the sprite pass evaluates it."""
+    s = np.ascontiguousarray(s, np.float32).reshape(-1)
+    return np.array([lib().or_cos_half_pi(float(v)) for v in s], np.float32)
+
+
+def depth_sprites(xyz, cam, radius: float) -> np.ndarray:
+    """or_depth_sprites: the sprite depth image (H x W) of the particles xyz seen
+    by cam (view, proj, inv_pv, eye as float32)."""
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    m = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (cam.view, cam.proj, cam.inv_pv, cam.eye)]
+    out = np.zeros((cam.height, cam.width), np.float32)
+    assert lib().or_depth_sprites(xyz.reshape(-1), xyz.shape[0], m[0], m[1], m[2], m[3], float(radius),
+                                  cam.width, cam.height, out.reshape(-1)) == 0
+    return out
 
 
 def depth_smooth(depth, N: int) -> np.ndarray:

@@ -1595,3 +1595,103 @@ int or_depth_smooth(const float *depth, uint32_t W, uint32_t H, int32_t N, float
         }
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Depth sprites -- DepthRenderPass (DepthRenderPass.cpp:243-250), the sprite */
+/* quads of AdvancedRenderer::CollectRenderData (AdvancedRenderer.cpp:455-474)*/
+/* and depth.vert / depth.frag, depth test LESS against a clear of 1.0: every */
+/* particle a camera-facing quad p + R (+-x +-y), x and y the first two rows  */
+/* of View (CameraController3D.cpp:80).  At the centre of pixel (px, py) the  */
+/* pixel's ray meets the quad's plane at P; UV = ((P - p).x, -(P - p).y) / R  */
+/* (the perspective-correct interpolation of the vertex UVs over the planar   */
+/* quad); discard when |UV|^2 > 1; depth = projected z of                     */
+/* View P - R (0, 0, cos(pi/2 |UV|)).  Plane depths outside [0, 1] are        */
+/* clipped, written depths clamped to [0, 1]; a particle with a quad corner   */
+/* behind the camera (w <= 0) is not drawn.  Each pixel keeps the least depth.*/
+/* cos(pi/2 s) is its Taylor polynomial through s^12.                         */
+/* ------------------------------------------------------------------------- */
+
+static void sp_xform(const float *M, float x, float y, float z, float o[4])
+{
+    for (int r = 0; r < 4; r++) o[r] = (M[r] * x + M[4 + r] * y) + (M[8 + r] * z + M[12 + r] * 1.0f);
+}
+
+float or_cos_half_pi(float s)
+{
+    const float t = 1.57079632679f * s, z = t * t;
+    float c = 1.0f / 479001600.0f;
+    c = c * z + -1.0f / 3628800.0f;
+    c = c * z + 1.0f / 40320.0f;
+    c = c * z + -1.0f / 720.0f;
+    c = c * z + 1.0f / 24.0f;
+    c = c * z + -0.5f;
+    return c * z + 1.0f;
+}
+
+int or_depth_sprites(const float *xyz, uint64_t n, const float view[16], const float proj[16], const float inv_pv[16],
+                     const float cam[3], float R, uint32_t W, uint32_t H, float *depth)
+{
+    if (!view || !proj || !inv_pv || !cam || !depth || W == 0 || H == 0 || !(R > 0.0f)) return -1;
+    uint32_t *bits = (uint32_t *)depth;
+    for (uint64_t k = 0; k < (uint64_t)W * H; k++) bits[k] = 0x3f800000u;
+    const float *V = view, *P = proj;
+    const float ax = V[0], ay = V[4], az = V[8], bx = V[1], by = V[5], bz = V[9];
+    const float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+    const float o0 = cam[0], o1 = cam[1], o2 = cam[2];
+    const float invR = 1.0f / R;
+    for (uint64_t i = 0; i < n; i++) {
+        const float qx = xyz[3 * i], qy = xyz[3 * i + 1], qz = xyz[3 * i + 2];
+        float lo_x = 3.0e38f, hi_x = -3.0e38f, lo_y = 3.0e38f, hi_y = -3.0e38f;
+        int drawn = 1;
+        for (int k = 0; k < 4; k++) {
+            const float sx = (k & 1) ? 1.0f : -1.0f, sy = (k & 2) ? 1.0f : -1.0f;
+            const float cx = qx + R * (sx * ax + sy * bx), cy = qy + R * (sx * ay + sy * by),
+                        cz = qz + R * (sx * az + sy * bz);
+            float v[4], c[4];
+            sp_xform(V, cx, cy, cz, v);
+            sp_xform(P, v[0], v[1], v[2], c);
+            if (!(c[3] > 0.0f)) { drawn = 0; break; }
+            const float fx = (c[0] / c[3] + 1.0f) * 0.5f * (float)W, fy = (c[1] / c[3] + 1.0f) * 0.5f * (float)H;
+            lo_x = fminf(lo_x, fx); hi_x = fmaxf(hi_x, fx);
+            lo_y = fminf(lo_y, fy); hi_y = fmaxf(hi_y, fy);
+        }
+        if (!drawn) continue;
+        const float x0f = fmaxf(ceilf(lo_x - 0.5f), 0.0f), x1f = fminf(floorf(hi_x - 0.5f), (float)W - 1.0f);
+        const float y0f = fmaxf(ceilf(lo_y - 0.5f), 0.0f), y1f = fminf(floorf(hi_y - 0.5f), (float)H - 1.0f);
+        if (!(x0f <= x1f && y0f <= y1f)) continue;
+        const float pn = ((qx - o0) * nx + (qy - o1) * ny) + (qz - o2) * nz;
+        for (int py = (int)y0f; py <= (int)y1f; py++)
+            for (int px = (int)x0f; px <= (int)x1f; px++) {
+                const float ndx = ((float)px + 0.5f) * (2.0f / (float)W) - 1.0f;
+                const float ndy = ((float)py + 0.5f) * (2.0f / (float)H) - 1.0f;
+                float w[4];
+                sp_xform(inv_pv, ndx, ndy, 1.0f, w);
+                const float d0 = w[0] / w[3] - o0, d1 = w[1] / w[3] - o1, d2 = w[2] / w[3] - o2;
+                const float den = (d0 * nx + d1 * ny) + d2 * nz;
+                if (den == 0.0f) continue;
+                const float t = pn / den;
+                if (!(t > 0.0f)) continue;
+                const float X = o0 + t * d0, Y = o1 + t * d1, Z = o2 + t * d2;
+                const float rx = X - qx, ry = Y - qy, rz = Z - qz;
+                const float u = ((rx * ax + ry * ay) + rz * az) * invR;
+                const float v = -(((rx * bx + ry * by) + rz * bz) * invR);
+                const float l2 = u * u + v * v;
+                if (l2 > 1.0f) continue;
+                float vp[4], cp[4], cs[4];
+                sp_xform(V, X, Y, Z, vp);
+                sp_xform(P, vp[0], vp[1], vp[2], cp);
+                const float zp = cp[2] / cp[3];
+                if (!(zp >= 0.0f && zp <= 1.0f)) continue;
+                const float off = or_cos_half_pi(sqrtf(l2));
+                sp_xform(P, vp[0], vp[1], vp[2] - R * off, cs);
+                float z = cs[2] / cs[3];
+                if (z != z) continue;
+                z = z <= 0.0f ? 0.0f : (z > 1.0f ? 1.0f : z);
+                uint32_t zb;
+                memcpy(&zb, &z, 4);
+                uint32_t *dst = &bits[(size_t)py * W + (size_t)px];
+                if (zb < *dst) *dst = zb;
+            }
+    }
+    return 0;
+}

@@ -174,6 +174,11 @@ uint32_t or_neighbors_ext(const or_grid *g, const float q[3], float hx, float *o
 int or_gauss_kernel(int32_t N, float *K);
 int or_depth_smooth(const float *depth, uint32_t W, uint32_t H, int32_t N, float *out);
 
+/* ---- depth sprites (DepthRenderPass, depth.vert/frag, AdvancedRenderer.cpp:455-474) ---- */
+float or_cos_half_pi(float s);
+int or_depth_sprites(const float *xyz, uint64_t n, const float view[16], const float proj[16], const float inv_pv[16],
+                     const float cam[3], float R, uint32_t W, uint32_t H, float *depth);
+
 /* ---- ray generation (RayMarcher.cpp:268-274) and AABB slab test ---- */
 void or_ray(const float invpv[16], const float cam[3], uint32_t W, uint32_t H, uint32_t idx,
             float o[3], float d[3]);
