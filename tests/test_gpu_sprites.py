@@ -1,6 +1,8 @@
 """GPU parity of the depth-sprite pass (DepthRenderPass, depth.vert/frag):
 the sprite depth image equals the oracle's bit for bit (the per-pixel minimum
-does not depend on the order of the particles), and the reference march
+does not depend on the order of the particles; the GPU's early depth test
+skips only atomics that cannot lower it), for the reference's perspective and
+an oblique projection, and the reference march
 started from the GPU's sprite depths (PerPixel_Isotropic's depth-buffer start,
 RayMarcher.cpp:256-344) equals the oracle's march started from its own."""
 import numpy as np
@@ -23,6 +25,22 @@ def test_depth_sprites_bitexact(marcher, cam_i):
     want = O.depth_sprites(sc.xyz, cam, sc.h)
     assert np.array_equal(got, want)
     assert 0.2 < (want < 1.0).mean() < 0.95
+
+
+def test_depth_sprites_oblique_projection(marcher):
+    """A projection whose depth also depends on x: the pass takes any View and
+    Projection, not only the reference's perspective."""
+    import dataclasses
+    sc = scenes.lattice_cube(0)
+    cam = sc.cameras[0]
+    proj = cam.proj.copy()
+    proj[2] = np.float32(0.02)                         # clip z += 0.02 x_view
+    ob = dataclasses.replace(cam, proj=proj)
+    print("depth sprites oblique projection", flush=True)
+    marcher.set_frame(0, sc.xyz, sc.h)
+    got = marcher.depth_sprites(0, ob, sc.h)
+    assert np.array_equal(got, O.depth_sprites(sc.xyz, ob, sc.h))
+    assert not np.array_equal(got, marcher.depth_sprites(0, cam, sc.h))
 
 
 def test_depth_sprites_dam_and_edges(marcher):
