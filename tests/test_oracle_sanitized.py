@@ -1,7 +1,8 @@
 """The CPU oracle (test infrastructure, plain C) built with AddressSanitizer
 and UBSan and driven through its march modes (configured and reference-style
 hashed, every skip, both normal forms, isotropic and anisotropic, FP64),
-the density queries, the extended walk and the depth smoothing, in a child
+the density queries, the extended walk, the depth smoothing and the depth
+sprites, in a child
 process with the sanitizer runtimes preloaded: no memory or undefined-
 behaviour report.  No GPU."""
 import os
@@ -32,6 +33,10 @@ SCRIPT = textwrap.dedent("""
     G.density(q); G.density64(q); G.density_aniso(O.make_settings(**scenes.aniso_settings(sc)), q)
     G.neighbors_ext(q[0], 0.2); G.cdiff_union(q[0], sc.normal_eps)
     O.depth_smooth(np.random.default_rng(0).uniform(0, 1, (20, 30)).astype(np.float32), 8)
+    O.depth_sprites(sc.xyz, cam, sc.h)
+    from pmarch.camera import make_camera
+    inside = make_camera(sc.xyz.mean(0), sc.xyz.mean(0) + [0.0, 0.0, 1.0], 31, 17)
+    O.depth_sprites(sc.xyz, inside, 4 * sc.h)           # quads behind the eye, quads past the image
     O.HashedSearch(sc.xyz, sc.h).march(G, O.make_settings(**sc.settings()), cam.inv_pv, cam.eye, 96, 72, threads=4)
     print("done")
 """)
