@@ -1666,7 +1666,10 @@ int or_depth_sprites(const float *xyz, uint64_t n, const float view[16], const f
                 const float ndy = ((float)py + 0.5f) * (2.0f / (float)H) - 1.0f;
                 float w[4];
                 sp_xform(inv_pv, ndx, ndy, 1.0f, w);
-                const float d0 = w[0] / w[3] - o0, d1 = w[1] / w[3] - o1, d2 = w[2] / w[3] - o2;
+                /* the ray direction scaled by w (no division): t d is unchanged; the
+                 * far point of a perspective projection has w = 1 / far > 0 */
+                if (!(w[3] > 0.0f)) continue;
+                const float d0 = w[0] - o0 * w[3], d1 = w[1] - o1 * w[3], d2 = w[2] - o2 * w[3];
                 const float den = (d0 * nx + d1 * ny) + d2 * nz;
                 if (den == 0.0f) continue;
                 const float t = pn / den;
