@@ -1230,11 +1230,13 @@ int pm_depth_smooth_device(pm_ctx *c, const float *d_depth, uint32_t W, uint32_t
     return PM_OK;
 }
 
-int pm_depth_sprites(pm_ctx *c, int32_t frame, const float view[16], const float proj[16], const float inv_pv[16],
-                     const float cam[3], float radius, uint32_t W, uint32_t H, float *depth)
+// d_depth != nullptr: into the caller's device memory, enqueued on the context
+// stream without a wait; else into `depth` (host) and waited for.
+static int sprites_launch(pm_ctx *c, int32_t frame, const float view[16], const float proj[16], const float inv_pv[16],
+                          const float cam[3], float radius, uint32_t W, uint32_t H, float *depth, float *d_depth)
 {
     if (!c) return PM_ERR_ARG;
-    if (!view || !proj || !inv_pv || !cam || !depth) return fail(c, PM_ERR_ARG, "pm_depth_sprites: null argument");
+    if (!view || !proj || !inv_pv || !cam || !(depth || d_depth)) return fail(c, PM_ERR_ARG, "pm_depth_sprites: null argument");
     if (!(radius > 0.0f) || !std::isfinite(radius)) return fail(c, PM_ERR_ARG, "pm_depth_sprites: radius must be finite and > 0");
     if (W == 0 || H == 0 || (uint64_t)W * H > (1ull << 30)) return fail(c, PM_ERR_SIZE, "bad image size %ux%u", W, H);
     if (c->pending) return fail(c, PM_ERR_STATE, "pm_depth_sprites: a march is in flight");
@@ -1252,12 +1254,35 @@ int pm_depth_sprites(pm_ctx *c, int32_t frame, const float view[16], const float
     S.H = H;
     HIPCHK(c, hipSetDevice(c->device));
     const uint64_t npx = (uint64_t)W * H;
+    if (d_depth) {
+        if (!device_range(c->device, d_depth, 4 * npx))
+            return fail(c, PM_ERR_ARG, "pm_depth_sprites_device: the image must be device memory of GPU %d, W*H floats",
+                        c->device);
+        HIPCHK(c, launch_depth_sprites(S, (const float4 *)f->sorted.p, (uint32_t)f->info.n, (uint32_t *)d_depth,
+                                       c->stream));
+        return PM_OK;
+    }
     HIPCHK(c, c->d_out.ensure(4 * npx));
     HIPCHK(c, launch_depth_sprites(S, (const float4 *)f->sorted.p, (uint32_t)f->info.n, (uint32_t *)c->d_out.p,
                                    c->stream));
     HIPCHK(c, hipMemcpyAsync(depth, c->d_out.p, 4 * npx, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PM_OK;
+}
+
+int pm_depth_sprites(pm_ctx *c, int32_t frame, const float view[16], const float proj[16], const float inv_pv[16],
+                     const float cam[3], float radius, uint32_t W, uint32_t H, float *depth)
+{
+    if (!depth) return c ? fail(c, PM_ERR_ARG, "pm_depth_sprites: null argument") : PM_ERR_ARG;
+    return sprites_launch(c, frame, view, proj, inv_pv, cam, radius, W, H, depth, nullptr);
+}
+
+int pm_depth_sprites_device(pm_ctx *c, int32_t frame, const float view[16], const float proj[16],
+                            const float inv_pv[16], const float cam[3], float radius, uint32_t W, uint32_t H,
+                            float *d_depth)
+{
+    if (!d_depth) return c ? fail(c, PM_ERR_ARG, "pm_depth_sprites_device: null image") : PM_ERR_ARG;
+    return sprites_launch(c, frame, view, proj, inv_pv, cam, radius, W, H, nullptr, d_depth);
 }
 
 // ---------------------------------------------------------------- learned --

@@ -106,6 +106,7 @@ def load():
     L.pm_depth_smooth.argtypes = [vp, vp, u32, u32, i32, vp]
     L.pm_depth_smooth_device.argtypes = [vp, vp, u32, u32, i32, vp, vp]
     L.pm_depth_sprites.argtypes = [vp, i32, vp, vp, vp, vp, f32, u32, u32, vp]
+    L.pm_depth_sprites_device.argtypes = [vp, i32, vp, vp, vp, vp, f32, u32, u32, vp]
     L.pm_load_bgeo.argtypes = [vp, i32, C.c_char_p, f32]
     L.pm_set_frame_stream.argtypes = [vp, i32, vp, u64, f32]
     L.pm_stream_copy_gbps.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
@@ -226,6 +227,15 @@ class Marcher:
         self._check(self.L.pm_depth_sprites(self.ctx, frame, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]), _ptr(m[3]),
                                             float(radius), cam.width, cam.height, _ptr(out)), "pm_depth_sprites")
         return out
+
+    def depth_sprites_device(self, frame: int, cam, radius: float, d_depth: int):
+        """pm_depth_sprites_device: the sprite depths into device memory
+        (integer device address), enqueued on the context stream."""
+        m = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (cam.view, cam.proj, cam.inv_pv, cam.eye)]
+        self._keep_sprites = m
+        self._check(self.L.pm_depth_sprites_device(self.ctx, frame, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]), _ptr(m[3]),
+                                                   float(radius), cam.width, cam.height, d_depth),
+                    "pm_depth_sprites_device")
 
     def depth_smooth_device(self, d_depth: int, d_out: int, W: int, H: int, N: int = 8, stream: int = 0):
         """pm_depth_smooth_device: enqueue the smoothing of device memory

@@ -286,6 +286,14 @@ int pm_depth_smooth_device(pm_ctx *ctx, const float *d_depth, uint32_t W, uint32
  * the depth_in of a reference-mode march. */
 int pm_depth_sprites(pm_ctx *ctx, int32_t frame, const float view[16], const float proj[16], const float inv_pv[16],
                      const float cam[3], float radius, uint32_t W, uint32_t H, float *depth);
+/* pm_depth_sprites_device: the same image into d_depth, W*H floats of device
+ * memory of the context's GPU, enqueued on the context's stream without a
+ * wait: a pm_march_async_device on the same context that reads d_depth runs
+ * after it (the reference pipeline with no host copy); pm_wait, or any call
+ * that waits for the context, waits for it too. */
+int pm_depth_sprites_device(pm_ctx *ctx, int32_t frame, const float view[16], const float proj[16],
+                            const float inv_pv[16], const float cam[3], float radius, uint32_t W, uint32_t H,
+                            float *d_depth);
 
 /* ---- learned step proposal (no counterpart in the reference; it replaces the
  * role of the DensityGrid skip, Dataset.cpp:94-164, RayMarcher.cpp:284-304) ----

@@ -68,3 +68,39 @@ def test_reference_march_from_sprites(marcher):
     assert np.array_equal(pos, opos)
     assert np.array_equal(nrm, onrm, equal_nan=True)
     assert stats["hits"] == ocnt["hits"] and stats["hits"] > 10000
+
+
+def test_sprites_to_march_on_device(marcher):
+    """pm_depth_sprites_device then pm_march_async_device reading those depths:
+    the reference pipeline with no host copy equals the host hand-off; a host
+    address for the depth image is refused before any launch."""
+    from pmarch import PmError
+    from test_gpu_device_out import _alloc, _not_device, _to_host
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipPointerGetAttributes.argtypes = [C.c_void_p, C.c_void_p]
+    sc = scenes.lattice_cube(2, width=203, height=77)
+    cam = sc.cameras[0]
+    W, H = cam.width, cam.height
+    print("sprites -> march on device 203x77", flush=True)
+    marcher.set_frame(0, sc.xyz, sc.h)
+    st = dict(mode=1, max_steps=128, step=0.009, iso=sc.iso, normal_mode=1, skip_mode=0, compensated=0)
+    depth = marcher.depth_sprites(0, cam, sc.h)
+    want = marcher.march(0, st, cam.inv_pv, cam.eye, W, H, depth=depth)
+    bufs = [_alloc(hip, 4 * W * H), _alloc(hip, 16 * W * H), _alloc(hip, 16 * W * H)]
+    try:
+        marcher.depth_sprites_device(0, cam, sc.h, bufs[0])
+        marcher.march_device(0, st, cam.inv_pv, cam.eye, W, H, bufs[1], bufs[2], 0, d_depth=bufs[0])
+        assert np.array_equal(_to_host(hip, bufs[0], (H, W)), depth)
+        assert np.array_equal(_to_host(hip, bufs[1], (H, W, 4)), want[0])
+        assert np.array_equal(_to_host(hip, bufs[2], (H, W, 4)), want[1], equal_nan=True)
+        host = np.zeros((H, W), np.float32)
+        assert _not_device(hip, host.ctypes.data)
+        with pytest.raises(PmError):
+            marcher.depth_sprites_device(0, cam, sc.h, host.ctypes.data)
+    finally:
+        for b in bufs:
+            hip.hipFree(b)
