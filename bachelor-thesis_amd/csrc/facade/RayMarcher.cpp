@@ -9,6 +9,7 @@
 #include "RayMarcher.h"
 
 #include <cstdio>
+#include <cstring>
 
 #include "pmarch.h"
 
@@ -23,6 +24,17 @@ RayMarcher::RayMarcher()
         std::fprintf(stderr, "RayMarcher: no GPU available for the march\n");
         m_Ctx = nullptr;
     }
+}
+
+// Every pixel a miss: (0, 0, 0, 0) in both images, as the reference leaves a
+// pixel whose ray finds no surface (RayMarcher.cpp:262-263).  Used whenever the
+// GPU march cannot deliver the frame, so the caller never uploads what its
+// staging buffers held before.
+void RayMarcher::WriteMisses()
+{
+    const size_t n = (size_t)m_Width * m_Height;
+    if (m_Positions) std::memset((void*)m_Positions, 0, n * sizeof(glm::vec4));
+    if (m_Normals) std::memset((void*)m_Normals, 0, n * sizeof(glm::vec4));
 }
 
 void RayMarcher::Exit()
@@ -56,6 +68,7 @@ void RayMarcher::Prepare(const VisualizationSettings& settings,
         for (int r = 0; r < 4; r++) m_InvProjectionView[4 * c + r] = M[c][r];
     for (int a = 0; a < 3; a++) m_CameraPosition[a] = camera.Position[a];
 
+    m_FrameReady = false;
     if (!m_Ctx) return;
     const size_t f = (size_t)settings.Frame;
     if (m_FrameOnDevice.size() <= f) m_FrameOnDevice.resize(f + 1, false);
@@ -67,11 +80,18 @@ void RayMarcher::Prepare(const VisualizationSettings& settings,
         else
             std::fprintf(stderr, "RayMarcher: %s\n", pm_last_error(m_Ctx));
     }
+    m_FrameReady = m_FrameOnDevice[f];
 }
 
 void RayMarcher::Start()
 {
-    if (!m_Ctx) return;
+    m_InFlight = false;
+    if (!m_Ctx || !m_FrameReady) {
+        // no context, or the frame could not be put on the GPU: the images are
+        // answered as misses and LastError() says why
+        WriteMisses();
+        return;
+    }
     pm_settings s{};
     s.mode = PM_MODE_REFERENCE;
     s.max_steps = m_Settings.MaxSteps;
@@ -91,7 +111,10 @@ void RayMarcher::Start()
     int rc = pm_march_async(m_Ctx, m_Settings.Frame, &s, m_InvProjectionView, m_CameraPosition, m_Width, m_Height,
                             m_Depth, (float*)m_Positions, (float*)m_Normals, nullptr);
     m_InFlight = rc == PM_OK;
-    if (rc != PM_OK) std::fprintf(stderr, "RayMarcher: %s\n", pm_last_error(m_Ctx));
+    if (rc != PM_OK) {
+        std::fprintf(stderr, "RayMarcher: %s\n", pm_last_error(m_Ctx));
+        WriteMisses();
+    }
 }
 
 bool RayMarcher::IsDone()
@@ -100,7 +123,10 @@ bool RayMarcher::IsDone()
     int rc = pm_query(m_Ctx);
     if (rc == PM_PENDING) return false;
     m_InFlight = false;
-    if (rc != PM_OK) std::fprintf(stderr, "RayMarcher: %s\n", pm_last_error(m_Ctx));
+    if (rc != PM_OK) {
+        std::fprintf(stderr, "RayMarcher: %s\n", pm_last_error(m_Ctx));
+        WriteMisses();
+    }
     return true;
 }
 

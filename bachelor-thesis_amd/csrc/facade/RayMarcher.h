@@ -65,6 +65,8 @@ public:
     const char* LastError() const;
 
 private:
+    void WriteMisses();
+
     VisualizationSettings m_Settings{};
     Dataset* m_Dataset = nullptr;
 
@@ -81,4 +83,5 @@ private:
     pm_ctx* m_Ctx = nullptr;
     std::vector<bool> m_FrameOnDevice;
     bool m_InFlight = false;
+    bool m_FrameReady = false;   // the frame Prepare selected is on the GPU
 };

@@ -15,7 +15,7 @@
 //      cells within one cell of q (the same particles in the same order; the
 //      extended walk when the list was capped).
 // One ray per lane, one 8x8 tile per wave, each lane to the end of its ray
-// (march, bisection, FP64 step, normal, outputs); variant 2 of the configured
+// (march, bisection, FP64 step, normal, outputs); schedule 1 of the configured
 // march runs persistent waves with ray refill and a finish pass over the hits
 // instead (below).  All float arithmetic is in the order of the CPU oracle
 // (oracle/pm_aniso.inc), -ffp-contract=off.
@@ -803,8 +803,8 @@ hipError_t launch_march_aniso(const MarchLaunch &L, hipStream_t st)
     const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
     // one ray per lane to its end (the default: neighbouring rays of a tile
     // sample neighbouring points, so the wave's gathers share cache lines), or
-    // (variant 2) persistent waves with ray refill and the hit-list finish
-    if (L.s.mode == PM_MODE_REFERENCE || L.s.variant != 2 || !L.hit_rec) {
+    // (schedule 1) persistent waves with ray refill and the hit-list finish
+    if (L.s.mode == PM_MODE_REFERENCE || L.s.schedule != 1 || !L.hit_rec) {
         hipLaunchKernelGGL(k_march_aniso, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
         return hipGetLastError();
     }

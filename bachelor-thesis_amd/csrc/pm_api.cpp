@@ -73,6 +73,11 @@ struct pm_ctx {
     bool bound_used = false;              // a march has used the bound skip ...
     float bound_iso_last = 0.0f;          // ... with this iso (its bits' key)
     hipEvent_t ev_launch = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
+    // a depth-sprite pass into the caller's device memory, queued on `stream`
+    // and not yet waited for (pm_depth_sprites_device)
+    hipEvent_t ev_sprites = nullptr;
+    bool sprites_pending = false;
+    int32_t sprites_frame = 0;
     std::map<int32_t, Frame *> frames;
     std::string err;
     // outputs of the last march
@@ -120,6 +125,16 @@ static MlpDev mlp_dev(pm_ctx *c)
     n.W2T = (const uint16_t *)c->mlp_W2T.p;
     n.p = (const float *)c->mlp_p.p;
     return n;
+}
+
+// The learned step's network and knobs (defaults: 70 % of the predicted
+// distance, jumps of 3 to 8 samples).
+static void set_learned(pm_ctx *c, const pm_settings *s, MarchLaunch &L)
+{
+    L.net = mlp_dev(c);
+    L.mlp_alpha = s->learned_alpha > 0.0f ? s->learned_alpha : 0.7f;
+    L.mlp_min_skip = s->learned_min_skip > 0 ? s->learned_min_skip : 3;
+    L.mlp_max_skip = s->learned_max_skip > 0 ? s->learned_max_skip : 8;
 }
 
 static int fail(pm_ctx *c, int code, const char *fmt, ...)
@@ -176,6 +191,7 @@ int pm_create(int device, pm_ctx **out)
         hipStreamCreateWithFlags(&c->stream_build, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev_launch) != hipSuccess || hipEventCreate(&c->ev_k0) != hipSuccess ||
         hipEventCreate(&c->ev_k1) != hipSuccess || hipEventCreate(&c->ev_done) != hipSuccess ||
+        hipEventCreate(&c->ev_sprites) != hipSuccess ||
         c->d_counters.ensure(sizeof(unsigned long long) * kCtrWords) != hipSuccess ||
         c->d_band.ensure(sizeof(uint32_t) * 64) != hipSuccess ||
         hipHostMalloc((void **)&c->h_counters, sizeof(unsigned long long) * kCtrWords, hipHostMallocDefault) != hipSuccess) {
@@ -209,6 +225,7 @@ void pm_destroy(pm_ctx *c)
     if (c->ev_k0) hipEventDestroy(c->ev_k0);
     if (c->ev_k1) hipEventDestroy(c->ev_k1);
     if (c->ev_done) hipEventDestroy(c->ev_done);
+    if (c->ev_sprites) hipEventDestroy(c->ev_sprites);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->stream_build) {
         hipStreamSynchronize(c->stream_build);
@@ -389,6 +406,9 @@ int pm_set_frame_stream(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, 
     HIPCHK(c, hipSetDevice(c->device));
     auto it = c->frames.find(frame);
     Frame *f = it == c->frames.end() ? (c->frames[frame] = new Frame()) : it->second;
+    // a depth-sprite pass queued on the context stream may still read this
+    // frame's grid: the rebuild on the build stream starts after it
+    if (c->sprites_pending && c->sprites_frame == frame) HIPCHK(c, hipStreamWaitEvent(c->stream_build, c->ev_sprites, 0));
     HIPCHK(c, f->xyz.ensure(sizeof(float) * 3 * n));
     HIPCHK(c, hipMemcpyAsync(f->xyz.p, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream_build));
     int rc = build_frame(c, frame, (const float *)f->xyz.p, n, h, c->stream_build);
@@ -505,10 +525,15 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
         return fail(c, PM_ERR_ARG, "the DensityGrid skip belongs to the reference march");
     if (s->mode == PM_MODE_REFERENCE && (s->skip_mode == PM_SKIP_EMPTY || s->skip_mode == PM_SKIP_BOUND))
         return fail(c, PM_ERR_ARG, "the empty-stencil and bound skips belong to the configured march");
-    if (s->variant < 0 || s->variant > 7) return fail(c, PM_ERR_ARG, "unknown kernel variant %d", s->variant);
+    if (s->schedule != 0 && s->schedule != 1) return fail(c, PM_ERR_ARG, "unknown kernel schedule %d", s->schedule);
     if (s->learned && !c->mlp_ready) return fail(c, PM_ERR_STATE, "learned step: no network loaded (pm_mlp_set)");
     if (s->learned && s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "the learned step belongs to the configured march");
     if (s->learned && !(s->learned_alpha >= 0.0f && s->learned_alpha <= 4.0f)) return fail(c, PM_ERR_ARG, "learned_alpha out of [0,4]");
+    if (s->learned && (s->learned_min_skip < 0 || s->learned_min_skip == 1 || s->learned_max_skip < 0 ||
+                       s->learned_max_skip > 64))
+        return fail(c, PM_ERR_ARG, "learned: min_skip must be 0 or >= 2, max_skip in [0, 64]");
+    if (s->instrument && (s->mode != PM_MODE_CONFIG || s->anisotropic || s->schedule != 0))
+        return fail(c, PM_ERR_ARG, "instrument: configured isotropic march, schedule 0 only");
     if (s->mode == PM_MODE_REFERENCE && (s->max_steps < 0 || s->max_steps > (1 << 24)))
         return fail(c, PM_ERR_ARG, "max_steps out of range");
     if (s->anisotropic) {
@@ -631,22 +656,16 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
         L.hit_rec = (uint4 *)c->d_hits.p;
     }
     if (s->anisotropic) L.an = aniso_params(f, s);
-    // tile order and miss pre-pass for the persistent schedules (variant 7: without)
-    const bool persistent =
-        !s->anisotropic && (s->learned || (s->variant != 1 && s->variant != 4 && s->variant != 5));
-    if (s->mode == PM_MODE_CONFIG && persistent && s->variant != 7) {
+    // tile order and miss pre-pass for the persistent schedule
+    const bool persistent = !s->anisotropic && (s->learned || s->schedule == 0);
+    if (s->mode == PM_MODE_CONFIG && persistent) {
         uint64_t tiles = (uint64_t)((W + 7) / 8) * ((H + 7) / 8);
         HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (3 * (tiles + 1) + scan_tmp_size((uint32_t)(tiles + 1)))));
         L.tile_buf = (uint32_t *)c->d_tiles.p;
     }
     c->prepass = L.tile_buf != nullptr;
     L.cg = f->cg;
-    if (s->learned) {
-        L.net = mlp_dev(c);
-        L.mlp_alpha = s->learned_alpha > 0.0f ? s->learned_alpha : 0.7f;
-        L.mlp_min_skip = s->learned_min_skip > 0 ? s->learned_min_skip : 4;
-        L.mlp_max_skip = s->learned_max_skip > 0 ? s->learned_max_skip : 24;
-    }
+    if (s->learned) set_learned(c, s, L);
 
     HIPCHK(c, hipEventRecord(c->ev_launch, c->stream));
     if (s->mode == PM_MODE_REFERENCE) {
@@ -662,7 +681,7 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
     HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
     HIPCHK(c, launch_march(L, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->d_counters.p, sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_counters, L.counters, sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, c->stream));
 
     if (dev) positions_out = normals_out = t_out = nullptr;
     c->user_pos = positions_out;
@@ -730,17 +749,39 @@ static int complete(pm_ctx *c)
     S.cand = sum[C_CAND];
     S.march_samples = sum[C_MARCH_SAMPLES];
     S.mlp_evals = sum[C_MLP];
+    S.march_cand = sum[C_MARCH_CAND];
+    S.cert_walks = sum[C_CERT];
+    S.cert_skipped = sum[C_CERT_SKIP];
+    S.cand_stencil = sum[C_CAND_STENCIL];
+    S.cand27 = sum[C_CAND27];
     hipEventElapsedTime(&S.kernel_ms, c->ev_k0, c->ev_k1);
     hipEventElapsedTime(&S.total_ms, c->ev_launch, c->ev_done);
     c->pending = false;
     c->finished = true;
+    c->sprites_pending = false;      // queued before the march on the same stream
+    return PM_OK;
+}
+
+// A sprite pass into device memory completes with the context stream; pm_query
+// and pm_wait report it (and a march queued after it completes after it).
+static int sprites_query(pm_ctx *c, bool block)
+{
+    if (!c->sprites_pending) return PM_OK;
+    hipError_t e = block ? hipEventSynchronize(c->ev_sprites) : hipEventQuery(c->ev_sprites);
+    if (e == hipErrorNotReady) return PM_PENDING;
+    c->sprites_pending = false;
+    if (e != hipSuccess) return fail(c, PM_ERR_HIP, "depth-sprite pass failed: %s", hipGetErrorString(e));
     return PM_OK;
 }
 
 int pm_query(pm_ctx *c)
 {
     if (!c) return PM_ERR_ARG;
-    if (!c->pending) return c->finished ? PM_OK : fail(c, PM_ERR_STATE, "no march launched");
+    if (!c->pending) {
+        const int rs = sprites_query(c, false);
+        if (rs != PM_OK) return rs;
+        return c->finished ? PM_OK : fail(c, PM_ERR_STATE, "no march launched");
+    }
     hipError_t e = hipEventQuery(c->ev_done);
     if (e == hipErrorNotReady) return PM_PENDING;
     if (e != hipSuccess) { c->pending = false; return fail(c, PM_ERR_HIP, "march failed: %s", hipGetErrorString(e)); }
@@ -750,7 +791,7 @@ int pm_query(pm_ctx *c)
 int pm_wait(pm_ctx *c)
 {
     if (!c) return PM_ERR_ARG;
-    if (!c->pending) return PM_OK;
+    if (!c->pending) return sprites_query(c, true);
     hipError_t e = hipEventSynchronize(c->ev_done);
     if (e != hipSuccess) { c->pending = false; return fail(c, PM_ERR_HIP, "march failed: %s", hipGetErrorString(e)); }
     return complete(c);
@@ -933,10 +974,7 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
     B.base.counters = (unsigned long long *)c->d_counters.p;
     HIPCHK(c, c->b_hits.ensure(sizeof(uint4) * npx));
     if (s->learned) {
-        B.base.net = mlp_dev(c);
-        B.base.mlp_alpha = s->learned_alpha > 0.0f ? s->learned_alpha : 0.7f;
-        B.base.mlp_min_skip = s->learned_min_skip > 0 ? s->learned_min_skip : 4;
-        B.base.mlp_max_skip = s->learned_max_skip > 0 ? s->learned_max_skip : 24;
+        set_learned(c, s, B.base);
     }
     for (uint32_t v = 0; v < n_views; v++) {   // the shared fields of every job
         MarchLaunch &J = jobs[v];
@@ -1260,6 +1298,9 @@ static int sprites_launch(pm_ctx *c, int32_t frame, const float view[16], const 
                         c->device);
         HIPCHK(c, launch_depth_sprites(S, (const float4 *)f->sorted.p, (uint32_t)f->info.n, (uint32_t *)d_depth,
                                        c->stream));
+        HIPCHK(c, hipEventRecord(c->ev_sprites, c->stream));
+        c->sprites_pending = true;
+        c->sprites_frame = frame;
         return PM_OK;
     }
     HIPCHK(c, c->d_out.ensure(4 * npx));

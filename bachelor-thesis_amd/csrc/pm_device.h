@@ -136,10 +136,31 @@ __device__ __forceinline__ int32_t pm_key(float x, float inv_h)
     return x >= 0.0f ? (int32_t)v : (int32_t)v - 1;
 }
 
+// sqrtf(x) for the kernel weight, x >= 0: the hardware square root (within one
+// ulp) corrected to the correctly rounded result by the signs of the residuals
+// x - s' s of its neighbours s' = s -+ 1 ulp -- the expansion the compiler emits
+// for sqrtf at x >= 2^-96, without its scaling of smaller inputs and its
+// special-value fix-up.  Below 2^-96 the hardware result is kept: there
+// q = sqrt(x) / h < 2^-14 for any h > 2^-34, and W(q) = sigma (6 (q^3 - q^2) + 1)
+// rounds to sigma exactly whatever the last bit of the root.
+__device__ __forceinline__ float pm_sqrt_w(float x)
+{
+    float s = __builtin_amdgcn_sqrtf(x);
+    if (x >= 0x1p-96f) {
+        const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+        const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+        const float rm = __builtin_fmaf(-sm, s, x);
+        const float rp = __builtin_fmaf(-sp, s, x);
+        s = rm <= 0.0f ? sm : s;
+        s = rp > 0.0f ? sp : s;
+    }
+    return s;
+}
+
 // Kernel.cpp:16-32 on a squared distance known to be < h^2.
 __device__ __forceinline__ float pm_W_in(float r2, float inv_h, float sig)
 {
-    float q = sqrtf(r2) * inv_h;
+    float q = pm_sqrt_w(r2) * inv_h;
     if (q >= 0.5f) {
         float qq = 1.0f - q;
         return sig * (2.0f * qq * qq * qq);
@@ -180,9 +201,9 @@ __device__ __forceinline__ bool pm_stencil(const GridDev &g, float x, float y, f
 }
 
 // rho(q) = sum_j W(x_j - q), FP32, canonical order; adds the number of
-// stencil candidates to cand.
+// stencil candidates to cand and of particles within h to nsup.
 __device__ __forceinline__ float pm_density32(const GridDev &g, float qx, float qy, float qz,
-                                              bool compensated, uint32_t &cand)
+                                              bool compensated, uint32_t &cand, uint32_t &nsup)
 {
     FineWalk fw;
     float s = 0.0f, c = 0.0f;
@@ -201,11 +222,53 @@ __device__ __forceinline__ float pm_density32(const GridDev &g, float qx, float 
                     float w = pm_W_in(r2, g.inv_h, g.sig);
                     if (compensated) pm_neu_add(s, c, w);
                     else s += w;
+                    nsup++;
                 }
             }
         }
     }
     return compensated ? s + c : s;
+}
+
+__device__ __forceinline__ float pm_density32(const GridDev &g, float qx, float qy, float qz,
+                                              bool compensated, uint32_t &cand)
+{
+    uint32_t nsup = 0;
+    return pm_density32(g, qx, qy, qz, compensated, cand, nsup);
+}
+
+// Candidates of q's own trimmed fine stencil (what pm_density32 visits), with no
+// particle read: the per-sample count of instrumented launches.
+__device__ __forceinline__ uint32_t pm_stencil_cand(const GridDev &g, float qx, float qy, float qz)
+{
+    FineWalk fw;
+    if (!pm_fine_begin(g, qx, qy, qz, fw)) return 0u;
+    uint32_t cand = 0;
+    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++) {
+        const float tz2 = pm_fine_tz2(fw, dz);
+        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
+            uint32_t b, e;
+            pm_fine_row(g, fw, dy, dz, tz2, b, e);
+            cand += e - b;
+        }
+    }
+    return cand;
+}
+
+// Particles stored in the 27 search cells (cells of h) around q's cell, the
+// candidates of the reference's neighbour query (CompactNSearch's 27-cell
+// scan, Dataset.cpp:272-280): 0 when the 3x3x3 block leaves the grid.
+__device__ __forceinline__ uint32_t pm_cand27(const GridDev &g, float qx, float qy, float qz)
+{
+    int32_t cx, cy, cz;
+    if (!pm_stencil(g, qx, qy, qz, cx, cy, cz)) return 0u;
+    uint32_t cand = 0;
+    _Pragma("unroll 1") for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            const uint32_t base = (uint32_t)(cx - 1) + (uint32_t)g.nx * ((uint32_t)(cy + dy) + (uint32_t)g.ny * (uint32_t)(cz + dz));
+            cand += g.cell_start[base + 3] - g.cell_start[base];
+        }
+    return cand;
 }
 
 // Same sum as pm_density32, with the in-support candidates compacted through a
@@ -243,13 +306,14 @@ __device__ __forceinline__ void pm_test_append(const GridDev &g, float4 p, float
 // independent gathers in flight instead of one dependent load per iteration.
 template <int CAP, int UNROLL>
 __device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, float qy, float qz, float *buf,
-                                                  uint32_t &cand)
+                                                  uint32_t &cand, uint32_t &nsup)
 {
     static_assert(UNROLL >= 1 && UNROLL <= CAP, "unroll");
     FineWalk fw;
     float s = 0.0f, c = 0.0f;
     bool in = pm_fine_begin(g, qx, qy, qz, fw);
     int n = 0;
+    uint32_t drained = 0;
     if (in) {
         _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++) {
             const float tz2 = pm_fine_tz2(fw, dz);
@@ -260,6 +324,7 @@ __device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, fl
                 if (__ballot(e > b) == 0) continue;         // no lane has candidates in this row
                 if (__ballot(n + (int)(e - b) > CAP) != 0) {   // some lane might overflow:
                     pm_drain<CAP>(g, buf, n, s, c);           // all drain together
+                    drained += (uint32_t)n;
                     n = 0;
                 }
                 if ((int)(e - b) <= CAP) {
@@ -288,14 +353,23 @@ __device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, fl
                             for (int u = 0; u < UNROLL; u++) pm_test_append(g, v[u], qx, qy, qz, buf, n);
                         }
                         for (; p < ce; p++) pm_test_append(g, *p, qx, qy, qz, buf, n);
-                        if (p < pe) { pm_drain<CAP>(g, buf, n, s, c); n = 0; }
+                        if (p < pe) { pm_drain<CAP>(g, buf, n, s, c); drained += (uint32_t)n; n = 0; }
                     }
                 }
             }
         }
     }
     pm_drain<CAP>(g, buf, n, s, c);
+    nsup += drained + (uint32_t)n;
     return s + c;
+}
+
+template <int CAP, int UNROLL>
+__device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, float qy, float qz, float *buf,
+                                                  uint32_t &cand)
+{
+    uint32_t nsup = 0;
+    return pm_density32_lds<CAP, UNROLL>(g, qx, qy, qz, buf, cand, nsup);
 }
 
 // ---- fused central differences ----
@@ -494,6 +568,102 @@ __device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy
 #pragma unroll
     for (int k = 0; k < 6; k++) out.v[k] = comp ? a.s[k] + a.c[k] : a.s[k];
     return out;
+}
+
+// The six central-difference densities of pm_cdiff6 over a list.
+__device__ __forceinline__ void pm_list_cdiff6(const GridDev &g, const float *buf, uint32_t n, const float x[3],
+                                               float eps, bool comp, float v[6])
+{
+    const float xp[3] = { x[0] + eps, x[1] + eps, x[2] + eps }, xm[3] = { x[0] - eps, x[1] - eps, x[2] - eps };
+    Acc6 a;
+#pragma unroll
+    for (int k = 0; k < 6; k++) { a.s[k] = 0.0f; a.c[k] = 0.0f; }
+    for (uint32_t j = 0; j < n; j++)
+        pm_acc6(g, buf[(3 * j) * 64], buf[(3 * j + 1) * 64], buf[(3 * j + 2) * 64], x, xp, xm, comp, a);
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = comp ? a.s[k] + a.c[k] : a.s[k];
+}
+
+// ---- per-hit candidate lists ----
+// Every particle within h (1 + 2^-10) of the box [lo, hi], in storage order --
+// the canonical summation order -- as (x, y, z) triplets in the lane's slots
+// (entry j at buf[(3 j + c) 64]).  A sample anywhere in the box has every
+// particle within h of it in the list, so a density summed over the list (each
+// entry tested with the sample's own r^2 expression) equals the sample's walk
+// bit for bit.  n: particles found (the list holds the first CAPL); false when
+// the box's cells leave the march grid.
+template <int CAPL>
+__device__ __forceinline__ bool pm_list_build(const GridDev &g, const float lo[3], const float hi[3], float *buf,
+                                              uint32_t &n, uint32_t &cand)
+{
+    BoxWalk w;
+    n = 0;
+    if (!pm_box_begin(g, lo, hi, w)) return false;
+    const float he = g.h * 1.0009765625f, R2 = he * he;
+    uint32_t nb, ne;
+    pm_box_row(g, w, 0, 0, nb, ne);
+    const int nr = w.ny * w.nz;
+    int jy = 0, jz = 0;
+    _Pragma("unroll 1") for (int r = 0; __ballot(r < nr) != 0; r++) {
+        const uint32_t b = nb, e = ne;
+        cand += e - b;
+        if (++jy == w.ny) { jy = 0; jz++; }
+        pm_box_row(g, w, jy, jz, nb, ne);                 // next row in flight
+        const int m = (int)(e - b);
+        const float4 *const p0 = g.fsorted + b;
+        for (int i = 0; i < m; i += 4) {
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float dx = fmaxf(fmaxf(lo[0] - v[u].x, v[u].x - hi[0]), 0.0f);
+                const float dy = fmaxf(fmaxf(lo[1] - v[u].y, v[u].y - hi[1]), 0.0f);
+                const float dz = fmaxf(fmaxf(lo[2] - v[u].z, v[u].z - hi[2]), 0.0f);
+                // each candidate goes to the next free slot and stays only if it
+                // passes (n moves past it); a full list takes no more stores
+                // (n then only counts)
+                if (n < (uint32_t)CAPL) {
+                    buf[(3 * n) * 64] = v[u].x;
+                    buf[(3 * n + 1) * 64] = v[u].y;
+                    buf[(3 * n + 2) * 64] = v[u].z;
+                }
+                n += (i + u < m && (dx * dx + dy * dy) + dz * dz < R2) ? 1u : 0u;
+            }
+        }
+    }
+    return true;
+}
+
+// pm_density32 / pm_density32_lds over a list: the same sum, bit for bit.
+__device__ __forceinline__ float pm_list_density32(const GridDev &g, const float *buf, uint32_t n, float qx, float qy,
+                                                   float qz, bool comp)
+{
+    float s = 0.0f, c = 0.0f;
+    for (uint32_t j = 0; j < n; j++) {
+        const float rx = buf[(3 * j) * 64] - qx, ry = buf[(3 * j + 1) * 64] - qy, rz = buf[(3 * j + 2) * 64] - qz;
+        const float r2 = (rx * rx + ry * ry) + rz * rz;
+        if (r2 < g.h2) {
+            const float w = pm_W_in(r2, g.inv_h, g.sig);
+            if (comp) pm_neu_add(s, c, w);
+            else s += w;
+        }
+    }
+    return comp ? s + c : s;
+}
+
+// pm_density64 over a list.
+__device__ __forceinline__ double pm_list_density64(const GridDev &g, const float *buf, uint32_t n, double qx,
+                                                    double qy, double qz)
+{
+    double acc = 0.0;
+    for (uint32_t j = 0; j < n; j++) {
+        const double rx = (double)buf[(3 * j) * 64] - qx, ry = (double)buf[(3 * j + 1) * 64] - qy;
+        const double rz = (double)buf[(3 * j + 2) * 64] - qz;
+        const double r2 = (rx * rx + ry * ry) + rz * rz;
+        if (r2 < g.h2_d) acc += pm_W_in_d(r2, g.hinv_d, g.sig_d);
+    }
+    return acc;
 }
 
 // FP64 density; the stencil is chosen from the FP32-rounded point.

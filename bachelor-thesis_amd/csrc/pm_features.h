@@ -34,43 +34,42 @@ __device__ __forceinline__ float coarse_count(const GridDev &g, const CoarseDev 
     return (float)cg.count[c];
 }
 
-// rho (plain FP32 sum) and grad rho at q, for the network's inputs only.
-__device__ __forceinline__ void density_grad_feat(const GridDev &g, float qx, float qy, float qz, float &rho,
-                                                  float &gx, float &gy, float &gz, uint32_t &cand)
+// grad rho at q over the trimmed fine stencil (the particles within h, in the
+// canonical order), plain FP32 sums, for the network's inputs only; adds the
+// walk's candidates to cand.
+__device__ __forceinline__ void grad_feat(const GridDev &g, float qx, float qy, float qz, float &gx, float &gy,
+                                          float &gz, uint32_t &cand)
 {
-    int32_t cx, cy, cz;
-    rho = gx = gy = gz = 0.0f;
-    if (!pm_stencil(g, qx, qy, qz, cx, cy, cz)) return;
+    FineWalk fw;
+    gx = gy = gz = 0.0f;
+    if (!pm_fine_begin(g, qx, qy, qz, fw)) return;
     const float dwk = g.sig * g.inv_h;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++) {
-            uint32_t base = (uint32_t)(cx - 1) + (uint32_t)g.nx * ((uint32_t)(cy + dy) + (uint32_t)g.ny * (uint32_t)(cz + dz));
-            uint32_t b = g.cell_start[base], e = g.cell_start[base + 3];
+    _Pragma("unroll 1") for (int dz = -2; dz <= 2; dz++) {
+        const float tz2 = pm_fine_tz2(fw, dz);
+        _Pragma("unroll 1") for (int dy = -2; dy <= 2; dy++) {
+            uint32_t b, e;
+            pm_fine_row(g, fw, dy, dz, tz2, b, e);
             cand += e - b;
             for (uint32_t i = b; i < e; i++) {
-                float4 p = g.sorted[i];
-                float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-                float r2 = (rx * rx + ry * ry) + rz * rz;
-                if (r2 < g.h2) {
-                    float r = sqrtf(r2);
-                    float q = r * g.inv_h;
-                    float w, dw;
+                const float4 p = g.fsorted[i];
+                const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
+                const float r2 = (rx * rx + ry * ry) + rz * rz;
+                if (r2 < g.h2 && r2 > 0.0f) {
+                    const float r = sqrtf(r2);
+                    const float q = r * g.inv_h;
+                    float dw;
                     if (q >= 0.5f) {
-                        float qq = 1.0f - q;
-                        w = g.sig * (2.0f * qq * qq * qq);
+                        const float qq = 1.0f - q;
                         dw = -dwk * 6.0f * qq * qq;
                     } else {
-                        w = g.sig * (6.0f * (q * q * q - q * q) + 1.0f);
                         dw = dwk * 6.0f * (3.0f * q * q - 2.0f * q);
                     }
-                    rho += w;
-                    if (r > 0.0f) {
-                        float f = -dw / r;          // d/dx W(|x_j - x|) = -W'(r) r_vec / r
-                        gx += f * rx; gy += f * ry; gz += f * rz;
-                    }
+                    const float f = -dw / r;          // d/dx W(|x_j - x|) = -W'(r) r_vec / r
+                    gx += f * rx; gy += f * ry; gz += f * rz;
                 }
             }
         }
+    }
 }
 
 __device__ __forceinline__ void mlp_features(const GridDev &g, const CoarseDev &cg, const float *o, const float *d, float t, float tn,

@@ -10,8 +10,12 @@
 
 namespace pm {
 
-// Per-launch counters (u64 each), accumulated by the march kernel.
-enum Counter { C_RAYS = 0, C_IN_AABB, C_HITS, C_SAMPLES, C_CAND, C_MARCH_SAMPLES, C_MLP, C_COUNT };
+// Per-launch counters (u64 each), accumulated by the march kernels, in the
+// order of pm_stats.
+enum Counter {
+    C_RAYS = 0, C_IN_AABB, C_HITS, C_SAMPLES, C_CAND, C_MARCH_SAMPLES, C_MLP, C_MARCH_CAND, C_CERT, C_CERT_SKIP,
+    C_CAND_STENCIL, C_CAND27, C_COUNT
+};
 // The counters live in kCtrSlots copies, one per XCD (a workgroup adds to
 // slot blockIdx % 8), kCtrStride u64 (one 128-byte line) apart: atomics of
 // different XCDs never meet on one address.  The host sums the slots.
@@ -66,6 +70,7 @@ struct MarchLaunch {
     float inv_step;            // 1 / s.step (bound skip)
     // per pixel: the hit handed from the march to k_finish (1 | fp64 << 1,
     // lo, hi, 0), zeros for a miss; written for every pixel of a marched tile
+    // (the anisotropic march: its list of hits)
     uint4 *hit_rec;
     AnisoDev an;               // s.anisotropic
 };

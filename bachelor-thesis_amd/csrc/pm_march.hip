@@ -3,16 +3,19 @@
 // Replaces RayMarcher::PerPixel_Isotropic (RayMarcher.cpp:256-344) and the
 // ThreadPool that drives it (ThreadPool.cpp:38-55).
 //
-// Work mapping: one wave64 per 8x8 pixel tile, one ray per lane, four waves
-// per 256-thread workgroup.  Workgroups are dispatched round-robin over the 8
-// XCDs; the block index is remapped so that each XCD receives a contiguous
-// band of tiles and its L2 holds the particles of that band.
+// Work mapping: one wave64 per 8x8 pixel tile at a time, one ray per lane, four
+// waves per 256-thread workgroup; persistent workgroups take the tiles from
+// per-XCD bands (blockIdx % 8 names the XCD), so each XCD's L2 holds the
+// particles of its band.
 //
-// Every lane runs a small state machine (march -> bisection -> FP64 bisection
-// step -> normal samples) and each loop iteration performs exactly one density
-// evaluation per active lane, whatever its phase, so the expensive 27-cell
-// candidate loop stays converged across the wave while rays hit at different
-// steps.
+// Every lane runs a small state machine (march -> FP32 bisection; the learned
+// form adds step proposals) and each loop iteration performs exactly one FP32
+// density evaluation per busy lane, whatever its phase, so the candidate loop
+// stays converged across the wave while rays hit at different steps.  A ray's
+// march ends at its hit record; the FP64 step, the normal and the outputs
+// follow per tile (finish_hit).  Schedule 1 (k_march_simple) marches one ray
+// per lane to its end with a direct walk for every sample: the plain form the
+// persistent one is checked against.
 #include "pm_features.h"
 #include "pm_internal.h"
 #include "pm_ray.h"
@@ -49,19 +52,15 @@ __device__ __forceinline__ void start_bracket(const MarchLaunch &L, RayState &r,
 
 // First half of one step: the sample point of the lane's next FP32 density
 // evaluation, after the skips of the march phase; false when the lane takes
-// no sample this step (the ray ended, or it used up its skips).
-__device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, float &qx, float &qy, float &qz,
-                                            Counts &cn)
+// no sample this step (the ray ended, or it used up its skips).  Past an
+// exactly skipped stretch the learned march may ask for proposals again.
+__device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, float &qx, float &qy, float &qz)
 {
     const pm_settings &s = L.s;
     const GridDev &g = L.g;
     const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
 
-    if (r.phase == PH_LAND || r.phase == PH_BSEARCH) {
-        int kq = r.phase == PH_LAND ? r.khi : (r.klo + r.khi) / 2;
-        float t = r.tn + (float)kq * s.step;
-        qx = o0 + t * r.d0; qy = o1 + t * r.d1; qz = o2 + t * r.d2;
-    } else if (r.phase == PH_MARCH) {
+    if (r.phase == PH_MARCH) {
         // at most 8 skips per step: a lane crossing much skippable space then
         // idles for a step instead of holding the whole wave in this loop
         for (int hop = 0;; hop++) {
@@ -78,6 +77,7 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
             else break;
             if (kn == r.k) break;
             r.k = kn;
+            r.lrn = s.learned;
             if (hop == 7) return false;
         }
     } else {  // PH_BISECT
@@ -87,40 +87,44 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
     return true;
 }
 
+// The counts of one FP32 density evaluation at q (instrumented launches also
+// count its own stencil and its 27 search cells).
+template <bool INSTR>
+__device__ __forceinline__ void count_sample(const GridDev &g, float qx, float qy, float qz, uint32_t cand, bool march,
+                                             Counts &cn)
+{
+    cn.samples++;
+    cn.cand += cand;
+    if (march) {
+        cn.march++;
+        cn.mcand += cand;
+    }
+    if (INSTR) {
+        cn.cst += cand;
+        cn.c27 += pm_cand27(g, qx, qy, qz);
+    }
+}
+
 // Second half: the sample's density rho (and its stencil candidate count)
 // drives the lane's state machine.
-__device__ __forceinline__ void ray_finish(const MarchLaunch &L, RayState &r, float rho, uint32_t cand, Counts &cn)
+template <bool INSTR>
+__device__ __forceinline__ void ray_finish(const MarchLaunch &L, RayState &r, float rho, uint32_t cand, float qx,
+                                           float qy, float qz, Counts &cn)
 {
     const pm_settings &s = L.s;
     const bool fp64 = s.final_fp64 && s.bisect_iters > 0;
     const int n32 = fp64 ? s.bisect_iters - 1 : s.bisect_iters;
-    cn.samples++;
-    cn.cand += cand;
+    count_sample<INSTR>(L.g, qx, qy, qz, cand, r.phase == PH_MARCH, cn);
 
     if (r.phase == PH_MARCH) {
-        cn.march++;
         if (rho >= s.iso) {
             start_bracket(L, r, n32);
         } else if (r.lrn) {
-            r.klo = r.k;
+            r.rho = rho;
             r.phase = PH_PROPOSE;
         } else {
             r.k++;
         }
-    } else if (r.phase == PH_LAND) {
-        cn.march++;
-        if (rho >= s.iso) {
-            if (r.khi - r.klo == 1) { r.k = r.khi; start_bracket(L, r, n32); }
-            else r.phase = PH_BSEARCH;
-        } else {
-            r.klo = r.k = r.khi;
-            r.phase = PH_PROPOSE;
-        }
-    } else if (r.phase == PH_BSEARCH) {
-        cn.march++;
-        int mid = (r.klo + r.khi) / 2;
-        if (rho >= s.iso) r.khi = mid; else r.klo = mid;
-        if (r.khi - r.klo == 1) { r.k = r.khi; start_bracket(L, r, n32); }
     } else {  // PH_BISECT
         float tm = (r.lo + r.hi) * 0.5f;
         if (rho >= s.iso) r.hi = tm; else r.lo = tm;
@@ -134,14 +138,14 @@ __device__ __forceinline__ void ray_finish(const MarchLaunch &L, RayState &r, fl
 
 // One density evaluation of this ray, whatever its phase, and the transition
 // it implies.  Dens(q, cand) is the FP32 density.
-template <class Dens>
+template <bool INSTR, class Dens>
 __device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens dens, Counts &cn)
 {
     float qx, qy, qz;
-    if (!ray_prepare(L, r, qx, qy, qz, cn)) return;
+    if (!ray_prepare(L, r, qx, qy, qz)) return;
     uint32_t cand = 0;
     float rho = dens(qx, qy, qz, cand);
-    ray_finish(L, r, rho, cand, cn);
+    ray_finish<INSTR>(L, r, rho, cand, qx, qy, qz, cn);
 }
 
 // ---- lane groups: the end of a persistent launch ----
@@ -251,8 +255,9 @@ __device__ __forceinline__ bool group_prepare(const MarchLaunch &L, RayState &r,
 }
 
 // The group's transition from its lanes' densities.  Called by all 64 lanes.
+template <bool INSTR>
 __device__ __forceinline__ void group_finish(const MarchLaunch &L, RayState &r, int G, int j, bool want, bool nonskip,
-                                             float rho, uint32_t cand, Counts &cn)
+                                             float rho, uint32_t cand, float qx, float qy, float qz, Counts &cn)
 {
     const pm_settings &s = L.s;
     const bool fp64 = s.final_fp64 && s.bisect_iters > 0;
@@ -270,9 +275,11 @@ __device__ __forceinline__ void group_finish(const MarchLaunch &L, RayState &r, 
         const uint32_t cn_n = __shfl(cand, (int)gbase + node - 1, 64);
         if (bis && r.it < n32 && ((wants >> (node - 1)) & 1ull)) {
             const float tm = (r.lo + r.hi) * 0.5f;
+            if (j == 0)
+                count_sample<INSTR>(L.g, L.cam[0] + tm * r.d0, L.cam[1] + tm * r.d1, L.cam[2] + tm * r.d2, cn_n, false,
+                                    cn);
             if (rn >= s.iso) { r.hi = tm; node = 2 * node; } else { r.lo = tm; node = 2 * node + 1; }
             r.it++;
-            if (j == 0) { cn.samples++; cn.cand += cn_n; }
         }
     }
     if (bis && r.it >= n32) {
@@ -282,11 +289,7 @@ __device__ __forceinline__ void group_finish(const MarchLaunch &L, RayState &r, 
     }
     if (r.phase != PH_MARCH || !(wants & 1ull)) return;
     const int jc = cross ? __ffsll((long long)cross) - 1 : G;
-    if (want && nonskip && j <= jc) {
-        cn.samples++;
-        cn.march++;
-        cn.cand += cand;
-    }
+    if (want && nonskip && j <= jc) count_sample<INSTR>(L.g, qx, qy, qz, cand, true, cn);
     if (cross) {
         r.k += jc;
         start_bracket(L, r, n32, j == 0);
@@ -373,76 +376,235 @@ __global__ __launch_bounds__(256) void k_miss_tiles(MarchLaunch L, uint32_t T)
     }
 }
 
-// Schedule "tile": one 8x8 tile per wave, one ray per lane, all lanes in
-// lockstep until the last ray of the tile is done.
-template <bool LDS, int UNROLL>
-__global__ __launch_bounds__(256) void k_march_tile(MarchLaunch L)
+// Schedule 1: one 8x8 tile per wave, one ray per lane to its end -- march,
+// bisection, FP64 step, normal, outputs -- every sample with its own direct
+// walk (no compaction, no lists, no hit records).  The plain form of the
+// configured march the persistent schedule is checked against.
+__global__ __launch_bounds__(256) void k_march_simple(MarchLaunch L)
 {
-    __shared__ float lds_r2[LDS ? 4 * kLdsCap * 64 : 1];
     uint32_t px, py;
-    if (!pm_tile(L.W, L.H, px, py)) return;   // wave-uniform
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Counts cn;
-    RayState r{};
-    r.phase = PH_DONE;
-    if (px < L.W && py < L.H) ray_setup(L, r, px, py, cn);
-    if (LDS && L.s.compensated) {
-        DensLds<kLdsCap, UNROLL> dens{ &L.g, lds_r2 + wave * kLdsCap * 64 + lane };
-        while (r.phase != PH_DONE) ray_step(L, r, dens, cn);
-    } else {
-        DensPlain dens{ &L.g, L.s.compensated != 0 };
-        while (r.phase != PH_DONE) ray_step(L, r, dens, cn);
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
+    const bool comp = s.compensated != 0;
+    if (pm_tile(L.W, L.H, px, py) && px < L.W && py < L.H) {
+        RayState r{};
+        cn.rays++;
+        r.idx = py * L.W + px;
+        float d[3];
+        pm_ray_dir(L, px, py, d);
+        r.d0 = d[0]; r.d1 = d[1]; r.d2 = d[2];
+        r.id0 = 1.0f / d[0]; r.id1 = 1.0f / d[1]; r.id2 = 1.0f / d[2];
+        const float o[3] = { L.cam[0], L.cam[1], L.cam[2] };
+        bool hit = false;
+        if (pm_slab(o, d, g.bmin, g.bmax, r.tn, r.tf)) {
+            cn.in_aabb++;
+            r.k = 0;
+            for (;;) {
+                const float t = r.tn + (float)r.k * s.step;
+                if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) break;
+                const float qx = o[0] + t * d[0], qy = o[1] + t * d[1], qz = o[2] + t * d[2];
+                int kn = r.k;
+                if (s.skip_mode == PM_SKIP_BOUND) kn = pm_skip_bound(g, o[0], o[1], o[2], r, qx, qy, qz, L.inv_step);
+                else if (s.skip_mode == PM_SKIP_EMPTY) kn = pm_skip_empty(g, o[0], o[1], o[2], r, qx, qy, qz, s.step);
+                if (kn != r.k) { r.k = kn; continue; }
+                uint32_t cand = 0;
+                const float rho = pm_density32(g, qx, qy, qz, comp, cand);
+                cn.samples++; cn.march++; cn.cand += cand; cn.mcand += cand;
+                if (rho >= s.iso) { hit = true; break; }
+                r.k++;
+            }
+        }
+        if (!hit) {
+            write_miss(L, r.idx);
+        } else {
+            const bool fp64 = s.final_fp64 && s.bisect_iters > 0;
+            const int n32 = fp64 ? s.bisect_iters - 1 : s.bisect_iters;
+            float lo = r.tn + (float)(r.k - 1) * s.step, hi = r.tn + (float)r.k * s.step;
+            float xs[3], th;
+            if (r.k == 0) {
+                th = r.tn;
+                xs[0] = o[0] + th * d[0]; xs[1] = o[1] + th * d[1]; xs[2] = o[2] + th * d[2];
+            } else {
+                for (int it = 0; it < n32; it++) {
+                    const float tm = (lo + hi) * 0.5f;
+                    uint32_t cand = 0;
+                    const float rho = pm_density32(g, o[0] + tm * d[0], o[1] + tm * d[1], o[2] + tm * d[2], comp, cand);
+                    cn.samples++; cn.cand += cand;
+                    if (rho >= s.iso) hi = tm; else lo = tm;
+                }
+                if (fp64) {
+                    const double tmd = ((double)lo + (double)hi) * 0.5;
+                    const double rho = pm_density64(g, (double)o[0] + tmd * (double)d[0], (double)o[1] + tmd * (double)d[1],
+                                                    (double)o[2] + tmd * (double)d[2]);
+                    cn.samples++;
+                    double dlo = lo, dhi = hi;
+                    if (rho >= (double)s.iso) dhi = tmd; else dlo = tmd;
+                    const double tt = (dlo + dhi) * 0.5;
+#pragma unroll
+                    for (int a = 0; a < 3; a++) xs[a] = (float)((double)o[a] + tt * (double)d[a]);
+                    th = (float)tt;
+                } else {
+                    th = (lo + hi) * 0.5f;
+                    xs[0] = o[0] + th * d[0]; xs[1] = o[1] + th * d[1]; xs[2] = o[2] + th * d[2];
+                }
+            }
+            float nx, ny, nz;
+            if (s.normal_mode == PM_NORMAL_GRADW) {
+                float sx, sy, sz;
+                uint32_t cand = 0;
+                pm_sum_gradw(g, xs[0], xs[1], xs[2], sx, sy, sz, cand);
+                cn.cand += cand;
+                pm_normalize(sx, sy, sz, nx, ny, nz);
+            } else {
+                float gv[3];
+                for (int j = 0; j < 6; j++) {
+                    float q[3] = { xs[0], xs[1], xs[2] };
+                    q[j >> 1] = (j & 1) ? xs[j >> 1] - s.normal_eps : xs[j >> 1] + s.normal_eps;
+                    uint32_t cand = 0;
+                    const float rho = pm_density32(g, q[0], q[1], q[2], comp, cand);
+                    cn.samples++; cn.cand += cand;
+                    if (j & 1) gv[j >> 1] -= rho; else gv[j >> 1] = rho;
+                }
+                const float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
+                nx = ny = nz = 0.0f;
+                if (dd > 0.0f) {
+                    const float inv = 1.0f / sqrtf(dd);
+                    nx = -gv[0] * inv; ny = -gv[1] * inv; nz = -gv[2] * inv;
+                }
+            }
+            L.pos[r.idx] = make_float4(xs[0], xs[1], xs[2], 1.0f);
+            L.nrm[r.idx] = make_float4(nx, ny, nz, 1.0f);
+            if (L.t) L.t[r.idx] = th;
+            cn.hits++;
+        }
     }
-    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+    flush_counts(L.counters, cn);
 }
 
-// A step proposal for every lane in PH_PROPOSE: features at the lane's current
-// sample (one stencil pass for rho and grad rho), one MLP evaluation for the
-// whole wave (BF16 MFMA), and the jump to the proposed sample on the same
-// sample lattice t_k = t_entry + k step.  Called by all 64 lanes together.
-// Kept out of line: the feature pass and the network need registers the
-// density loop should not be charged for.
+// ---- learned step proposal with certified jumps ----
+// An upper bound U of the density over every point within `margin` of the
+// segment [a, b]: the sum over the particles near the segment of W at their
+// distance to the segment less the margin.  W falls with the distance, so
+// rho(q) <= U for every such q.  The walk covers the march-grid cells near the
+// segment's box (pm_box_begin); it stops once every lane's U reached lim.
+// INFINITY when those cells leave the grid (nothing is certified).
+__device__ __forceinline__ float pm_segment_bound(const GridDev &g, const float a[3], const float b[3], float margin,
+                                                  float lim, uint32_t &cand)
+{
+    float lo[3], hi[3], u[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        lo[k] = fminf(a[k], b[k]) - margin;
+        hi[k] = fmaxf(a[k], b[k]) + margin;
+        u[k] = b[k] - a[k];
+    }
+    BoxWalk w;
+    if (!pm_box_begin(g, lo, hi, w)) return INFINITY;
+    const float uu = (u[0] * u[0] + u[1] * u[1]) + u[2] * u[2];
+    const float iuu = uu > 0.0f ? 1.0f / uu : 0.0f;
+    float U = 0.0f;
+    const int nr = w.ny * w.nz;
+    int jy = 0, jz = 0;
+    _Pragma("unroll 1") for (int r = 0; __ballot(r < nr && U < lim) != 0; r++) {
+        uint32_t rb, re;
+        pm_box_row(g, w, jy, jz, rb, re);
+        if (++jy == w.ny) { jy = 0; jz++; }
+        if (!(U < lim)) continue;
+        cand += re - rb;
+        for (uint32_t i = rb; i < re; i++) {
+            const float4 p = g.fsorted[i];
+            const float ax = p.x - a[0], ay = p.y - a[1], az = p.z - a[2];
+            const float sp = fminf(fmaxf(((ax * u[0] + ay * u[1]) + az * u[2]) * iuu, 0.0f), 1.0f);
+            const float dx = ax - sp * u[0], dy = ay - sp * u[1], dz = az - sp * u[2];
+            const float dist = sqrtf((dx * dx + dy * dy) + dz * dz) - margin;
+            if (dist < g.h) {
+                const float dl = fmaxf(dist, 0.0f);
+                U += pm_W_in(dl * dl, g.inv_h, g.sig);
+            }
+        }
+    }
+    return U;
+}
+
+// A step proposal for every lane in PH_PROPOSE, all at once: features at the
+// lane's current sample k (its density is known; one fine-stencil pass for
+// grad rho), one MLP evaluation for the whole wave (BF16 MFMA), a jump of n
+// samples, and the certificate of the samples k+1 .. k+n-1 it passes over: the
+// segment bound U < iso (1 - 2^-9), which covers the rounding of the bound
+// and of the samples' own FP32 densities.  A certified lane next evaluates
+// sample k+n exactly; the others step to k+1 and ask for no proposal until the
+// exact skip next moves them.  Every sample the learned march does not
+// evaluate lies below iso, so its first crossing -- and its image -- are the
+// plain march's.  Called by all 64 lanes together; kept out of line so the
+// feature pass, the network and the certificate do not add to the registers
+// of the density loop.
 __device__ __noinline__ void wave_propose(const MarchLaunch &L, RayState &r, Counts &cn, uint16_t *X, uint16_t *H,
                                           float *Y)
 {
     const int l = threadIdx.x & 63;
+    const pm_settings &s = L.s;
+    const GridDev &g = L.g;
     const bool want = r.phase == PH_PROPOSE;
     float f[PM_MLP_IN];
 #pragma unroll
     for (int j = 0; j < PM_MLP_IN; j++) f[j] = 0.0f;
+    const float t = r.tn + (float)r.k * s.step;
     if (want) {
-        float t = r.tn + (float)r.k * L.s.step;
-        float q0 = L.cam[0] + t * r.d0, q1 = L.cam[1] + t * r.d1, q2 = L.cam[2] + t * r.d2;
-        float rho, gx, gy, gz;
+        const float q0 = L.cam[0] + t * r.d0, q1 = L.cam[1] + t * r.d1, q2 = L.cam[2] + t * r.d2;
+        float gx, gy, gz;
         uint32_t cand = 0;
-        density_grad_feat(L.g, q0, q1, q2, rho, gx, gy, gz, cand);
-        cn.samples++;
+        grad_feat(g, q0, q1, q2, gx, gy, gz, cand);
         cn.cand += cand;
         const float d[3] = { r.d0, r.d1, r.d2 };
-        mlp_features(L.g, L.cg, L.cam, d, t, r.tn, r.tf, rho, sqrtf((gx * gx + gy * gy) + gz * gz), (float)cand,
-                     L.s.iso, f);
+        mlp_features(g, L.cg, L.cam, d, t, r.tn, r.tf, r.rho, sqrtf((gx * gx + gy * gy) + gz * gz),
+                     (float)pm_cand27(g, q0, q1, q2), s.iso, f);
     }
     wave_lds_sync();
 #pragma unroll
     for (int j = 0; j < PM_MLP_K1; j++) X[l * PM_MLP_K1 + j] = j < PM_MLP_IN ? f2bf(f[j]) : (uint16_t)0;
     wave_lds_sync();
-    float y = mlp_wave_forward_halves(L.net, X, H, Y);
+    const float y = mlp_wave_forward_halves(L.net, X, H, Y);
     wave_lds_sync();
+    int n = 0;
     if (want) {
         cn.mlp++;
-        float dist = L.g.h * (exp2f(fminf(y, 8.0f)) - 1.0f);
-        float sk = fminf(floorf(L.mlp_alpha * dist / L.s.step), (float)L.mlp_max_skip);
-        int kmax = (int)floorf((r.tf - r.tn) / L.s.step);
-        if (kmax > 0 && r.tn + (float)kmax * L.s.step > r.tf) kmax--;
-        if (!(sk >= (float)L.mlp_min_skip) || r.k + L.mlp_min_skip > kmax) {
-            r.lrn = 0;                       // continue step by step from the first unevaluated sample
-            r.k = r.klo + 1;
+        const float dist = g.h * (exp2f(fminf(y, 8.0f)) - 1.0f);
+        const float sk = fminf(floorf(L.mlp_alpha * dist * L.inv_step), (float)L.mlp_max_skip);
+        int kmax = (int)floorf((r.tf - r.tn) * L.inv_step);           // last sample inside the AABB
+        while (kmax > 0 && r.tn + (float)kmax * s.step > r.tf) kmax--;
+        if (sk >= (float)L.mlp_min_skip && r.k + (int)sk <= kmax) n = (int)sk;
+        if (n < 2) {
+            r.lrn = 0;
+            r.k++;
             r.phase = PH_MARCH;
-        } else {
-            int kt = r.k + (int)fminf(sk, 1.0e6f);
-            r.khi = kt > kmax ? kmax : kt;
-            r.phase = PH_LAND;
         }
+    }
+    const bool cert = want && n >= 2;
+    if (__ballot(cert) == 0) return;
+    float U = INFINITY;
+    const float lim = s.iso * (1.0f - 0x1p-9f);
+    if (cert) {
+        const float ta = r.tn + (float)(r.k + 1) * s.step, tb = r.tn + (float)(r.k + n - 1) * s.step;
+        const float a[3] = { L.cam[0] + ta * r.d0, L.cam[1] + ta * r.d1, L.cam[2] + ta * r.d2 };
+        const float b[3] = { L.cam[0] + tb * r.d0, L.cam[1] + tb * r.d1, L.cam[2] + tb * r.d2 };
+        // the lattice samples between a and b lie within a few ulps of the
+        // segment; the margin covers them and the rounding of the distances
+        float m = g.h;
+#pragma unroll
+        for (int k = 0; k < 3; k++) m = fmaxf(m, fmaxf(fabsf(a[k]), fabsf(b[k])));
+        uint32_t cand = 0;
+        U = pm_segment_bound(g, a, b, m * 0x1p-18f, lim, cand);
+        cn.cand += cand;
+        cn.cert++;
+        if (U < lim) {
+            r.k += n;
+            cn.cskip += (uint32_t)(n - 1);
+        } else {
+            r.k++;
+            r.lrn = 0;
+        }
+        r.phase = PH_MARCH;
     }
 }
 
@@ -452,6 +614,7 @@ __device__ __noinline__ void wave_propose(const MarchLaunch &L, RayState &r, Cou
 // and the pixel's outputs.  Every lane of a wave does the same kind of work
 // and none of the march's state is held, so this runs packed and short; the
 // march kernels keep only the phases that decide where the hit is.
+template <bool INSTR>
 __device__ __forceinline__ void finish_hit(const MarchLaunch &L, uint32_t px, uint32_t py, uint4 rec, float *buf,
                                            Counts &cn)
 {
@@ -471,6 +634,10 @@ __device__ __forceinline__ void finish_hit(const MarchLaunch &L, uint32_t px, ui
         const double z = (double)o2 + tmd * (double)d[2];
         const double rho = pm_density64(g, x, y, z);
         cn.samples++;
+        if (INSTR) {
+            cn.cst += pm_stencil_cand(g, (float)x, (float)y, (float)z);
+            cn.c27 += pm_cand27(g, (float)x, (float)y, (float)z);
+        }
         double dlo = lo, dhi = hi;
         if (rho >= (double)s.iso) dhi = tmd; else dlo = tmd;
         const double tt = (dlo + dhi) * 0.5;
@@ -511,6 +678,15 @@ __device__ __forceinline__ void finish_hit(const MarchLaunch &L, uint32_t px, ui
                 if (k & 1) gv[k >> 1] -= rho; else gv[k >> 1] = rho;
             }
         }
+        if (INSTR) {
+#pragma unroll 1
+            for (int k = 0; k < 6; k++) {
+                float q[3] = { xs[0], xs[1], xs[2] };
+                q[k >> 1] = (k & 1) ? xs[k >> 1] - e : xs[k >> 1] + e;
+                cn.cst += pm_stencil_cand(g, q[0], q[1], q[2]);
+                cn.c27 += pm_cand27(g, q[0], q[1], q[2]);
+            }
+        }
         // RayMarcher.cpp:331-338 with central differences
         const float dd = (gv[0] * gv[0] + gv[1] * gv[1]) + gv[2] * gv[2];
         nx = ny = nz = 0.0f;
@@ -527,13 +703,14 @@ __device__ __forceinline__ void finish_hit(const MarchLaunch &L, uint32_t px, ui
 
 // One 8x8 tile per wave, one pixel per lane: the lanes whose pixel holds a
 // hit record finish it.
+template <bool INSTR>
 __device__ __forceinline__ void finish_tile(const MarchLaunch &L, uint32_t tile, float *buf, Counts &cn)
 {
     const uint32_t lane = threadIdx.x & 63, tiles_x = (L.W + 7) / 8;
     const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
     if (px >= L.W || py >= L.H) return;
     const uint4 rec = L.hit_rec[py * L.W + px];
-    if (rec.x & 1u) finish_hit(L, px, py, rec, buf, cn);
+    if (rec.x & 1u) finish_hit<INSTR>(L, px, py, rec, buf, cn);
 }
 
 // Schedule 0: persistent waves.  The image is cut into 8x8 tiles and the tiles
@@ -546,14 +723,14 @@ __device__ __forceinline__ void finish_tile(const MarchLaunch &L, uint32_t tile,
 // once no tile is left and its lanes are idle.
 // LEARNED adds the step proposals; its LDS region per wave holds either the
 // density compaction buffer or the network's activations (never both at once).
-// CAP: compaction slots per lane; WPE: minimum waves per SIMD the register
-// allocation must allow.  The walk waits on the cache far more than it
-// computes, so the default keeps 4 waves per SIMD (<= 128 VGPRs; all 1,024
-// workgroups resident at once) even at the price of a few spilled registers.
-template <int UNROLL, bool LEARNED, int CAP = kLdsCap, int WPE = 4>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_march_persist(
+// INSTR adds the per-sample counts of instrumented launches.  The walk waits
+// on the cache far more than it computes, so the kernel keeps 4 waves per SIMD
+// (<= 128 VGPRs; all 1,024 workgroups resident at once).
+template <bool LEARNED, bool INSTR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_march_persist(
     MarchLaunch L, uint32_t *__restrict__ band_ctr)
 {
+    constexpr int CAP = kLdsCap;
     // floats per wave: the compaction buffer, or X[64][32] + H[32][64] bf16 + Y[64]
     constexpr int kRegion = LEARNED ? 2112 : CAP * 64;
     __shared__ float lds[4 * kRegion];
@@ -585,20 +762,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             if (qpos >= 64) {
                 uint32_t t = 0xffffffffu;
                 if (lane == 0) {
-                    // with the pre-pass the bands split the active tiles only
-                    const uint32_t TA = L.tile_buf ? L.tile_buf[2 * (T + 1) + T] : T;
+                    // the bands split the active tiles (the pre-pass lists them)
+                    const uint32_t TA = L.tile_buf[2 * (T + 1) + T];
                     while (steal < 8) {
                         uint32_t b = (xcd + steal) & 7u;
                         uint32_t lo = band_lo(TA, b), hi = band_lo(TA, b + 1);
                         uint32_t v = atomicAdd(&band_ctr[b], 1u);
-                        if (v < hi - lo) { t = L.tile_buf ? L.tile_buf[2 * (T + 1) + lo + v] : lo + v; break; }
+                        if (v < hi - lo) { t = L.tile_buf[2 * (T + 1) + lo + v]; break; }
                         steal++;
                     }
                 }
                 t = __shfl(t, 0, 64);
                 steal = __shfl(steal, 0, 64);
                 if (t == 0xffffffffu) { exhausted = true; break; }
-                if (!LEARNED && L.tile_buf && nown < kOwnTiles) {
+                if (nown < kOwnTiles) {
                     if (lane == 0) own[wave][nown] = t;
                     nown++;
                 }
@@ -617,18 +794,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
         if (__ballot(r.phase != PH_DONE) == 0) break;
         // The network is batched across the wave (the wave pays for it
         // whenever any lane needs it): it runs once half of the busy lanes wait
-        // for a proposal, or when every busy lane waits.  Waiting lanes sit out.
+        // for a proposal, or at the end (before the lane groups, which take no
+        // proposals).  Waiting lanes sit out.
         if (LEARNED) {
             const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
             const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
-            if (nwant > 0 && (2 * nwant >= nbusy || exhausted || nwant == nbusy))
+            if (nwant > 0 && (2 * nwant >= nbusy || exhausted))
                 wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
         }
-        if (!LEARNED && exhausted && group_ok) regroup(r, G, region);
+        if (exhausted && group_ok) regroup(r, G, region);
         if (G == 1) {
-            if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
-                if (use_lds) ray_step(L, r, DensLds<CAP, UNROLL>{ &L.g, buf }, cn);
-                else ray_step(L, r, DensPlain{ &L.g, false }, cn);
+            if (r.phase == PH_MARCH || r.phase == PH_BISECT) {
+                if (use_lds) ray_step<INSTR>(L, r, DensLds<CAP, 4>{ &L.g, buf }, cn);
+                else ray_step<INSTR>(L, r, DensPlain{ &L.g, false }, cn);
             }
         } else {
             const int j = (int)(lane & (uint32_t)(G - 1));
@@ -636,27 +814,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             bool nonskip = false, want = false;
             uint32_t cand = 0;
             if (r.phase != PH_DONE) want = group_prepare(L, r, G, j, qx, qy, qz, nonskip);
-            if (want) rho = use_lds ? pm_density32_lds<CAP, UNROLL>(L.g, qx, qy, qz, buf, cand)
+            if (want) rho = use_lds ? pm_density32_lds<CAP, 4>(L.g, qx, qy, qz, buf, cand)
                                     : pm_density32(L.g, qx, qy, qz, false, cand);
-            group_finish(L, r, G, j, want, nonskip, rho, cand, cn);
+            group_finish<INSTR>(L, r, G, j, want, nonskip, rho, cand, qx, qy, qz, cn);
         }
     }
     // The tiles this wave marched have all their rays done: their FP64 steps,
     // normals and outputs are computed here, while other waves still march
     // (k_finish then skips them).
-    if (!LEARNED && nown > 0) {
+    if (nown > 0) {
         wave_lds_sync();
         for (uint32_t i = 0; i < nown; i++) {
             const uint32_t t = own[wave][i];
-            finish_tile(L, t, buf, cn);
+            finish_tile<INSTR>(L, t, buf, cn);
             if (lane == 0) L.tile_buf[t] = 2u;
         }
     }
-    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
-    if (LEARNED) {
-        unsigned long long m = wave_sum(cn.mlp);
-        if (lane == 0 && m) atomicAdd(&ctr_slot(L.counters)[C_MLP], m);
-    }
+    flush_counts(L.counters, cn);
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n)
@@ -752,7 +926,7 @@ __global__ __launch_bounds__(256) void k_march_reference(MarchLaunch L)
 // ray by ray as in k_march_persist; a wave moves to its next chunk -- and
 // loads that job's grid and camera -- once all its lanes are idle.
 template <bool LEARNED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1 : 4, 8))) void k_march_batch(BatchLaunch B)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_march_batch(BatchLaunch B)
 {
     constexpr int kRegion = LEARNED ? 2112 : kLdsCap * 64;
     __shared__ float lds[4 * kRegion];
@@ -810,18 +984,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1
             if (exhausted) break;
             continue;
         }
-        // batched proposals, as in k_march_persist
+        // batched proposals, as in k_march_persist (before the lane groups)
+        const bool draining = exhausted || (qpos >= 64 && tile_i >= ntiles);
         if (LEARNED) {
             const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
             const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
-            if (nwant > 0 && (2 * nwant >= nbusy || exhausted || nwant == nbusy || (qpos >= 64 && tile_i >= ntiles)))
+            if (nwant > 0 && (2 * nwant >= nbusy || draining))
                 wave_propose(*Lp, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
         }
-        if (!LEARNED && group_ok && (exhausted || (qpos >= 64 && tile_i >= ntiles))) regroup(r, G, region);
+        if (group_ok && draining) regroup(r, G, region);
         if (G == 1) {
-            if (r.phase != PH_DONE && r.phase != PH_PROPOSE) {
-                if (use_lds) ray_step(*Lp, r, DensLds<kLdsCap, 4>{ &Lp->g, buf }, cn);
-                else ray_step(*Lp, r, DensPlain{ &Lp->g, false }, cn);
+            if (r.phase == PH_MARCH || r.phase == PH_BISECT) {
+                if (use_lds) ray_step<false>(*Lp, r, DensLds<kLdsCap, 4>{ &Lp->g, buf }, cn);
+                else ray_step<false>(*Lp, r, DensPlain{ &Lp->g, false }, cn);
             }
         } else {
             const int j = (int)(lane & (uint32_t)(G - 1));
@@ -831,21 +1006,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LEARNED ? 1
             if (r.phase != PH_DONE) want = group_prepare(*Lp, r, G, j, qx, qy, qz, nonskip);
             if (want) rho = use_lds ? pm_density32_lds<kLdsCap, 4>(Lp->g, qx, qy, qz, buf, cand)
                                     : pm_density32(Lp->g, qx, qy, qz, false, cand);
-            group_finish(*Lp, r, G, j, want, nonskip, rho, cand, cn);
+            group_finish<false>(*Lp, r, G, j, want, nonskip, rho, cand, qx, qy, qz, cn);
         }
     }
-    flush_counters(B.base.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
-    if (LEARNED) {
-        unsigned long long m = wave_sum(cn.mlp);
-        if (lane == 0 && m) atomicAdd(&ctr_slot(B.base.counters)[C_MLP], m);
-    }
+    flush_counts(B.base.counters, cn);
 }
 
-// Single launch: the tiles the march visited (its active-tile list, or all);
-// batch: the tiles of the chunks, one chunk per workgroup at a time.  Four
-// waves per SIMD (registers capped at 128) hide more of the walks' latency
-// than three would at 150 registers.
-template <bool BATCH>
+// Single launch: the active tiles no march wave finished; batch: the tiles of
+// the chunks, one chunk per workgroup at a time.  Four waves per SIMD
+// (registers capped at 128) hide more of the walks' latency than three would
+// at 150 registers.
+template <bool BATCH, bool INSTR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_finish(MarchLaunch L0, const MarchLaunch *__restrict__ jobs,
                                                 const BatchChunk *__restrict__ chunks, uint32_t nchunks)
 {
@@ -857,25 +1028,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
             const BatchChunk ch = chunks[c];
             const MarchLaunch *Lp = jobs + __builtin_amdgcn_readfirstlane(ch.job);
-            for (uint32_t t = wave; t < ch.ntiles; t += 4) finish_tile(*Lp, ch.tile0 + t, buf, cn);
+            for (uint32_t t = wave; t < ch.ntiles; t += 4) finish_tile<INSTR>(*Lp, ch.tile0 + t, buf, cn);
         }
     } else {
         const uint32_t T = ((L0.W + 7) / 8) * ((L0.H + 7) / 8);
-        const uint32_t TA = L0.tile_buf ? L0.tile_buf[2 * (T + 1) + T] : T;
+        const uint32_t TA = L0.tile_buf[2 * (T + 1) + T];
         for (uint32_t i = blockIdx.x * 4 + wave; i < TA; i += gridDim.x * 4) {
-            const uint32_t tile = L0.tile_buf ? L0.tile_buf[2 * (T + 1) + i] : i;
-            if (L0.tile_buf && L0.tile_buf[tile] == 2u) continue;     // finished by the march
-            finish_tile(L0, tile, buf, cn);
+            const uint32_t tile = L0.tile_buf[2 * (T + 1) + i];
+            if (L0.tile_buf[tile] == 2u) continue;     // finished by the march
+            finish_tile<INSTR>(L0, tile, buf, cn);
         }
     }
-    flush_counters(L0.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+    flush_counts(L0.counters, cn);
 }
 
 static hipError_t launch_finish(const MarchLaunch &L0, hipStream_t st)
 {
     const uint32_t T = ((L0.W + 7) / 8) * ((L0.H + 7) / 8);
     const uint32_t b = (T + 3) / 4;
-    hipLaunchKernelGGL((k_finish<false>), dim3(b < 2048 ? b : 2048), dim3(256), 0, st, L0, nullptr, nullptr, 0u);
+    const dim3 grid(b < 2048 ? b : 2048), blk(256);
+    if (L0.s.instrument) hipLaunchKernelGGL((k_finish<false, true>), grid, blk, 0, st, L0, nullptr, nullptr, 0u);
+    else hipLaunchKernelGGL((k_finish<false, false>), grid, blk, 0, st, L0, nullptr, nullptr, 0u);
     return hipGetLastError();
 }
 
@@ -886,7 +1059,7 @@ hipError_t launch_march_batch(const BatchLaunch &B, uint32_t blocks, hipStream_t
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint32_t fb = B.nchunks < 8192 ? B.nchunks : 8192;
-    hipLaunchKernelGGL((k_finish<true>), dim3(fb), dim3(256), 0, st, B.base, B.jobs, B.chunks, B.nchunks);
+    hipLaunchKernelGGL((k_finish<true, false>), dim3(fb), dim3(256), 0, st, B.base, B.jobs, B.chunks, B.nchunks);
     return hipGetLastError();
 }
 
@@ -930,60 +1103,50 @@ hipError_t launch_miss_tiles(const MarchLaunch &L, uint32_t T, hipStream_t st)
     return hipGetLastError();
 }
 
-static hipError_t launch_march_kernel(const MarchLaunch &L, hipStream_t st);
-
-hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
+// The configured isotropic march of one view: schedule 1, or the tile
+// pre-pass, the persistent march and the finish of the tiles its waves left
+// (L.tile_buf and L.hit_rec set).
+static hipError_t launch_march_config(const MarchLaunch &L, hipStream_t st)
 {
-    if (L.s.anisotropic) return launch_march_aniso(L, st);
-    uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
-    if (L.s.mode == PM_MODE_REFERENCE) {
-        hipLaunchKernelGGL(k_march_reference, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
+    const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
+    if (L.s.schedule == 1 && !L.s.learned) {
+        hipLaunchKernelGGL(k_march_simple, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
         return hipGetLastError();
     }
-    hipError_t e = launch_march_kernel(L, st);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L.band_ctr, 8);
+    uint32_t *flag = L.tile_buf, *pos = L.tile_buf + (tiles + 1);
+    hipLaunchKernelGGL(k_tile_flags, dim3((tiles + 1 + 3) / 4), dim3(256), 0, st, L, tiles);
+    hipError_t e = launch_scan_u32(flag, pos, tiles + 1, L.tile_buf + 3 * (tiles + 1), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tile_compact, dim3((tiles + 1 + 255) / 256), dim3(256), 0, st, L.tile_buf, tiles);
+    hipLaunchKernelGGL(k_miss_tiles, dim3((tiles + 3) / 4), dim3(256), 0, st, L, tiles);
+    // enough resident waves to fill the chip; extra waves find no tile and exit
+    uint32_t blocks = L.persist_blocks ? L.persist_blocks : 1024;
+    const uint32_t need = (tiles + 3) / 4;
+    if (blocks > need) blocks = (need + 7) & ~7u;
+    const dim3 grid(blocks), blk(256);
+    const bool instr = L.s.instrument != 0;
+    if (L.s.learned) {
+        if (instr) hipLaunchKernelGGL((k_march_persist<true, true>), grid, blk, 0, st, L, L.band_ctr);
+        else hipLaunchKernelGGL((k_march_persist<true, false>), grid, blk, 0, st, L, L.band_ctr);
+    } else {
+        if (instr) hipLaunchKernelGGL((k_march_persist<false, true>), grid, blk, 0, st, L, L.band_ctr);
+        else hipLaunchKernelGGL((k_march_persist<false, false>), grid, blk, 0, st, L, L.band_ctr);
+    }
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_finish(L, st);
 }
 
-static hipError_t launch_march_kernel(const MarchLaunch &L, hipStream_t st)
+hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
 {
-    uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
-    dim3 tb((tiles + 3) / 4), blk(256);
-    switch (L.s.learned ? 0 : L.s.variant) {
-    case 1: hipLaunchKernelGGL((k_march_tile<false, 1>), tb, blk, 0, st, L); return hipGetLastError();
-    case 4: hipLaunchKernelGGL((k_march_tile<true, 4>), tb, blk, 0, st, L); return hipGetLastError();
-    case 5: hipLaunchKernelGGL((k_march_tile<true, 8>), tb, blk, 0, st, L); return hipGetLastError();
-    default: break;
-    }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L.band_ctr, 8);
-    if (L.tile_buf) {
-        uint32_t *flag = L.tile_buf, *pos = L.tile_buf + (tiles + 1);
-        hipLaunchKernelGGL(k_tile_flags, dim3((tiles + 1 + 3) / 4), dim3(256), 0, st, L, tiles);
-        hipError_t e = launch_scan_u32(flag, pos, tiles + 1, L.tile_buf + 3 * (tiles + 1), st);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_tile_compact, dim3((tiles + 1 + 255) / 256), dim3(256), 0, st, L.tile_buf, tiles);
-        hipLaunchKernelGGL(k_miss_tiles, dim3((tiles + 3) / 4), dim3(256), 0, st, L, tiles);
-    }
-    // enough resident waves to fill the chip; extra waves find no tile and exit
-    uint32_t blocks = L.persist_blocks ? L.persist_blocks : 1024;
-    uint32_t need = (tiles + 3) / 4;
-    if (blocks > need) blocks = (need + 7) & ~7u;
-    if (L.s.learned) {
-        hipLaunchKernelGGL((k_march_persist<4, true, kLdsCap, 1>), dim3(blocks), blk, 0, st, L, L.band_ctr);
+    if (L.s.anisotropic) return launch_march_aniso(L, st);
+    if (L.s.mode == PM_MODE_REFERENCE) {
+        const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
+        hipLaunchKernelGGL(k_march_reference, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
         return hipGetLastError();
     }
-    switch (L.s.variant) {
-    case 2: hipLaunchKernelGGL((k_march_persist<1, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
-    case 3: hipLaunchKernelGGL((k_march_persist<8, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
-    case 6: {   // registers capped for five waves per SIMD
-        uint32_t ob = L.persist_blocks ? L.persist_blocks : 1280u;
-        if (ob > need) ob = (need + 7) & ~7u;
-        hipLaunchKernelGGL((k_march_persist<4, false, 32, 5>), dim3(ob), blk, 0, st, L, L.band_ctr);
-        break;
-    }
-    default: hipLaunchKernelGGL((k_march_persist<4, false>), dim3(blocks), blk, 0, st, L, L.band_ctr); break;
-    }
-    return hipGetLastError();
+    return launch_march_config(L, st);
 }
 
 // ---- point queries used by the parity tests ----

@@ -80,15 +80,18 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a)
         float t = tn + (float)k * a.step;
         if (t >= tend) break;
         float q[3] = { a.cam[0] + t * d[0], a.cam[1] + t * d[1], a.cam[2] + t * d[2] };
-        float rho, gx, gy, gz;
+        // the features the learned march computes at a sample below iso: its
+        // (compensated) density, grad rho over the fine stencil, the 27-cell count
         uint32_t cand = 0;
-        density_grad_feat(g, q[0], q[1], q[2], rho, gx, gy, gz, cand);
-        if (rho >= a.iso) break;                 // at or past the crossing in FP32-plain terms
+        const float rho = pm_density32(g, q[0], q[1], q[2], true, cand);
+        if (rho >= a.iso) break;                 // at or past the crossing
         uint32_t row = atomicAdd(a.count, 1u);
         if (row >= a.cap) break;
+        float gx, gy, gz;
+        grad_feat(g, q[0], q[1], q[2], gx, gy, gz, cand);
         float f[PM_MLP_IN];
         float gmag = sqrtf((gx * gx + gy * gy) + gz * gz);
-        mlp_features(g, a.cg, a.cam, d, t, tn, tf, rho, gmag, (float)cand, a.iso, f);
+        mlp_features(g, a.cg, a.cam, d, t, tn, tf, rho, gmag, (float)pm_cand27(g, q[0], q[1], q[2]), a.iso, f);
         for (int j = 0; j < PM_MLP_IN; j++) a.feat[(uint64_t)row * PM_MLP_IN + j] = f[j];
         float dist = (tend - t) * g.inv_h;
         a.label[row] = log2f(1.0f + fminf(dist, 255.0f));

@@ -8,13 +8,11 @@
 
 namespace pm {
 
-// PH_BISECT: FP32 bisection steps (the hit then goes to k_finish: the FP64
-// step, the normal and the outputs).
-// PH_PROPOSE: waiting for a step proposal (taken by the whole wave at once);
-// PH_LAND: density at the proposed sample; PH_BSEARCH: binary search for the
-// first sample at or above iso between the last sample below iso and the landing.
+// PH_MARCH: the next lattice sample; PH_BISECT: FP32 bisection steps (the
+// anisotropic march and the one-ray-per-lane schedule); PH_PROPOSE: waiting for
+// a step proposal of the learned march (taken by the whole wave at once).
 enum Phase : int {
-    PH_MARCH = 0, PH_BISECT = 1, PH_DONE = 5, PH_PROPOSE = 6, PH_LAND = 7, PH_BSEARCH = 8
+    PH_MARCH = 0, PH_BISECT = 1, PH_DONE = 5, PH_PROPOSE = 6
 };
 
 // glm mat4 * vec4 (type_mat4x4.inl): (m0 x + m1 y) + (m2 z + m3 w), column-major M.
@@ -132,6 +130,29 @@ __device__ __forceinline__ void flush_counters(unsigned long long *ctr, uint32_t
     }
 }
 
+// Per-lane counters of one launch (pm_stats): cand counts every candidate a
+// walk reads; mcand those of the march samples' walks; cst and c27 (instrumented
+// launches only) the candidates of every sample's own fine stencil and of its
+// 27 search cells, however the sample was evaluated.
+struct Counts {
+    uint32_t rays = 0, in_aabb = 0, hits = 0, samples = 0, march = 0, mlp = 0, cert = 0, cskip = 0;
+    unsigned long long cand = 0, mcand = 0, cst = 0, c27 = 0;
+};
+
+__device__ __forceinline__ void flush_counts(unsigned long long *ctr, const Counts &cn)
+{
+    const unsigned long long v[C_COUNT] = {
+        wave_sum(cn.rays), wave_sum(cn.in_aabb), wave_sum(cn.hits), wave_sum(cn.samples), wave_sum(cn.cand),
+        wave_sum(cn.march), wave_sum(cn.mlp), wave_sum(cn.mcand), wave_sum(cn.cert), wave_sum(cn.cskip),
+        wave_sum(cn.cst), wave_sum(cn.c27) };
+    if ((threadIdx.x & 63) == 0) {
+        unsigned long long *p = ctr_slot(ctr);
+#pragma unroll
+        for (int i = 0; i < C_COUNT; i++)
+            if (v[i]) atomicAdd(&p[i], v[i]);
+    }
+}
+
 // XCD-aware tile of this wave; false if the wave has no tile.
 __device__ __forceinline__ bool pm_tile(uint32_t W, uint32_t H, uint32_t &px, uint32_t &py)
 {
@@ -155,15 +176,12 @@ struct RayState {
     float lo, hi;         // bisection bracket
     int k, it, phase;     // sample index, bisection step
     uint32_t idx;
-    int klo, khi;         // learned step: last sample below iso / landing sample
-    int lrn;              // 1 while the ray still takes proposals
+    int lrn;              // learned march: 1 while the ray may ask for a proposal
+    float rho;            // learned march: density of sample k (below iso) for the proposal
     float id0, id1, id2;  // 1 / direction (bound skip)
 };
 
-struct Counts {
-    uint32_t rays = 0, in_aabb = 0, hits = 0, samples = 0, march = 0, mlp = 0;
-    unsigned long long cand = 0;
-};
+
 
 __device__ __forceinline__ void write_miss(const MarchLaunch &L, uint32_t idx)
 {
@@ -204,10 +222,6 @@ __device__ __forceinline__ void ray_setup(const MarchLaunch &L, RayState &r, uin
         r.phase = PH_MARCH;
         r.k = 0;
         r.lrn = L.s.learned;
-        if (r.lrn) {            // first proposal at the entry; klo = -1 stands for "before the entry"
-            r.klo = -1;
-            r.phase = PH_PROPOSE;
-        }
         cn.in_aabb++;
     } else {
         march_miss(L, r.idx);

@@ -29,18 +29,19 @@ class PmSettings(C.Structure):
         ("mode", C.c_int32), ("max_steps", C.c_int32), ("step", C.c_float), ("iso", C.c_float),
         ("bisect_iters", C.c_int32), ("final_fp64", C.c_int32), ("normal_eps", C.c_float),
         ("normal_mode", C.c_int32), ("skip_mode", C.c_int32), ("compensated", C.c_int32),
-        ("learned", C.c_int32), ("variant", C.c_int32), ("learned_alpha", C.c_float),
+        ("learned", C.c_int32), ("schedule", C.c_int32), ("learned_alpha", C.c_float),
         ("learned_min_skip", C.c_int32), ("learned_max_skip", C.c_int32),
         ("anisotropic", C.c_int32), ("k_n", C.c_float), ("k_r", C.c_float), ("k_s", C.c_float),
-        ("n_eps", C.c_int32), ("ext_mult", C.c_float), ("reserved", C.c_int32 * 3),
+        ("n_eps", C.c_int32), ("ext_mult", C.c_float), ("instrument", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
 class PmStats(C.Structure):
     _fields_ = [
         ("rays", C.c_uint64), ("rays_in_aabb", C.c_uint64), ("hits", C.c_uint64), ("samples", C.c_uint64),
-        ("cand", C.c_uint64), ("march_samples", C.c_uint64), ("mlp_evals", C.c_uint64), ("reserved", C.c_uint64),
-        ("kernel_ms", C.c_float), ("total_ms", C.c_float),
+        ("cand", C.c_uint64), ("march_samples", C.c_uint64), ("mlp_evals", C.c_uint64), ("march_cand", C.c_uint64),
+        ("cert_walks", C.c_uint64), ("cert_skipped", C.c_uint64), ("cand_stencil", C.c_uint64),
+        ("cand27", C.c_uint64), ("reserved", C.c_uint64), ("kernel_ms", C.c_float), ("total_ms", C.c_float),
     ]
 
     def as_dict(self):

@@ -74,17 +74,15 @@ typedef struct pm_settings {
     int32_t skip_mode;     /* PM_SKIP_*                                               */
     int32_t compensated;   /* 1 = Neumaier-compensated FP32 density sums              */
     int32_t learned;       /* 1 = learned step proposal (needs pm_set_mlp)            */
-    int32_t variant;       /* kernel schedule (same results, for A/B): 0 = persistent
-                              waves with ray refill; 1 = one 8x8 tile per wave, direct
-                              sums; 2, 3 = persistent with 1 or 8 loads in flight;
-                              4, 5 = one tile per wave with compaction; 6 = persistent
-                              with registers capped for 5 waves per SIMD; 7 = persistent
-                              over every tile (the persistent schedules otherwise
-                              take only tiles with a ray entering the AABB and write
-                              the others' misses in a pre-pass)                       */
+    int32_t schedule;      /* kernel schedule, same images and counters: 0 = persistent
+                              waves over the tiles a ray of which enters the AABB, with
+                              ray refill, lane groups and per-wave finishing (the
+                              product); 1 = one 8x8 tile per wave, one ray per lane to
+                              its end, direct sums (the plain schedule the persistent
+                              one is checked against)                                */
     float   learned_alpha; /* learned: fraction of the predicted distance skipped (0 = 0.7) */
-    int32_t learned_min_skip; /* learned: shorter proposals end the proposals (0 = 4)  */
-    int32_t learned_max_skip; /* learned: longest jump in steps (0 = 24)              */
+    int32_t learned_min_skip; /* learned: shorter proposals are not taken (0 = 3, >= 2) */
+    int32_t learned_max_skip; /* learned: longest jump in steps (0 = 8)               */
     /* PerPixel_Anisotropic (RayMarcher.cpp:346-423): per sample, WPCA over the
        particles within ext_mult * h (RayMarcher.cpp:114-254), then the
        anisotropic kernel (Kernel.cpp:63-107) over those within h.  Either
@@ -96,21 +94,34 @@ typedef struct pm_settings {
     int32_t n_eps;         /* N_eps                                                   */
     float   ext_mult;      /* ParticleRadiusExt / ParticleRadius (Dataset.cpp:169-177),
                               >= 1; 0 = 2 (config.yml particleRadiusMultiplier)       */
-    int32_t reserved[3];
+    int32_t instrument;    /* configured isotropic march, schedule 0: 1 = also count,
+                              per density evaluation, the candidates of its own fine
+                              stencil and of its 27 search cells (pm_stats.cand_stencil,
+                              cand27; same images, slower)                          */
+    int32_t reserved[2];
 } pm_settings;
 
-/* Per-launch counters, written by the march kernel itself. */
+/* Per-launch counters, written by the march kernels themselves. */
 typedef struct pm_stats {
-    uint64_t rays;          /* primary rays (W*H per view)                   */
-    uint64_t rays_in_aabb;  /* rays that enter the padded particle AABB      */
-    uint64_t hits;          /* rays with an iso crossing                     */
-    uint64_t samples;       /* density evaluations of every kind             */
-    uint64_t cand;          /* sum over samples of particles in the 27 cells */
-    uint64_t march_samples; /* density evaluations of the march phase        */
-    uint64_t mlp_evals;     /* learned step: network evaluations             */
+    uint64_t rays;          /* primary rays (W*H per view)                                */
+    uint64_t rays_in_aabb;  /* rays that enter the padded particle AABB                   */
+    uint64_t hits;          /* rays with an iso crossing                                  */
+    uint64_t samples;       /* density evaluations: march samples, bisection samples (FP32
+                               and FP64) and central-difference samples                   */
+    uint64_t cand;          /* candidates read by the launch's FP32 walks, each walk once
+                               (a hit's six central differences share one walk)          */
+    uint64_t march_samples; /* density evaluations of the march phase                     */
+    uint64_t mlp_evals;     /* learned step: network evaluations                          */
+    uint64_t march_cand;    /* candidates of the march samples' own fine stencils         */
+    uint64_t cert_walks;    /* learned step: segment certificates walked                  */
+    uint64_t cert_skipped;  /* learned step: lattice samples certified and not evaluated  */
+    uint64_t cand_stencil;  /* instrumented launches: sum over every density evaluation of
+                               the candidates of its own fine stencil                      */
+    uint64_t cand27;        /* instrumented launches: sum over every density evaluation of
+                               the particles in its 27 search cells (cells of h)           */
     uint64_t reserved;
-    float    kernel_ms;     /* march kernel time, HIP events on the context stream */
-    float    total_ms;      /* launch to completion incl. copies             */
+    float    kernel_ms;     /* march kernels' time, HIP events on the context stream      */
+    float    total_ms;      /* launch to completion incl. copies                          */
 } pm_stats;
 
 /* Frame description after pm_set_frame (the GPU grid of Frame::BuildSearch). */
@@ -299,9 +310,12 @@ int pm_depth_sprites_device(pm_ctx *ctx, int32_t frame, const float view[16], co
  * role of the DensityGrid skip, Dataset.cpp:94-164, RayMarcher.cpp:284-304) ----
  * A 24 -> 64 -> 64 -> 1 MLP, BF16 weights with FP32 accumulation on MFMA,
  * predicts log2(1 + d/h) for d the distance along the ray to the first iso
- * crossing.  The configured march with learned = 1 jumps ahead on the same
- * sample lattice, binary-searches back to the first sample at or above iso and
- * then bisects exactly as the plain march does.
+ * crossing.  The configured march with learned = 1 keeps the exact skips; at a
+ * sample below iso they do not pass, it may jump n lattice samples ahead, but
+ * only when a bound of the density over the segment of the n - 1 samples it
+ * passes (the sum of W at each particle's distance to the segment) proves them
+ * below iso.  The first sample at or above iso -- and the image -- are
+ * therefore the plain march's.
  * Parameters: PM_MLP_NPARAM floats, W1[64][32] (inputs 24..31 zero), b1[64],
  * W2[64][64], b2[64], w3[64], b3. */
 #define PM_MLP_INPUTS 24

@@ -53,6 +53,9 @@ class Counters(C.Structure):
         ("samples", C.c_uint64),
         ("cand", C.c_uint64),
         ("march_samples", C.c_uint64),
+        ("march_cand", C.c_uint64),
+        ("cand_stencil", C.c_uint64),
+        ("cand27", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -117,6 +120,10 @@ def lib():
         L.or_neighbors_fine.restype = C.c_uint32
         L.or_neighbors_fine.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), fp(np.float32, flags="C"),
                                         C.c_uint32, C.POINTER(C.c_uint32)]
+        L.or_stencil_cand.restype = C.c_uint32
+        L.or_stencil_cand.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C")]
+        L.or_cand27.restype = C.c_uint32
+        L.or_cand27.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C")]
         L.or_cdiff_union.restype = C.c_int
         L.or_cdiff_union.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), C.c_float,
                                      C.POINTER(C.c_uint32), C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
@@ -286,6 +293,16 @@ class Grid:
         cand = C.c_uint32(0)
         n = lib().or_neighbors_fine(C.byref(self.g), q, out, 1 << 15, C.byref(cand))
         return out[:3 * n].reshape(-1, 3).copy(), int(cand.value)
+
+    def stencil_cand(self, q) -> np.ndarray:
+        """Per point: candidates of its own trimmed fine stencil (no particle read)."""
+        q = np.ascontiguousarray(np.asarray(q, np.float32).reshape(-1, 3))
+        return np.array([lib().or_stencil_cand(C.byref(self.g), p) for p in q], np.uint64)
+
+    def cand27(self, q) -> np.ndarray:
+        """Per point: particles stored in the 27 search cells around its cell."""
+        q = np.ascontiguousarray(np.asarray(q, np.float32).reshape(-1, 3))
+        return np.array([lib().or_cand27(C.byref(self.g), p) for p in q], np.uint64)
 
     def cdiff_union(self, q, eps):
         """The fused central-difference walk around q: (candidate count,

@@ -1146,6 +1146,41 @@ static void sum_gradW(dens_ctx *dc, const float q[3], float out[3], or_counters 
     (void)cnt; (void)cand;
 }
 
+uint32_t or_stencil_cand(const or_grid *g, const float q[3])
+{
+    or_fine fs;
+    uint32_t cand = 0;
+    if (!or_fine_centre(g, q, &fs)) return 0;
+    for (int dz = -2; dz <= 2; dz++)
+        for (int dy = -2; dy <= 2; dy++) {
+            uint32_t b, e;
+            if (or_fine_row(g, &fs, dy, dz, &b, &e)) cand += e - b;
+        }
+    return cand;
+}
+
+uint32_t or_cand27(const or_grid *g, const float q[3])
+{
+    int32_t c[3];
+    uint32_t cand = 0;
+    if (!stencil_centre(g, q[0], q[1], q[2], c)) return 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            cand += g->cell_start[cell_lin(g, c[0] + 1, c[1] + dy, c[2] + dz) + 1] -
+                    g->cell_start[cell_lin(g, c[0] - 1, c[1] + dy, c[2] + dz)];
+    return cand;
+}
+
+/* The per-sample counts of a configured isotropic march over the dense grid. */
+static void count_sample(const dens_ctx *dc, or_counters *cnt, const float q[3], int march)
+{
+    if (dc->hs || dc->an) return;
+    const uint32_t c = or_stencil_cand(dc->g, q);
+    cnt->cand_stencil += c;
+    cnt->cand27 += or_cand27(dc->g, q);
+    if (march) cnt->march_cand += c;
+}
+
 /* ------------------------------------------------------------------------- */
 /* One pixel of the configured march: AABB entry, fixed step, first sample    */
 /* with rho >= iso, bisection (last iteration in FP64), normals.              */
@@ -1185,6 +1220,7 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
         }
         float rho = dens_f(dc, x, cnt);
         cnt->march_samples++;
+        count_sample(dc, cnt, x, 1);
         if (rho >= s->iso) { hitk = k; break; }
         k++;
     }
@@ -1202,6 +1238,7 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
             float tm = (lo + hi) * 0.5f;
             float x[3] = { o[0] + tm * d[0], o[1] + tm * d[1], o[2] + tm * d[2] };
             float rho = dens_f(dc, x, cnt);
+            count_sample(dc, cnt, x, 0);
             if (rho >= s->iso) hi = tm; else lo = tm;
         }
         if (s->final_fp64 && s->bisect_iters > 0) {
@@ -1211,6 +1248,8 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
                             (double)o[2] + tm * (double)d[2] };
             double rho = dc->an ? or_density_aniso_d(g, dc->an, x) : or_density_d(g, x);
             cnt->samples++;
+            const float xf[3] = { (float)x[0], (float)x[1], (float)x[2] };
+            count_sample(dc, cnt, xf, 0);
             if (rho >= (double)s->iso) dhi = tm; else dlo = tm;
             double tt = (dlo + dhi) * 0.5;
             for (int a = 0; a < 3; a++) xs[a] = (float)((double)o[a] + tt * (double)d[a]);
@@ -1238,6 +1277,8 @@ static void pixel_config(dens_ctx *dc, const or_settings *s, const float M[16], 
             xm[a] = xs[a] - s->normal_eps;
             float rp = dens_f(dc, xp, c6);
             float rm = dens_f(dc, xm, c6);
+            count_sample(dc, cnt, xp, 0);
+            count_sample(dc, cnt, xm, 0);
             gv[a] = rp - rm;
         }
         if (fused) {
@@ -1374,6 +1415,7 @@ static void *march_worker(void *arg)
     pthread_mutex_lock(&j->lock);
     j->cnt.rays += c.rays; j->cnt.rays_in_aabb += c.rays_in_aabb; j->cnt.hits += c.hits;
     j->cnt.samples += c.samples; j->cnt.cand += c.cand; j->cnt.march_samples += c.march_samples;
+    j->cnt.march_cand += c.march_cand; j->cnt.cand_stencil += c.cand_stencil; j->cnt.cand27 += c.cand27;
     pthread_mutex_unlock(&j->lock);
     return NULL;
 }

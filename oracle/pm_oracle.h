@@ -53,8 +53,17 @@ typedef struct or_counters {
     uint64_t rays_in_aabb;  /* rays that intersect the padded AABB         */
     uint64_t hits;          /* rays that found an iso crossing             */
     uint64_t samples;       /* density evaluations of every kind           */
-    uint64_t cand;          /* sum over samples of stencil candidates      */
+    uint64_t cand;          /* sum over samples of stencil candidates (the
+                               six central-difference samples: their fused
+                               walk's candidates once)                     */
     uint64_t march_samples; /* density evaluations of the march phase      */
+    /* configured isotropic march over the dense grid only (0 otherwise):   */
+    uint64_t march_cand;    /* march samples' own fine-stencil candidates  */
+    uint64_t cand_stencil;  /* every density evaluation's own fine-stencil
+                               candidates (the FP64 sample: those of its
+                               FP32-rounded point)                         */
+    uint64_t cand27;        /* every density evaluation's 27 search-cell
+                               candidates                                  */
 } or_counters;
 
 /* Dense uniform grid, cell width h, cell keys as CompactNSearch computes them
@@ -188,6 +197,11 @@ int  or_slab(const float o[3], const float d[3], const float bmin[3], const floa
 int or_march(const or_grid *g, const or_settings *s, const float invpv[16], const float cam[3],
              uint32_t W, uint32_t H, const float *depth, uint32_t row_step, int nthreads,
              float *pos, float *nrm, float *tout, or_counters *cnt);
+/* Candidates of q's own trimmed fine stencil (no particle read) and the
+ * particles in the 27 search cells around q's cell (0 when those leave the
+ * grid): the per-sample counts of the GPU's instrumented launches. */
+uint32_t or_stencil_cand(const or_grid *g, const float q[3]);
+uint32_t or_cand27(const or_grid *g, const float q[3]);
 int or_march_f64(const or_grid *g, const or_settings *s, const float invpv[16], const float cam[3],
                  uint32_t W, uint32_t H, uint32_t row_step, int nthreads,
                  double *pos, double *nrm, double *tout);

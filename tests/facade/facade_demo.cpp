@@ -59,12 +59,17 @@ int main(int argc, char** argv)
     vs.k_s = ks;
     vs.N_eps = neps;
 
+    // the staging buffers hold stale data, as mapped Vulkan memory does: the
+    // facade must overwrite every pixel (or answer all of them as misses)
     std::vector<glm::vec4> pos((size_t)W * H), nrm((size_t)W * H);
+    for (size_t i = 0; i < pos.size(); i++)
+        for (int a = 0; a < 4; a++) { pos[i][a] = -7.0f; nrm[i][a] = -7.0f; }
     RayMarcher rm;
     rm.Prepare(vs, ctl, &ds, pos.data(), nrm.data(), depth.data());
     rm.Start();
     int32_t polls = 0;
     while (!rm.IsDone()) polls++;
+    if (rm.LastError()[0]) std::fprintf(stderr, "facade_demo: last error: %s\n", rm.LastError());
     rm.Exit();
 
     FILE* o = std::fopen(argv[2], "wb");

@@ -39,14 +39,6 @@ def assert_within(m, ext, normal_key="normal_max_deg"):
     assert m[normal_key] <= NORMAL_DEG, m
 
 
-def assert_within_budget(m, ext):
-    """The learned march against the plain one: the same 0.1 % outlier budget the
-    mask rule allows is applied to depth and normal errors (99.9th percentile)."""
-    assert m["mask_agree"] >= MASK_MIN, m
-    assert m["depth_p999"] <= DEPTH_REL * ext, (m, DEPTH_REL * ext)
-    assert m["normal_p999_deg"] <= NORMAL_DEG, m
-
-
 def depth_in_front(cam, bmin, bmax, margin):
     """A depth buffer for the reference march: the NDC depth of the point `margin`
     in front of each ray's entry into [bmin, bmax] (1.0, the clear value, where the

@@ -99,15 +99,15 @@ def test_march_aniso_config_bitexact(marcher, cam_i, normal_mode, skip):
 
 
 def test_march_aniso_schedules_agree(marcher):
-    """One ray per lane to its end (variant 0) and the persistent march +
-    hit-list finish (variant 2) give the same images and counters."""
+    """One ray per lane to its end (schedule 0) and the persistent march +
+    hit-list finish (schedule 1) give the same images and counters."""
     sc = scenes.lattice_cube(1)
     cam = sc.cameras[0]
-    print("aniso config march lattice_cube variants 0/2", flush=True)
+    print("aniso config march lattice_cube schedules 0/1", flush=True)
     marcher.set_frame(0, sc.xyz, sc.h)
     out = []
-    for v in (0, 2):
-        st = scenes.aniso_settings(sc, skip_mode=1, variant=v)
+    for v in (0, 1):
+        st = scenes.aniso_settings(sc, skip_mode=1, schedule=v)
         out.append(marcher.march(0, st, cam.inv_pv, cam.eye, cam.width, cam.height))
     for a, b in zip(out[0][:3], out[1][:3]):
         assert np.array_equal(a, b)
@@ -121,8 +121,8 @@ def test_march_aniso_dam_bitexact(marcher):
     cam = sc.cameras[0]
     print("aniso config march dam_break_aniso 480x270", flush=True)
     marcher.set_frame(3, sc.xyz, sc.h)
-    for skip, variant in ((1, 0), (4, 0), (4, 2)):
-        st = scenes.aniso_settings(sc, skip_mode=skip, variant=variant)
+    for skip, schedule in ((1, 0), (4, 0), (4, 1)):
+        st = scenes.aniso_settings(sc, skip_mode=skip, schedule=schedule)
         pos, stats = _march_pair(marcher, 3, sc, st, cam, 1)
         assert stats["hits"] > 10000
 

@@ -61,7 +61,7 @@ def test_batch_graph_replays_match_eager(marcher):
         marcher.batch_run()
         marcher.wait()
         s = marcher.stats()
-        for k in ("rays", "hits", "samples", "cand", "march_samples"):
+        for k in ("rays", "hits", "samples", "march_cand", "march_samples"):
             assert s[k] == eager_stats[k], (rep, k)
         for i in range(len(views)):
             for x, y in zip(marcher.batch_download(i), eager[i]):

@@ -41,7 +41,7 @@ def test_bound_march_matches_oracle(marcher, name):
     p0, n0, t0, c0 = O.march(G, _oracle_settings(sc, skip_mode=0), c.inv_pv, c.eye, c.width, c.height)
     assert np.array_equal(pos, p0) and np.array_equal(nrm, n0) and np.array_equal(t, t0)
     p4, n4, t4, c4 = O.march(G, _oracle_settings(sc, skip_mode=4), c.inv_pv, c.eye, c.width, c.height)
-    for k in ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples"):
+    for k in ("rays", "rays_in_aabb", "hits", "samples", "march_cand", "march_samples"):
         assert st[k] == c4[k], (k, st[k], c4[k])
     print("march samples", c0["march_samples"], "->", st["march_samples"], flush=True)
     assert st["march_samples"] < c0["march_samples"]

@@ -49,7 +49,7 @@ def test_clustered_pool_4k_size_independent(marcher, pool4k):
     marcher.set_frame(50, sc.xyz, sc.h)
     a = marcher.march(50, sc.settings(skip_mode=0), c.inv_pv, c.eye, c.width, c.height)
     b = marcher.march(50, sc.settings(skip_mode=4), c.inv_pv, c.eye, c.width, c.height)
-    d = marcher.march(50, sc.settings(skip_mode=4, variant=7), c.inv_pv, c.eye, c.width, c.height)
+    d = marcher.march(50, sc.settings(skip_mode=4, schedule=1), c.inv_pv, c.eye, c.width, c.height)
     for x, y, z in zip(a[:3], b[:3], d[:3]):
         assert np.array_equal(x, y) and np.array_equal(x, z)
     pos, nrm, t = a[:3]
@@ -97,13 +97,12 @@ def test_error_vs_fp64_oracle(marcher, name):
     assert_within(m, extent(sc.xyz))
 
 
-def test_wave_tile_overflow_matches_finish_kernel(marcher):
+def test_camera_inside_aabb_many_tiles(marcher):
     """A 5120x2880 view from inside the particles' AABB: every one of the
-    230,400 tiles is active and most rays cross empty space quickly, so waves of
-    the persistent march take far more than the 32 tiles they finish themselves;
-    the rest go to the finish kernel.  The image equals the schedule without the
-    tile pre-pass (every tile finished by the finish kernel) and the oracle on a
-    row sample."""
+    230,400 tiles is active, most rays cross empty space quickly and each wave
+    of the persistent march takes many tiles; the finish kernel packs the hits
+    of runs of tiles.  The image equals the one-ray-per-lane schedule and the
+    oracle on a row sample."""
     rng = np.random.default_rng(21)
     blobs = [rng.normal(cen, 0.03, (5000, 3)) for cen in ((0.3, 0.5, 0.7), (0.6, 0.4, 0.8), (0.5, 0.6, 0.5))]
     xyz = np.concatenate(blobs + [rng.uniform(0.0, 1.0, (5000, 3))]).astype(np.float32)
@@ -113,13 +112,13 @@ def test_wave_tile_overflow_matches_finish_kernel(marcher):
     marcher.set_frame(80, sc.xyz, sc.h)
     st = sc.settings(skip_mode=4)
     a = marcher.march(80, st, c.inv_pv, c.eye, c.width, c.height)
-    b = marcher.march(80, dict(st, variant=7), c.inv_pv, c.eye, c.width, c.height)
+    b = marcher.march(80, dict(st, schedule=1), c.inv_pv, c.eye, c.width, c.height)
     print(a[3], flush=True)
     assert a[3]["rays_in_aabb"] == c.width * c.height
     assert a[3]["hits"] > 100_000
     for x, y in zip(a[:3], b[:3]):
         assert np.array_equal(x, y)
-    for k in ("hits", "samples", "cand", "march_samples"):
+    for k in ("hits", "samples", "march_cand", "march_samples"):
         assert a[3][k] == b[3][k], k
     G = O.Grid(sc.xyz, sc.h)
     step = 360

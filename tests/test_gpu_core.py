@@ -80,13 +80,13 @@ def test_density_bitexact(marcher, compensated):
     assert np.array_equal(marcher.density64(3, q64), G.density64(q64))
 
 
-@pytest.mark.parametrize("seed,cam,variant", [(0, 0, 0), (1, 1, 0), (2, 0, 0), (0, 0, 1), (1, 1, 1)])
-def test_march_lattice_cube(marcher, seed, cam, variant):
+@pytest.mark.parametrize("seed,cam,schedule", [(0, 0, 0), (1, 1, 0), (2, 0, 0), (0, 0, 1), (1, 1, 1)])
+def test_march_lattice_cube(marcher, seed, cam, schedule):
     sc = scenes.lattice_cube(seed)
     c = sc.cameras[cam]
-    print(f"march lattice_cube seed={seed} cam={cam} variant={variant} {c.width}x{c.height}", flush=True)
+    print(f"march lattice_cube seed={seed} cam={cam} schedule={schedule} {c.width}x{c.height}", flush=True)
     marcher.set_frame(0, sc.xyz, sc.h)
-    st = sc.settings(variant=variant)
+    st = sc.settings(schedule=schedule)
     pos, nrm, t, stats = marcher.march(0, st, c.inv_pv, c.eye, c.width, c.height)
     G = O.Grid(sc.xyz, sc.h)
     opos, onrm, ot, ocnt = O.march(G, O.make_settings(**st), c.inv_pv, c.eye, c.width, c.height)
@@ -96,24 +96,31 @@ def test_march_lattice_cube(marcher, seed, cam, variant):
     assert m["hits_b"] > 50000
     # same canonical order and no contraction on either side: identical images
     assert np.array_equal(pos, opos) and np.array_equal(nrm, onrm) and np.array_equal(t, ot)
-    for k in ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples"):
+    for k in ("rays", "rays_in_aabb", "hits", "samples", "march_cand", "march_samples"):
         assert stats[k] == ocnt[k], (k, stats[k], ocnt[k])
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
-def test_march_variants_identical(marcher, variant):
-    """Every kernel schedule produces the same image as the default one, with
-    and without the bound skip and with plain (uncompensated) sums."""
+@pytest.mark.parametrize("kw", [dict(), dict(skip_mode=4), dict(compensated=0), dict(skip_mode=1),
+                                dict(normal_mode=1), dict(final_fp64=0), dict(bisect_iters=0)])
+def test_march_schedules_identical(marcher, kw):
+    """The persistent march (compacted walks, lane groups, per-wave finishing)
+    and the one-ray-per-lane schedule (a direct walk for every sample) give the
+    same images and the same sample counts, with and without the skips, with
+    plain (uncompensated) sums, gradW normals and without the FP64 step or any
+    bisection; an instrumented launch gives the same images too."""
     sc = scenes.dam_break(1, nx=40, ny=80, nz=40, width=320, height=240)
     c = sc.cameras[0]
-    print(f"variant {variant}", flush=True)
+    print(f"schedules {kw}", flush=True)
     marcher.set_frame(4, sc.xyz, sc.h)
-    for kw in (dict(), dict(skip_mode=4), dict(compensated=0)):
-        ref = marcher.march(4, sc.settings(**kw), c.inv_pv, c.eye, c.width, c.height)
-        got = marcher.march(4, sc.settings(variant=variant, **kw), c.inv_pv, c.eye, c.width, c.height)
-        for a, b in zip(ref[:3], got[:3]):
-            assert np.array_equal(a, b), kw
-        assert ref[3]["samples"] == got[3]["samples"] and ref[3]["cand"] == got[3]["cand"], kw
+    ref = marcher.march(4, sc.settings(schedule=1, **kw), c.inv_pv, c.eye, c.width, c.height)
+    got = marcher.march(4, sc.settings(**kw), c.inv_pv, c.eye, c.width, c.height)
+    ins = marcher.march(4, sc.settings(instrument=1, **kw), c.inv_pv, c.eye, c.width, c.height)
+    print(ref[3], got[3], sep="\n", flush=True)
+    assert ref[3]["hits"] > 2000
+    for a, b, i in zip(ref[:3], got[:3], ins[:3]):
+        assert np.array_equal(a, b) and np.array_equal(a, i), kw
+    for k in ("rays", "hits", "samples", "march_samples", "march_cand"):
+        assert ref[3][k] == got[3][k] == ins[3][k], (k, kw)
 
 
 def test_dense_support_overflow(marcher):
@@ -131,12 +138,11 @@ def test_dense_support_overflow(marcher):
     n, _ = marcher.neighbor_counts(5, np.array([[0.5, 0.5 + 0.072, 0.5]], np.float32))
     print("particles within h at the surface", n[0], flush=True)
     assert n[0] > 64
-    ref = marcher.march(5, dict(st, variant=1), c.inv_pv, c.eye, c.width, c.height)
+    ref = marcher.march(5, dict(st, schedule=1), c.inv_pv, c.eye, c.width, c.height)
     assert ref[3]["hits"] > 1000
-    for v in (0, 3, 6):
-        got = marcher.march(5, dict(st, variant=v), c.inv_pv, c.eye, c.width, c.height)
-        for a, b in zip(ref[:3], got[:3]):
-            assert np.array_equal(a, b), v
+    got = marcher.march(5, st, c.inv_pv, c.eye, c.width, c.height)
+    for a, b in zip(ref[:3], got[:3]):
+        assert np.array_equal(a, b)
     G = O.Grid(sc.xyz, sc.h)
     op, on, ot, _ = O.march(G, O.make_settings(**st), c.inv_pv, c.eye, c.width, c.height)
     assert np.array_equal(ref[0], op) and np.array_equal(ref[1], on) and np.array_equal(ref[2], ot)
@@ -153,3 +159,26 @@ def test_march_vs_fp64(marcher):
     m = compare(pos, nrm, t, p64, n64, t64)
     print("vs FP64:", m, flush=True)
     assert_within(m, extent(sc.xyz))
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(skip_mode=4), dict(final_fp64=0, compensated=0)])
+def test_instrumented_counts_match_oracle(marcher, kw):
+    """An instrumented launch counts, for every density evaluation however the
+    kernels took it (march walk, finish list, FP64 step, the six central
+    differences), the candidates of its own fine stencil and of its 27 search
+    cells: both sums equal the oracle's, sample by sample restated."""
+    sc = scenes.lattice_cube(1)
+    c = sc.cameras[0]
+    print(f"instrumented counts lattice_cube {kw}", flush=True)
+    marcher.set_frame(0, sc.xyz, sc.h)
+    st = sc.settings(instrument=1, **kw)
+    pos, nrm, t, stats = marcher.march(0, st, c.inv_pv, c.eye, c.width, c.height)
+    G = O.Grid(sc.xyz, sc.h)
+    osettings = O.make_settings(**{k: v for k, v in st.items() if k in dict(O.Settings._fields_)})
+    opos, onrm, ot, ocnt = O.march(G, osettings, c.inv_pv, c.eye, c.width, c.height)
+    assert np.array_equal(pos, opos) and np.array_equal(nrm, onrm)
+    print(stats, ocnt, sep="\n", flush=True)
+    for k in ("samples", "march_samples", "march_cand", "cand_stencil", "cand27"):
+        assert stats[k] == ocnt[k], (k, stats[k], ocnt[k])
+    # a sample's 27 search cells hold about 2.5x its trimmed fine stencil's candidates
+    assert 1.5 * stats["cand_stencil"] < stats["cand27"]

@@ -2,44 +2,15 @@
 AdvancedRenderer drives the reference (Prepare, Start, poll IsDone;
 AdvancedRenderer.cpp:257-298): its images equal the C ABI's reference march and
 the CPU oracle's."""
-import pathlib
-import struct
-import subprocess
-
 import numpy as np
 import pytest
 
 import oracle as O
+from facade_io import DEMO, run_demo as _run_demo
 from parity import depth_in_front
 from pmarch import scenes
 
 pytestmark = pytest.mark.gpu
-ROOT = pathlib.Path(__file__).resolve().parent.parent
-DEMO = ROOT / "bachelor-thesis_amd" / "facade_demo"
-
-
-def _run_demo(tmp_path, sc, c, depth, max_steps, step, iso, aniso):
-    W, H = c.width, c.height
-    inp = tmp_path / "in.bin"
-    with open(inp, "wb") as f:
-        f.write(b"PMFD")
-        f.write(struct.pack("<3I", W, H, sc.n))
-        f.write(np.float32(sc.h).tobytes())
-        f.write(np.ascontiguousarray(c.inv_pv, np.float32).tobytes())
-        f.write(np.ascontiguousarray(c.eye, np.float32).tobytes())
-        f.write(struct.pack("<i", max_steps))
-        f.write(np.float32(step).tobytes())
-        f.write(np.float32(iso).tobytes())
-        # EnableAnisotropy, k_n, k_r, k_s, N_eps (AdvancedRenderer.cpp:18-28), ParticleRadiusExt
-        f.write(struct.pack("<i3fi", int(aniso), 0.5, 2.0, 2000.0, 1))
-        f.write((np.float32(2.0) * np.float32(sc.h)).tobytes())
-        f.write(np.ascontiguousarray(depth, np.float32).tobytes())
-        f.write(np.ascontiguousarray(sc.xyz, np.float32).tobytes())
-    out = tmp_path / "out.bin"
-    print(f"facade_demo {W}x{H} anisotropic={aniso}", flush=True)
-    subprocess.run([str(DEMO), str(inp), str(out)], check=True, timeout=120)
-    raw = np.fromfile(out, np.float32)
-    return raw[: 4 * W * H].reshape(H, W, 4), raw[4 * W * H: 8 * W * H].reshape(H, W, 4)
 
 
 @pytest.mark.parametrize("aniso", [0, 1])

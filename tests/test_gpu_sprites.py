@@ -104,3 +104,38 @@ def test_sprites_to_march_on_device(marcher):
     finally:
         for b in bufs:
             hip.hipFree(b)
+
+
+def test_sprites_device_completion_and_rebuild(marcher):
+    """pm_depth_sprites_device is complete once pm_wait returns: a null-stream
+    copy afterwards reads the whole image (equal to the host pass).  A frame
+    rebuilt on the build stream (pm_set_frame_stream) right after a sprite pass
+    of that frame starts after the pass: the image is that of the old
+    particles, and the new frame then draws its own."""
+    from test_gpu_device_out import _alloc, _to_host
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    sc = scenes.dam_break(0, nx=40, ny=80, nz=40, width=480, height=270)
+    moved = (sc.xyz + np.float32(0.05)).astype(np.float32)
+    cam = sc.cameras[0]
+    W, H = cam.width, cam.height
+    print("sprites on device: pm_wait, then a rebuild of the same frame 480x270", flush=True)
+    marcher.set_frame(0, sc.xyz, sc.h)
+    want_old = marcher.depth_sprites(0, cam, sc.h)
+    d = _alloc(hip, 4 * W * H)
+    try:
+        marcher.depth_sprites_device(0, cam, sc.h, d)
+        marcher.wait()
+        assert np.array_equal(_to_host(hip, d, (H, W)), want_old)
+        marcher.depth_sprites_device(0, cam, sc.h, d)
+        marcher.set_frame_stream(0, moved, sc.h)
+        marcher.wait()
+        assert np.array_equal(_to_host(hip, d, (H, W)), want_old)
+        want_new = marcher.depth_sprites(0, cam, sc.h)
+        assert not np.array_equal(want_new, want_old)
+        assert np.array_equal(want_new, O.depth_sprites(moved, cam, sc.h))
+    finally:
+        hip.hipFree(d)

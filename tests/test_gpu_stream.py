@@ -85,7 +85,7 @@ def test_autoplay_double_buffer(marcher):
             marcher.set_frame_stream(10 + ((i + 1) % 2), frames[i + 1], base.h)
         marcher.wait()
         imgs.append((pos, nrm, t, marcher.stats()))
-    keys = ("rays", "rays_in_aabb", "hits", "samples", "cand", "march_samples")
+    keys = ("rays", "rays_in_aabb", "hits", "samples", "march_cand", "march_samples")
     for i in range(4):
         marcher.set_frame(12, frames[i], base.h)
         want = marcher.march(12, st, cam.inv_pv, cam.eye, cam.width, cam.height)

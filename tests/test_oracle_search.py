@@ -149,3 +149,29 @@ def test_fine_density_matches_fp64_brute_force():
     w = np.where(d < 0.5, 6.0 * (d ** 3 - d ** 2) + 1.0, 2.0 * np.clip(1.0 - d, 0.0, None) ** 3) * sig
     ref = w.sum(1)
     assert np.max(np.abs(rho - ref) / np.maximum(ref, sig)) < 2e-6
+
+
+def test_per_sample_candidate_counts():
+    """or_stencil_cand is the trimmed fine stencil's candidate count (the walk of
+    or_neighbors_fine), and or_cand27 the number of particles whose search-cell
+    key is within one of the point's on every axis (brute force), 0 where the
+    3x3x3 block leaves the grid."""
+    rng = np.random.default_rng(4)
+    xyz = rng.uniform(-0.3, 0.5, (6000, 3)).astype(np.float32)
+    h = np.float32(0.05)
+    G = O.Grid(xyz, h)
+    q = np.concatenate([rng.uniform(-0.35, 0.55, (300, 3)), xyz[:50] + 0.001]).astype(np.float32)
+    st = G.stencil_cand(q)
+    c27 = G.cand27(q)
+    inv = np.float32(1.0) / h
+    keys = np.where(xyz >= 0, (inv * xyz).astype(np.int32), (inv * xyz).astype(np.int32) - 1)
+    for i, p in enumerate(q):
+        assert st[i] == G.neighbors_fine(p)[1]
+        kp = np.where(p >= 0, (inv * p).astype(np.int32), (inv * p).astype(np.int32) - 1)
+        inside = np.all(np.abs(keys - kp) <= 1, axis=1).sum()
+        c = kp - np.array(G.g.origin)
+        dims = np.array(G.g.dims)
+        if np.all(c >= 1) and np.all(c <= dims - 2):
+            assert c27[i] == inside, i
+        else:
+            assert c27[i] == 0 and inside == 0, i
