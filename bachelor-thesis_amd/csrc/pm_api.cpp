@@ -78,6 +78,9 @@ struct pm_ctx {
     hipEvent_t ev_sprites = nullptr;
     bool sprites_pending = false;
     int32_t sprites_frame = 0;
+    // the static scratch array of pm_march_device_noalloc on this device
+    void *scratch = nullptr;
+    hipError_t scratch_rc = hipSuccess;
     std::map<int32_t, Frame *> frames;
     std::string err;
     // outputs of the last march
@@ -198,6 +201,9 @@ int pm_create(int device, pm_ctx **out)
         pm_destroy(c);
         return PM_ERR_HIP;
     }
+    // set-up of pm_march_device_noalloc's scratch (once per device); a failure
+    // is reported by that entry point
+    c->scratch_rc = noalloc_scratch_init(device, &c->scratch);
     *out = c;
     return PM_OK;
 }
@@ -594,9 +600,12 @@ static bool device_range(int device, const void *p, size_t bytes)
 // caller's device memory dev[0..2] (t may be null) and depth_in is device
 // memory; otherwise into the context's buffers, copied to the host pointers
 // (any may be null) at completion.
+// na != nullptr (pm_march_device_noalloc): every scratch buffer comes from the
+// static array and nothing is allocated (the bound bits must exist, fills are
+// kernels).
 static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const float inv_pv[16], const float cam[3],
                         uint32_t W, uint32_t H, const float *depth_in, float *positions_out, float *normals_out,
-                        float *t_out, void *const *dev)
+                        float *t_out, void *const *dev, const NoAllocScratch *na = nullptr)
 {
     if (!c) return PM_ERR_ARG;
     if (c->pending) return fail(c, PM_ERR_STATE, "pm_march_async: previous march still in flight");
@@ -613,8 +622,14 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
         // the anisotropic density is exactly 0 where no particle lies within h:
         // the bound bits for PM_ZERO_ISO mark those sub-boxes
         const float biso = s->anisotropic ? PM_ZERO_ISO : s->iso;
-        rc = ensure_bound(c, f, biso, c->stream);
-        if (rc) return rc;
+        if (na) {
+            if (!(f->bound_ok && f->bound_iso == biso))
+                return fail(c, PM_ERR_STATE, "pm_march_device_noalloc: frame %d has no bound bits for iso %g "
+                            "(build them first: pm_get_bound_grid or one pm_march_async_device)", frame, (double)biso);
+        } else {
+            rc = ensure_bound(c, f, biso, c->stream);
+            if (rc) return rc;
+        }
         c->bound_iso_last = biso;
         c->bound_used = true;
     }
@@ -651,7 +666,11 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
     L.t = dev ? (float *)dev[2] : (float *)c->d_t.p;
     L.counters = (unsigned long long *)c->d_counters.p;
     L.band_ctr = (uint32_t *)c->d_band.p;
-    if (s->mode == PM_MODE_CONFIG) {   // hit records; the anisotropic march's hit list
+    if (na) {
+        L.counters = na->counters;
+        L.band_ctr = na->band;
+        L.hit_rec = na->hit_rec;
+    } else if (s->mode == PM_MODE_CONFIG) {   // hit records; the anisotropic march's hit list
         HIPCHK(c, c->d_hits.ensure(sizeof(uint4) * npx));
         L.hit_rec = (uint4 *)c->d_hits.p;
     }
@@ -660,8 +679,12 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
     const bool persistent = !s->anisotropic && (s->learned || s->schedule == 0);
     if (s->mode == PM_MODE_CONFIG && persistent) {
         uint64_t tiles = (uint64_t)((W + 7) / 8) * ((H + 7) / 8);
-        HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (3 * (tiles + 1) + scan_tmp_size((uint32_t)(tiles + 1)))));
-        L.tile_buf = (uint32_t *)c->d_tiles.p;
+        if (na) {
+            L.tile_buf = na->tiles;
+        } else {
+            HIPCHK(c, c->d_tiles.ensure(sizeof(uint32_t) * (3 * (tiles + 1) + scan_tmp_size((uint32_t)(tiles + 1)))));
+            L.tile_buf = (uint32_t *)c->d_tiles.p;
+        }
     }
     c->prepass = L.tile_buf != nullptr;
     L.cg = f->cg;
@@ -677,7 +700,8 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
             L.depth = (const float *)c->d_depth.p;
         }
     }
-    HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, sizeof(unsigned long long) * kCtrWords, c->stream));
+    if (na) HIPCHK(c, launch_zero_u64(L.counters, kCtrWords, c->stream));
+    else HIPCHK(c, hipMemsetAsync(c->d_counters.p, 0, sizeof(unsigned long long) * kCtrWords, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
     HIPCHK(c, launch_march(L, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
@@ -725,6 +749,29 @@ int pm_march_async_device(pm_ctx *c, int32_t frame, const pm_settings *s, const 
     void *dev[3] = { d_positions, d_normals, d_t };
     return march_launch(c, frame, s, inv_pv, cam, W, H, d_depth, nullptr, nullptr, nullptr, dev);
 }
+
+int pm_march_device_noalloc(pm_ctx *c, int32_t frame, const pm_settings *s, const float inv_pv[16],
+                            const float cam[3], uint32_t W, uint32_t H, void *d_positions, void *d_normals, void *d_t)
+{
+    if (!c) return PM_ERR_ARG;
+    if (!d_positions || !d_normals) return fail(c, PM_ERR_ARG, "pm_march_device_noalloc: need device positions and normals");
+    if (!s) return fail(c, PM_ERR_ARG, "settings is NULL");
+    if (s->mode != PM_MODE_CONFIG || s->anisotropic || s->schedule != 0)
+        return fail(c, PM_ERR_ARG, "pm_march_device_noalloc: the configured isotropic march, schedule 0");
+    if (W == 0 || H == 0 || W > 16384 || H > 16384 || (uint64_t)W * H > PM_NOALLOC_MAX_PIXELS)
+        return fail(c, PM_ERR_SIZE, "pm_march_device_noalloc: %ux%u is beyond the scratch limits (W, H <= 16384, "
+                    "W*H <= %u)", W, H, PM_NOALLOC_MAX_PIXELS);
+    if (c->scratch_rc != hipSuccess || !c->scratch)
+        return fail(c, PM_ERR_HIP, "pm_march_device_noalloc: no scratch on device %d: %s", c->device,
+                    hipGetErrorString(c->scratch_rc));
+    NoAllocScratch na{};
+    if (!noalloc_layout(c->scratch, W, H, na))
+        return fail(c, PM_ERR_SIZE, "pm_march_device_noalloc: %ux%u does not fit the scratch", W, H);
+    void *dev[3] = { d_positions, d_normals, d_t };
+    return march_launch(c, frame, s, inv_pv, cam, W, H, nullptr, nullptr, nullptr, nullptr, dev, &na);
+}
+
+uint64_t pm_noalloc_scratch_bytes(void) { return (uint64_t)noalloc_scratch_bytes(); }
 
 static int complete(pm_ctx *c)
 {

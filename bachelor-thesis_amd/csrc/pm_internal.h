@@ -198,6 +198,19 @@ hipError_t launch_mlp_adam(const AdamArgs &a, hipStream_t st);
 hipError_t launch_mlp_sync(const float *p, uint16_t *W1, uint16_t *W2, uint16_t *W2T, hipStream_t st);
 hipError_t launch_mlp_forward(const MlpDev &net, const float *feat, uint32_t n, float *y, hipStream_t st);
 
+// Scratch of pm_march_device_noalloc (pm_noalloc.hip): the static device
+// array of the device (looked up once per device), and its layout for a W x H
+// march (false beyond the limits).
+struct NoAllocScratch {
+    uint4 *hit_rec;
+    uint32_t *tiles;
+    unsigned long long *counters;
+    uint32_t *band;
+};
+hipError_t noalloc_scratch_init(int device, void **out);
+bool noalloc_layout(void *base, uint32_t W, uint32_t H, NoAllocScratch &s);
+size_t noalloc_scratch_bytes();
+
 // Host restatement of the CompactNSearch key, for the grid extent.
 inline int32_t host_key(float x, float inv_h)
 {

@@ -125,6 +125,9 @@ def load():
     L.pm_batch_set.argtypes = [vp, C.POINTER(PmSettings), vp, u32, C.c_int]
     L.pm_batch_run.argtypes = [vp]
     L.pm_batch_download.argtypes = [vp, u32, vp, vp, vp]
+    L.pm_march_device_noalloc.argtypes = [vp, i32, C.POINTER(PmSettings), vp, vp, u32, u32, vp, vp, vp]
+    L.pm_noalloc_scratch_bytes.argtypes = []
+    L.pm_noalloc_scratch_bytes.restype = u64
     _lib = L
     return L
 
@@ -285,6 +288,19 @@ class Marcher:
         self._keep = dict(s=s, inv_pv=inv_pv, cam=cam)
         self._check(self.L.pm_march_async_device(self.ctx, frame, C.byref(s), _ptr(inv_pv), _ptr(cam), W, H,
                                                  d_depth or None, d_pos, d_nrm, d_t or None), "pm_march_async_device")
+        self.wait()
+        return self.stats()
+
+    def march_device_noalloc(self, frame: int, settings: dict, inv_pv, cam, W: int, H: int, d_pos: int,
+                             d_nrm: int, d_t: int = 0):
+        """pm_march_device_noalloc + wait: the march into the caller's device
+        memory with the library's static scratch (no allocation)."""
+        s = settings_from(settings)
+        inv_pv = np.ascontiguousarray(inv_pv, np.float32).reshape(16)
+        cam = np.ascontiguousarray(cam, np.float32).reshape(3)
+        self._keep = dict(s=s, inv_pv=inv_pv, cam=cam)
+        self._check(self.L.pm_march_device_noalloc(self.ctx, frame, C.byref(s), _ptr(inv_pv), _ptr(cam), W, H,
+                                                   d_pos, d_nrm, d_t or None), "pm_march_device_noalloc")
         self.wait()
         return self.stats()
 

@@ -344,6 +344,26 @@ int pm_dataset_get(pm_ctx *ctx, uint64_t offset, uint64_t n, float *feat, float 
 int pm_mlp_train(pm_ctx *ctx, int epochs, uint32_t batch, float lr, float w_over, uint64_t seed,
                  float *loss_per_epoch);
 
+/* ---- a march that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc) ----
+ * pm_march_async_device's configured isotropic march (plain or learned), with
+ * every buffer it needs beyond its inputs and outputs -- hit records, the tile
+ * pre-pass's lists, counters -- in one static device array per GPU
+ * (pm_noalloc_scratch_bytes(), about 34 MB, held until the process ends).
+ * Limits: W * H <= PM_NOALLOC_MAX_PIXELS and W, H <= 16384 (PM_ERR_SIZE); the
+ * frame must be set and, with PM_SKIP_BOUND, its bound bits for s->iso built
+ * (one pm_march_async_device or pm_get_bound_grid does it; PM_ERR_STATE
+ * otherwise); the images are device memory of the context's GPU.  Nothing is
+ * queued when a check fails.  Completes like pm_march_async_device (pm_query,
+ * pm_wait, pm_get_stats).  Every call on one GPU shares that GPU's array:
+ * marches that overlap on one GPU (two contexts, two threads) race. */
+#define PM_NOALLOC_MAX_PIXELS (1u << 21)
+int pm_march_device_noalloc(pm_ctx *ctx, int32_t frame, const pm_settings *s, const float inv_pv[16],
+                            const float cam[3], uint32_t W, uint32_t H, void *d_positions, void *d_normals,
+                            void *d_t);
+uint64_t pm_noalloc_scratch_bytes(void);
+
 /* Measurement utility, no counterpart in the reference: times `iters`
  * launches of a streaming copy of `bytes` bytes (read + write) on the context
  * stream and returns the achieved bandwidth, counting 2*bytes per launch. */
