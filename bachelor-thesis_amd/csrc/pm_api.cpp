@@ -47,6 +47,7 @@ struct Frame {
     DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound;
     DevBuf fsorted, fcell_start;    // march grid (cells of h/2)
     bool bound_ok = false;      // bound holds the sub-box bits for bound_iso
+    hipEvent_t bound_ev[2] = { nullptr, nullptr };   // around the last bound build
     bool dg_ok = false;         // the DensityGrid flags are built
     float bound_iso = 0.0f;
     int32_t dg_dims[3] = { 0, 0, 0 };
@@ -58,6 +59,8 @@ struct Frame {
         scan_tmp.release(); xyz.release(); bounds_partial.release(); bounds.release(); scattered.release();
         empty27.release(); dg_box.release(); dg_cons.release(); dg_count.release(); bound.release();
         fsorted.release(); fcell_start.release();
+        for (hipEvent_t &e : bound_ev)
+            if (e) { hipEventDestroy(e); e = nullptr; }
     }
 };
 
@@ -462,6 +465,12 @@ int pm_frame_info_get(pm_ctx *c, int32_t frame, pm_frame_info *out)
     Frame *f = get_frame(c, frame);
     if (!f) return fail(c, PM_ERR_FRAME, "frame %d not set", frame);
     *out = f->info;
+    out->bound_ms = -1.0f;
+    out->bound_iso = f->bound_ok ? f->bound_iso : 0.0f;
+    if (f->bound_ok && f->bound_ev[1]) {
+        HIPCHK(c, hipEventSynchronize(f->bound_ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&out->bound_ms, f->bound_ev[0], f->bound_ev[1]));
+    }
     return PM_OK;
 }
 
@@ -511,7 +520,11 @@ static int ensure_bound(pm_ctx *c, Frame *f, float iso, hipStream_t st)
     if (f->bound_ok && f->bound_iso == iso) return PM_OK;
     HIPCHK(c, f->bound.ensure(sizeof(uint64_t) * f->info.ncells));
     f->g.bound = (const uint64_t *)f->bound.p;
+    for (hipEvent_t &e : f->bound_ev)
+        if (!e) HIPCHK(c, hipEventCreate(&e));
+    HIPCHK(c, hipEventRecord(f->bound_ev[0], st));
     HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, st));
+    HIPCHK(c, hipEventRecord(f->bound_ev[1], st));
     f->bound_ok = true;
     f->bound_iso = iso;
     c->frame_epoch++;
@@ -915,6 +928,8 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
     if (!s) return fail(c, PM_ERR_ARG, "settings is NULL");
     if (s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "pm_batch_set: the batch runs the configured march");
     if (s->anisotropic) return fail(c, PM_ERR_ARG, "pm_batch_set: the batch runs the isotropic march");
+    if (s->instrument || s->schedule != 0)
+        return fail(c, PM_ERR_ARG, "pm_batch_set: the batch runs schedule 0, not instrumented");
     HIPCHK(c, hipSetDevice(c->device));
     c->b_ready = false;
     if (c->b_exec) { hipGraphExecDestroy(c->b_exec); c->b_exec = nullptr; }

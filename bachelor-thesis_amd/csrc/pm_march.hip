@@ -527,37 +527,43 @@ __device__ __forceinline__ float pm_segment_bound(const GridDev &g, const float 
     return U;
 }
 
-// A step proposal for every lane in PH_PROPOSE, all at once: features at the
-// lane's current sample k (its density is known; one fine-stencil pass for
-// grad rho), one MLP evaluation for the whole wave (BF16 MFMA), a jump of n
-// samples, and the certificate of the samples k+1 .. k+n-1 it passes over: the
-// segment bound U < iso (1 - 2^-9), which covers the rounding of the bound
+// A step proposal for every lane in PH_PROPOSE (want), all at once: features at
+// the lane's current sample k (its density rho is known; one fine-stencil pass
+// for grad rho), one MLP evaluation for the whole wave (BF16 MFMA), a jump of
+// n samples, and the certificate of the samples k+1 .. k+n-1 it passes over:
+// the segment bound U < iso (1 - 2^-9), which covers the rounding of the bound
 // and of the samples' own FP32 densities.  A certified lane next evaluates
 // sample k+n exactly; the others step to k+1 and ask for no proposal until the
 // exact skip next moves them.  Every sample the learned march does not
 // evaluate lies below iso, so its first crossing -- and its image -- are the
-// plain march's.  Called by all 64 lanes together; kept out of line so the
-// feature pass, the network and the certificate do not add to the registers
-// of the density loop.
-__device__ __noinline__ void wave_propose(const MarchLaunch &L, RayState &r, Counts &cn, uint16_t *X, uint16_t *H,
-                                          float *Y)
+// plain march's.  Called by all 64 lanes together; kept out of line (with the
+// ray's state passed in and out by value, so it stays in registers) so the
+// feature pass, the network and the certificate do not add to the registers of
+// the density loop.
+struct Proposal {
+    int k, lrn;
+    uint32_t mlp, cert, cskip, cand;
+};
+
+__device__ __noinline__ Proposal wave_propose(const MarchLaunch &L, bool want, int k, float tn, float tf, float d0,
+                                              float d1, float d2, float rho, uint16_t *X, uint16_t *H, float *Y)
 {
     const int l = threadIdx.x & 63;
     const pm_settings &s = L.s;
     const GridDev &g = L.g;
-    const bool want = r.phase == PH_PROPOSE;
+    Proposal o{ k, 1, 0u, 0u, 0u, 0u };
     float f[PM_MLP_IN];
 #pragma unroll
     for (int j = 0; j < PM_MLP_IN; j++) f[j] = 0.0f;
-    const float t = r.tn + (float)r.k * s.step;
+    const float t = tn + (float)k * s.step;
     if (want) {
-        const float q0 = L.cam[0] + t * r.d0, q1 = L.cam[1] + t * r.d1, q2 = L.cam[2] + t * r.d2;
+        const float q0 = L.cam[0] + t * d0, q1 = L.cam[1] + t * d1, q2 = L.cam[2] + t * d2;
         float gx, gy, gz;
         uint32_t cand = 0;
         grad_feat(g, q0, q1, q2, gx, gy, gz, cand);
-        cn.cand += cand;
-        const float d[3] = { r.d0, r.d1, r.d2 };
-        mlp_features(g, L.cg, L.cam, d, t, r.tn, r.tf, r.rho, sqrtf((gx * gx + gy * gy) + gz * gz),
+        o.cand += cand;
+        const float d[3] = { d0, d1, d2 };
+        mlp_features(g, L.cg, L.cam, d, t, tn, tf, rho, sqrtf((gx * gx + gy * gy) + gz * gz),
                      (float)pm_cand27(g, q0, q1, q2), s.iso, f);
     }
     wave_lds_sync();
@@ -568,43 +574,58 @@ __device__ __noinline__ void wave_propose(const MarchLaunch &L, RayState &r, Cou
     wave_lds_sync();
     int n = 0;
     if (want) {
-        cn.mlp++;
+        o.mlp = 1;
         const float dist = g.h * (exp2f(fminf(y, 8.0f)) - 1.0f);
         const float sk = fminf(floorf(L.mlp_alpha * dist * L.inv_step), (float)L.mlp_max_skip);
-        int kmax = (int)floorf((r.tf - r.tn) * L.inv_step);           // last sample inside the AABB
-        while (kmax > 0 && r.tn + (float)kmax * s.step > r.tf) kmax--;
-        if (sk >= (float)L.mlp_min_skip && r.k + (int)sk <= kmax) n = (int)sk;
+        int kmax = (int)floorf((tf - tn) * L.inv_step);           // last sample inside the AABB
+        while (kmax > 0 && tn + (float)kmax * s.step > tf) kmax--;
+        if (sk >= (float)L.mlp_min_skip && k + (int)sk <= kmax) n = (int)sk;
         if (n < 2) {
-            r.lrn = 0;
-            r.k++;
-            r.phase = PH_MARCH;
+            o.lrn = 0;
+            o.k = k + 1;
         }
     }
     const bool cert = want && n >= 2;
-    if (__ballot(cert) == 0) return;
-    float U = INFINITY;
+    if (__ballot(cert) == 0) return o;
     const float lim = s.iso * (1.0f - 0x1p-9f);
     if (cert) {
-        const float ta = r.tn + (float)(r.k + 1) * s.step, tb = r.tn + (float)(r.k + n - 1) * s.step;
-        const float a[3] = { L.cam[0] + ta * r.d0, L.cam[1] + ta * r.d1, L.cam[2] + ta * r.d2 };
-        const float b[3] = { L.cam[0] + tb * r.d0, L.cam[1] + tb * r.d1, L.cam[2] + tb * r.d2 };
+        const float ta = tn + (float)(k + 1) * s.step, tb = tn + (float)(k + n - 1) * s.step;
+        const float a[3] = { L.cam[0] + ta * d0, L.cam[1] + ta * d1, L.cam[2] + ta * d2 };
+        const float b[3] = { L.cam[0] + tb * d0, L.cam[1] + tb * d1, L.cam[2] + tb * d2 };
         // the lattice samples between a and b lie within a few ulps of the
         // segment; the margin covers them and the rounding of the distances
         float m = g.h;
 #pragma unroll
-        for (int k = 0; k < 3; k++) m = fmaxf(m, fmaxf(fabsf(a[k]), fabsf(b[k])));
+        for (int c = 0; c < 3; c++) m = fmaxf(m, fmaxf(fabsf(a[c]), fabsf(b[c])));
         uint32_t cand = 0;
-        U = pm_segment_bound(g, a, b, m * 0x1p-18f, lim, cand);
-        cn.cand += cand;
-        cn.cert++;
+        const float U = pm_segment_bound(g, a, b, m * 0x1p-18f, lim, cand);
+        o.cand += cand;
+        o.cert = 1;
         if (U < lim) {
-            r.k += n;
-            cn.cskip += (uint32_t)(n - 1);
+            o.k = k + n;
+            o.cskip = (uint32_t)(n - 1);
         } else {
-            r.k++;
-            r.lrn = 0;
+            o.k = k + 1;
+            o.lrn = 0;
         }
+    }
+    return o;
+}
+
+// The proposals of a wave and their transitions (all 64 lanes).
+__device__ __forceinline__ void wave_propose_step(const MarchLaunch &L, RayState &r, Counts &cn, float *region)
+{
+    const bool want = r.phase == PH_PROPOSE;
+    const Proposal o = wave_propose(L, want, r.k, r.tn, r.tf, r.d0, r.d1, r.d2, r.rho, (uint16_t *)region,
+                                    (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+    if (want) {
+        r.k = o.k;
+        r.lrn = o.lrn;
         r.phase = PH_MARCH;
+        cn.mlp += o.mlp;
+        cn.cert += o.cert;
+        cn.cskip += o.cskip;
+        cn.cand += o.cand;
     }
 }
 
@@ -800,7 +821,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
             const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
             if (nwant > 0 && (2 * nwant >= nbusy || exhausted))
-                wave_propose(L, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+                wave_propose_step(L, r, cn, region);
         }
         if (exhausted && group_ok) regroup(r, G, region);
         if (G == 1) {
@@ -990,7 +1011,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             const int nbusy = __popcll(__ballot(r.phase != PH_DONE));
             const int nwant = __popcll(__ballot(r.phase == PH_PROPOSE));
             if (nwant > 0 && (2 * nwant >= nbusy || draining))
-                wave_propose(*Lp, r, cn, (uint16_t *)region, (uint16_t *)region + 64 * PM_MLP_K1, region + 2048);
+                wave_propose_step(*Lp, r, cn, region);
         }
         if (group_ok && draining) regroup(r, G, region);
         if (G == 1) {

@@ -54,7 +54,7 @@ class PmFrameInfo(C.Structure):
         ("n", C.c_uint64), ("ncells", C.c_uint64), ("origin", C.c_int32 * 3), ("dims", C.c_int32 * 3),
         ("h", C.c_float), ("pmin", C.c_float * 3), ("pmax", C.c_float * 3), ("bmin", C.c_float * 3),
         ("bmax", C.c_float * 3), ("build_ms", C.c_float), ("forigin", C.c_int32 * 3), ("fdims", C.c_int32 * 3),
-        ("fncells", C.c_uint64),
+        ("fncells", C.c_uint64), ("bound_ms", C.c_float), ("bound_iso", C.c_float),
     ]
 
 

@@ -138,6 +138,9 @@ typedef struct pm_frame_info {
     int32_t  forigin[3];
     int32_t  fdims[3];
     uint64_t fncells;
+    float    bound_ms;      /* GPU time of the last build of the sub-box bound bits
+                               (PM_SKIP_BOUND), -1 when none was built            */
+    float    bound_iso;     /* the iso those bits are for                         */
 } pm_frame_info;
 
 typedef struct pm_ctx pm_ctx;

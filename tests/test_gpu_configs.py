@@ -99,10 +99,10 @@ def test_error_vs_fp64_oracle(marcher, name):
 
 def test_camera_inside_aabb_many_tiles(marcher):
     """A 5120x2880 view from inside the particles' AABB: every one of the
-    230,400 tiles is active, most rays cross empty space quickly and each wave
-    of the persistent march takes many tiles; the finish kernel packs the hits
-    of runs of tiles.  The image equals the one-ray-per-lane schedule and the
-    oracle on a row sample."""
+    230,400 tiles is active and most rays cross empty space quickly, so waves of
+    the persistent march take far more than the 32 tiles they finish
+    themselves; the rest go to the finish kernel.  The image equals the
+    one-ray-per-lane schedule and the oracle on a row sample."""
     rng = np.random.default_rng(21)
     blobs = [rng.normal(cen, 0.03, (5000, 3)) for cen in ((0.3, 0.5, 0.7), (0.6, 0.4, 0.8), (0.5, 0.6, 0.5))]
     xyz = np.concatenate(blobs + [rng.uniform(0.0, 1.0, (5000, 3))]).astype(np.float32)
