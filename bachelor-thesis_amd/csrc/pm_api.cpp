@@ -134,13 +134,14 @@ static MlpDev mlp_dev(pm_ctx *c)
 }
 
 // The learned step's network and knobs (defaults: 70 % of the predicted
-// distance, jumps of 3 to 8 samples).
+// distance, jumps of 3 to 8 samples, proposals below a quarter of iso).
 static void set_learned(pm_ctx *c, const pm_settings *s, MarchLaunch &L)
 {
     L.net = mlp_dev(c);
     L.mlp_alpha = s->learned_alpha > 0.0f ? s->learned_alpha : 0.7f;
     L.mlp_min_skip = s->learned_min_skip > 0 ? s->learned_min_skip : 3;
     L.mlp_max_skip = s->learned_max_skip > 0 ? s->learned_max_skip : 8;
+    L.mlp_gate_rho = (s->learned_gate > 0.0f ? s->learned_gate : 0.25f) * s->iso;
 }
 
 static int fail(pm_ctx *c, int code, const char *fmt, ...)
@@ -548,6 +549,7 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
     if (s->learned && !c->mlp_ready) return fail(c, PM_ERR_STATE, "learned step: no network loaded (pm_mlp_set)");
     if (s->learned && s->mode != PM_MODE_CONFIG) return fail(c, PM_ERR_ARG, "the learned step belongs to the configured march");
     if (s->learned && !(s->learned_alpha >= 0.0f && s->learned_alpha <= 4.0f)) return fail(c, PM_ERR_ARG, "learned_alpha out of [0,4]");
+    if (s->learned && !(s->learned_gate >= 0.0f && s->learned_gate <= 1.0f)) return fail(c, PM_ERR_ARG, "learned_gate out of [0,1]");
     if (s->learned && (s->learned_min_skip < 0 || s->learned_min_skip == 1 || s->learned_max_skip < 0 ||
                        s->learned_max_skip > 64))
         return fail(c, PM_ERR_ARG, "learned: min_skip must be 0 or >= 2, max_skip in [0, 64]");

@@ -64,6 +64,7 @@ struct MarchLaunch {
     float mlp_alpha;           // fraction of the predicted distance that is skipped
     int32_t mlp_min_skip;      // below this many steps the ray continues step by step
     int32_t mlp_max_skip;      // longest jump, in steps
+    float mlp_gate_rho;        // a sample asks for a proposal only below this density
     // tiles with a ray entering the AABB, in row order: scratch of
     // 3 (T + 1) + scan_tmp_size(T + 1) u32; null = every tile
     uint32_t *tile_buf;

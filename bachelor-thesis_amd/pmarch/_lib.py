@@ -32,7 +32,8 @@ class PmSettings(C.Structure):
         ("learned", C.c_int32), ("schedule", C.c_int32), ("learned_alpha", C.c_float),
         ("learned_min_skip", C.c_int32), ("learned_max_skip", C.c_int32),
         ("anisotropic", C.c_int32), ("k_n", C.c_float), ("k_r", C.c_float), ("k_s", C.c_float),
-        ("n_eps", C.c_int32), ("ext_mult", C.c_float), ("instrument", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("n_eps", C.c_int32), ("ext_mult", C.c_float), ("instrument", C.c_int32), ("learned_gate", C.c_float),
+        ("reserved", C.c_int32 * 1),
     ]
 
 

@@ -98,7 +98,9 @@ typedef struct pm_settings {
                               per density evaluation, the candidates of its own fine
                               stencil and of its 27 search cells (pm_stats.cand_stencil,
                               cand27; same images, slower)                          */
-    int32_t reserved[2];
+    float   learned_gate;  /* learned: a sample asks for a proposal only when its
+                              density is below learned_gate x iso (0 = 0.25)       */
+    int32_t reserved[1];
 } pm_settings;
 
 /* Per-launch counters, written by the march kernels themselves. */
