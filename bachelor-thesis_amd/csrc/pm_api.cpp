@@ -1050,6 +1050,7 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
         J.mlp_alpha = B.base.mlp_alpha;
         J.mlp_min_skip = B.base.mlp_min_skip;
         J.mlp_max_skip = B.base.mlp_max_skip;
+        J.mlp_gate_rho = B.base.mlp_gate_rho;
     }
     HIPCHK(c, c->b_jobs.ensure(sizeof(MarchLaunch) * jobs.size()));
     HIPCHK(c, c->b_chunks.ensure(sizeof(BatchChunk) * chunks.size()));

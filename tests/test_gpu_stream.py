@@ -3,12 +3,13 @@
 the same positions set directly; a frame built on the second stream while a
 march of another frame is in flight (the reference's autoplay,
 AdvancedRenderer.cpp:283-290) leaves that march's images unchanged and then
-marches exactly like a frame set synchronously."""
+marches exactly like a frame set synchronously, and like the oracle."""
 import time
 
 import numpy as np
 import pytest
 
+import oracle as O
 from pmarch import PmError, scenes
 from test_bgeo import write_bgeo
 
@@ -63,6 +64,13 @@ def test_stream_frame_beside_march(marcher):
     want = marcher.march(2, st, cam.inv_pv, cam.eye, cam.width, cam.height)
     for x, y in zip(got[:3], want[:3]):
         assert np.array_equal(x, y)
+    # and equals the oracle's march of frame 1 (every 27th row)
+    step = 27
+    keys = dict(O.Settings._fields_)
+    op, on, ot, _ = O.march(O.Grid(frames[1], base.h), O.make_settings(**{k: v for k, v in st.items() if k in keys}),
+                            cam.inv_pv, cam.eye, cam.width, cam.height, row_step=step)
+    for x, y in zip(got[:3], (op, on, ot)):
+        assert np.array_equal(x[::step], y[::step])
     print(f"frame build beside the march: {t_build * 1e3:.2f} ms host time", flush=True)
 
 
