@@ -7,8 +7,10 @@ import pathlib
 import numpy as np
 import pytest
 
+import features_np
 import mlp_oracle as M
 import oracle as O
+from features_np import point
 from pmarch import scenes
 
 pytestmark = pytest.mark.gpu
@@ -77,8 +79,11 @@ def test_trace_rows_match_oracle(marcher):
     assert n > 2000 and np.all(np.isfinite(feat)) and np.all(label >= 0)
     G = O.Grid(sc.xyz, sc.h)
     _, _, t, _ = O.march(G, O.make_settings(**sc.settings()), c.inv_pv, c.eye, c.width, c.height)
+    F = features_np.Features(sc.xyz, sc.h, sc.iso, G.origin, G.dims)
     rng = np.random.default_rng(0)
-    for i in rng.choice(n, 300, replace=False):
+    rows = rng.choice(n, 300, replace=False)
+    varied = np.zeros(4, np.int64)
+    for i in rows:
         pix, k = int(meta[i, 0]), int(meta[i, 1])
         x, y = pix % c.width, pix // c.width
         o, d = O.ray(c.inv_pv, c.eye, c.width, c.height, x, y)
@@ -90,8 +95,18 @@ def test_trace_rows_match_oracle(marcher):
         tend = th if th >= 0 else tf
         expect = math.log2(1.0 + min((tend - float(tk)) / sc.h, 255.0))
         assert abs(label[i] - expect) < 1e-4, (i, label[i], expect)
-        # the features' direction and density inputs
-        assert np.allclose(feat[i, 16:19], d, atol=1e-6)
-        q = (o.astype(np.float64) + float(tk) * d.astype(np.float64))
-        rho = G.density64(q[None])[0]
-        assert feat[i, 20] == pytest.approx(min(rho / sc.iso, 4.0), rel=1e-3, abs=1e-5)
+        # all 24 features against the numpy restatement: counts exactly, the
+        # density to FP32 rounding, |grad rho| (plain FP32 sums) within a bound
+        # from the sum of the terms' magnitudes
+        f = F.row(o, d, tk, tn, tf)
+        counts = np.r_[0:16, 23]
+        assert np.array_equal(feat[i, counts], f[counts].astype(np.float32)), (i, feat[i, counts], f[counts])
+        assert np.allclose(feat[i, 16:19], f[16:19], rtol=0, atol=1e-6)
+        assert feat[i, 19] == pytest.approx(f[19], rel=1e-5, abs=1e-6)
+        assert feat[i, 22] == pytest.approx(f[22], rel=1e-5, abs=1e-6)
+        assert feat[i, 20] == pytest.approx(f[20], rel=5e-6, abs=2e-7)
+        _, g, scale = F.rho_grad(point(o, tk, d))
+        tol = 1e-5 * np.linalg.norm(scale) * sc.h / sc.iso / 4.0
+        assert abs(feat[i, 21] - f[21]) <= tol + 1e-6, (i, feat[i, 21], f[21], tol)
+        varied += feat[i, [0, 8, 21, 23]] != feat[rows[0], [0, 8, 21, 23]]
+    assert np.all(varied > 0)

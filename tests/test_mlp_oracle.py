@@ -6,7 +6,10 @@ import pathlib
 import numpy as np
 import pytest
 
+import features_np
 import mlp_oracle as M
+import oracle as O
+from pmarch import scenes
 
 GOLD = pathlib.Path(__file__).resolve().parent / "golden" / "mlp_fp64.npz"
 
@@ -69,3 +72,18 @@ def test_init_layout():
     assert np.all(W1[:, M.NIN:] == 0) and np.any(W1[:, :M.NIN] != 0)
     # trainable parameters of the 24 -> 64 -> 64 -> 1 network
     assert M.NIN * M.H + M.H + M.H * M.H + M.H + M.H + 1 == 5825
+
+
+def test_feature_restatement_gradient_matches_differences():
+    """The numpy gradient of the features is the derivative of the numpy
+    density: central differences in FP64 at random points of a lattice."""
+    sc = scenes.lattice_cube(2, n_side=10)
+    G = O.Grid(sc.xyz, sc.h)
+    F = features_np.Features(sc.xyz, sc.h, sc.iso, G.origin, G.dims)
+    rng = np.random.default_rng(4)
+    for q in rng.uniform(0.35, 0.65, (50, 3)):
+        rho, g, _ = F.rho_grad(q)
+        assert rho == pytest.approx(G.density64(q[None])[0], rel=1e-12)
+        e = 1e-6
+        fd = [(F.rho_grad(q + e * u)[0] - F.rho_grad(q - e * u)[0]) / (2 * e) for u in np.eye(3)]
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-6 * sc.iso / sc.h)

@@ -175,3 +175,86 @@ def test_per_sample_candidate_counts():
             assert c27[i] == inside, i
         else:
             assert c27[i] == 0 and inside == 0, i
+
+
+def _densitygrid_brute(xyz, h, bmin, dims, W0):
+    """Per node of the density grid: the particles strictly inside the L-infinity
+    box of half-width h around the node centre (all n particles tested), and the
+    flag of Dataset.cpp:136-163 with its 27-node weighted sum written out --
+    weight exp(-1000 r) in FP32, r the node offset's L1 length."""
+    cw = np.float32(h)
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    idx = np.stack([x.ravel(), y.ravel(), z.ravel()], -1).astype(np.float32)
+    ctr = (bmin.astype(np.float32) + (idx + np.float32(0.5)) * cw).astype(np.float32)
+    cnt = np.zeros(len(ctr), np.int64)
+    cons = np.zeros(len(ctr), np.int64)
+    p = xyz.astype(np.float32)
+    R2 = np.float32(np.float32(1.5) * np.float32(h)) * np.float32(1.001)
+    for s in range(0, len(ctr), 512):
+        d = np.abs(p[None, :, :] - ctr[s:s + 512, None, :])
+        cnt[s:s + 512] = np.all(d < cw, axis=2).sum(1)
+        cons[s:s + 512] = np.all(d < R2, axis=2).sum(1)
+    grid = cnt.reshape(nz, ny, nx).astype(np.float32)
+    Nc = np.zeros_like(grid)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                w = np.exp(np.float32(-1000.0) * np.float32(abs(dx) + abs(dy) + abs(dz)), dtype=np.float32)
+                sh = np.zeros_like(grid)
+                zs, ys, xs = (slice(max(0, -dz), nz - max(0, dz)), slice(max(0, -dy), ny - max(0, dy)),
+                              slice(max(0, -dx), nx - max(0, dx)))
+                zd, yd, xd = (slice(max(0, dz), nz - max(0, -dz)), slice(max(0, dy), ny - max(0, -dy)),
+                              slice(max(0, dx), nx - max(0, -dx)))
+                sh[zs, ys, xs] = grid[zd, yd, xd]
+                Nc = (Nc + w * sh).astype(np.float32)
+    flag = (Nc.ravel() * np.float32(W0)) > np.float32(1.0)
+    return ctr, cnt, flag, cons > 0
+
+
+@pytest.mark.parametrize("seed,h", [(0, 0.04), (1, 2.0)])
+def test_densitygrid_counts_equal_brute_force(seed, h):
+    """Dataset.cpp:119-130 (per-node particle count from the box query around
+    the node centre) and 143-163 (flag = approximate density > 1): the oracle's
+    count, flag and conservative flag equal a brute force over every particle
+    for every node, including particles placed exactly on box faces (excluded:
+    the box test is strict) and nodes on the grid border.  With h = 0.04,
+    W(0) = 8 / (pi h^3) > 1 and every occupied node is flagged; with h = 2,
+    W(0) = 0.32 and the flag needs four particles, so the threshold matters."""
+    rng = np.random.default_rng(seed)
+    L = 15.0 * h
+    xyz = np.concatenate([rng.uniform(0.0, L, (1500, 3)),
+                          rng.normal(0.5 * L, 0.25 * h, (600, 3)),
+                          rng.uniform(0.0, L, (4, 3)) * [1, 0, 0]]).astype(np.float32)
+    xyz[-4:, 1:] = 0.0                                              # on the lower faces of the bounds
+    G0 = O.Grid(xyz, h)
+    dims = tuple(G0.densitygrid()[0])
+    # particles exactly on faces of node boxes: centre +- h on one axis, kept
+    # where the FP32 distance to the centre is exactly h
+    m = 400
+    node = rng.integers(2, np.array(dims) - 2, (m, 3)).astype(np.float32)
+    ctr = (G0.bmin.astype(np.float32) + (node + np.float32(0.5)) * np.float32(h)).astype(np.float32)
+    axis = rng.integers(0, 3, m)
+    sign = rng.choice([-1.0, 1.0], m).astype(np.float32)
+    face = ctr.copy()
+    face[np.arange(m), axis] = (ctr[np.arange(m), axis] + sign * np.float32(h)).astype(np.float32)
+    exact = np.abs(face - ctr).max(1) == np.float32(h)
+    assert exact.sum() >= 40
+    face, node, ctr = face[exact][:40], node[exact][:40], ctr[exact][:40]
+    xyz = np.concatenate([xyz, face]).astype(np.float32)
+    G = O.Grid(xyz, h)
+    assert np.array_equal(G.bmin, G0.bmin)
+    dims, count_box, flag_box, flag_cons = G.densitygrid()
+    W0 = O.W(0.0, h)
+    bctr, bcnt, bflag, bcons = _densitygrid_brute(xyz, h, G.bmin, dims, W0)
+    lin = (node[:, 0] + dims[0] * (node[:, 1] + dims[1] * node[:, 2])).astype(np.int64)
+    assert np.array_equal(bctr[lin], ctr)                           # the nodes the face particles touch
+    assert np.array_equal(count_box.astype(np.int64), bcnt)
+    assert np.array_equal(flag_box.astype(bool), bflag)
+    assert np.array_equal(flag_cons.astype(bool), bcons)
+    assert 0 < bflag.sum() < len(bflag) and np.any(bcnt == 0)
+    if W0 < 1.0:
+        assert np.any((bcnt > 0) & ~bflag)                          # occupied but below the threshold
+    # the 26 neighbour weights vanish in FP32, so the flag is the node's own count
+    assert np.exp(np.float32(-1000.0), dtype=np.float32) == 0.0
+    assert np.array_equal(bflag, bcnt.astype(np.float32) * np.float32(W0) > np.float32(1.0))
