@@ -7,13 +7,14 @@
 //      381), walked on the search grid (cells of h) over the cells within two
 //      cells of q: rows dz outer, dy, x, by input index inside a cell;
 //   2. WPCA (RayMarcher.cpp:114-254): weights 1 - (|r|/hx)^3 (CubicKernel,
-//      Kernel.cpp:110-120), weighted mean and covariance C in two passes over
-//      that walk, a cyclic Jacobi eigen-decomposition of C, the clamp of [YT13]
-//      eq. 15 (:232-242) and G = h^-1 R diag(1/Sigma) R^T (:246-247);
+//      Kernel.cpp:110-120), weighted mean and covariance C about it in one
+//      pass over that walk (an_cov), a cyclic Jacobi eigen-decomposition of C,
+//      the clamp of [YT13] eq. 15 (:232-242) and G = h^-1 R diag(1/Sigma) R^T
+//      (:246-247);
 //   3. rho = sum over the extended neighbours within h of
 //      AnisotropicKernel::W(G, det G, r) (Kernel.cpp:63-83), walked over the
-//      cells within one cell of q (the same particles in the same order; the
-//      extended walk when the list was capped).
+//      trimmed 5x5x5 march-grid stencil around q in the canonical order (the
+//      extended walk up to the cap when the list was capped).
 // One ray per lane, one 8x8 tile per wave, each lane to the end of its ray
 // (march, bisection, FP64 step, normal, outputs); schedule 1 of the configured
 // march runs persistent waves with ray refill and a finish pass over the hits
@@ -26,7 +27,6 @@ namespace pm {
 
 static constexpr uint32_t kMaxNeighbors = 8192;       // RayMarcher.cpp:14
 static constexpr float kLimExt = 4.00390625f;         // 2 cells: 4 (1 + 2^-10)
-static constexpr float kLimNear = 1.0009765625f;      // 1 cell: 1 (1 + 2^-10)
 static constexpr int kJacobiSweeps = 5;
 
 // ---- search-grid walk ----
@@ -117,6 +117,33 @@ __device__ __forceinline__ uint32_t an_walk(const GridDev &g, const SWalk &w, fl
     return cnd;
 }
 
+// Every candidate of the trimmed 5x5x5 march-grid stencil around q (the
+// isotropic density's walk: the particles within h in the canonical order),
+// loaded as an_walk loads its rows.  Returns the number of candidates; none
+// when the stencil leaves the march grid (no particle lies within h of q).
+template <class F>
+__device__ __forceinline__ uint32_t an_fine_walk(const GridDev &g, const float qf[3], F &&f)
+{
+    FineWalk fw;
+    if (!pm_fine_begin(g, qf[0], qf[1], qf[2], fw)) return 0;
+    uint32_t cnd = 0, nb, ne;
+    pm_fine_row_at(g, fw, 0, nb, ne);
+    _Pragma("unroll 1") for (int r = 0; r < 25; r++) {
+        const uint32_t b = nb, e = ne;
+        if (r + 1 < 25) pm_fine_row_at(g, fw, r + 1, nb, ne);
+        cnd += e - b;
+        for (uint32_t i = b; i < e; i += 4) {
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = g.fsorted[i + u < e ? i + u : i];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (i + u < e) f(v[u]);
+        }
+    }
+    return cnd;
+}
+
 // ---- WPCA ----
 template <class P>
 __device__ __forceinline__ typename P::T an_wcubic(typename P::T r2, const P &a)
@@ -199,8 +226,16 @@ __device__ __forceinline__ void an_gmat(const typename P::T c[6], uint32_t N, co
     detG = (A0 - A1) + A2;
 }
 
-// The two WPCA passes: C in c; returns the extended-neighbour count (before
-// the cap) and adds both walks' candidates to cand.
+// WPCA's weighted mean and covariance (RayMarcher.cpp:138-169) in one walk.
+// The weights come from r_j = x_j - q; the moments are taken about the first
+// neighbour found, s: with u_j = x_j - s, W = sum w and mu = sum w u / W,
+//     C = sum w u u^T / W - mu mu^T,
+// which is the reference's second pass sum w (x - m)(x - m)^T / W about the
+// weighted mean m = s + mu in exact arithmetic.  s lies among the neighbours,
+// so u is of the size of their spread and the subtraction cancels little (a
+// single neighbour gives C = 0 exactly, as the two-pass form does).  C (lower
+// triangle) in c; returns the extended-neighbour count (before the cap) and
+// adds the walk's candidates to cand.
 template <class P>
 __device__ __forceinline__ uint32_t an_cov(const GridDev &g, const P &a, const float qf[3], const typename P::T q[3],
                                            typename P::T c[6], uint32_t &cand)
@@ -209,46 +244,35 @@ __device__ __forceinline__ uint32_t an_cov(const GridDev &g, const P &a, const f
     SWalk w;
     if (!an_centre(g, qf, 2, w)) return 0;
     uint32_t nx = 0;
-    T ws = (T)0, m0 = (T)0, m1 = (T)0, m2 = (T)0;
-    uint32_t cnd = an_walk<2>(g, w, kLimExt, [&](const float4 p) {
+    T sx = (T)0, sy = (T)0, sz = (T)0;
+    T ws = (T)0, s0 = (T)0, s1 = (T)0, s2 = (T)0;
+    T c0 = (T)0, c1 = (T)0, c2 = (T)0, c3 = (T)0, c4 = (T)0, c5 = (T)0;
+    cand += an_walk<2>(g, w, kLimExt, [&](const float4 p) {
         const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
         const T r2 = (rx * rx + ry * ry) + rz * rz;
         if (r2 < a.R2x) {
+            if (nx == 0) { sx = (T)p.x; sy = (T)p.y; sz = (T)p.z; }
             if (nx < kMaxNeighbors) {
                 const T wt = an_wcubic(r2, a);
+                const T ux = (T)p.x - sx, uy = (T)p.y - sy, uz = (T)p.z - sz;
+                const T w0 = wt * ux, w1 = wt * uy, w2 = wt * uz;
                 ws += wt;
-                m0 += wt * (T)p.x;
-                m1 += wt * (T)p.y;
-                m2 += wt * (T)p.z;
+                s0 += w0; s1 += w1; s2 += w2;
+                c0 += w0 * ux;
+                c1 += w1 * ux;
+                c2 += w1 * uy;
+                c3 += w2 * ux;
+                c4 += w2 * uy;
+                c5 += w2 * uz;
             }
             nx++;
         }
     });
-    if (nx == 0) { cand += cnd; return 0; }
-    const uint32_t N = nx < kMaxNeighbors ? nx : kMaxNeighbors;
+    if (nx == 0) return 0;
     const T inv = (T)1 / ws;
-    m0 = m0 * inv; m1 = m1 * inv; m2 = m2 * inv;
-    T c0 = (T)0, c1 = (T)0, c2 = (T)0, c3 = (T)0, c4 = (T)0, c5 = (T)0;
-    uint32_t n2 = 0;
-    cnd += an_walk<2>(g, w, kLimExt, [&](const float4 p) {
-        const T rx = (T)p.x - q[0], ry = (T)p.y - q[1], rz = (T)p.z - q[2];
-        const T r2 = (rx * rx + ry * ry) + rz * rz;
-        if (r2 < a.R2x && n2 < N) {
-            const T wt = an_wcubic(r2, a);
-            const T x0 = (T)p.x - m0, x1 = (T)p.y - m1, x2 = (T)p.z - m2;
-            const T w0 = wt * x0, w1 = wt * x1, w2 = wt * x2;
-            c0 += w0 * x0;
-            c1 += w1 * x0;
-            c2 += w1 * x1;
-            c3 += w2 * x0;
-            c4 += w2 * x1;
-            c5 += w2 * x2;
-            n2++;
-        }
-    });
-    cand += cnd;
-    c[0] = c0 * inv; c[1] = c1 * inv; c[2] = c2 * inv;
-    c[3] = c3 * inv; c[4] = c4 * inv; c[5] = c5 * inv;
+    const T m0 = s0 * inv, m1 = s1 * inv, m2 = s2 * inv;
+    c[0] = c0 * inv - m0 * m0; c[1] = c1 * inv - m1 * m0; c[2] = c2 * inv - m1 * m1;
+    c[3] = c3 * inv - m2 * m0; c[4] = c4 * inv - m2 * m1; c[5] = c5 * inv - m2 * m2;
     return nx;
 }
 
@@ -310,7 +334,10 @@ __device__ __forceinline__ void an_neu_add(T &s, T &c, T w)
 
 // Density at q (compensated: Neumaier sum), or with grad != NULL the sum of
 // gradW over the same neighbours (FP32).  qf is the FP32 point that picks the
-// cells.
+// cells.  The extended neighbours within h are all the particles within h
+// unless the list was capped: then they are taken from the extended walk up to
+// the cap, else from the trimmed march-grid stencil in the canonical order
+// (about 40 candidates per sample instead of the 27 search cells' 200).
 template <class P>
 __device__ __forceinline__ typename P::T an_sum(const GridDev &g, const P &a, const float qf[3],
                                                 const typename P::T q[3], bool comp, uint32_t &cand,
@@ -350,8 +377,7 @@ __device__ __forceinline__ typename P::T an_sum(const GridDev &g, const P &a, co
         an_centre(g, qf, 2, w);
         cnd = an_walk<2>(g, w, kLimExt, term);
     } else {
-        an_centre(g, qf, 1, w);
-        cnd = an_walk<1>(g, w, kLimNear, term);
+        cnd = an_fine_walk(g, qf, term);
     }
     cand += cnd;
     return comp ? s + cc : s;
@@ -558,6 +584,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         if (L.s.mode == PM_MODE_REFERENCE) an_pixel_reference(L, px, py, cn);
         else an_pixel_config(L, px, py, cn);
     }
+    flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
+}
+
+// The configured march over the active tiles only (the tile pre-pass has
+// written every pixel of the other tiles as a miss), one tile per wave: every
+// wave that starts has rays to march, and the heavy tiles are not dispatched
+// between thousands of empty ones.  Workgroups go to the 8 XCDs round-robin
+// (workgroup b to XCD b mod 8), so XCD x takes the x-th eighth of the active
+// list: all XCDs share the work and neighbouring tiles share an L2.  The grid
+// has a multiple of 8 workgroups and at least T / 32 per XCD.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_march_aniso_active(MarchLaunch L)
+{
+    const uint32_t T = ((L.W + 7) / 8) * ((L.H + 7) / 8);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t *order = L.tile_buf + 2 * (T + 1);
+    const uint32_t count = order[T], chunk = (count + 7) / 8;
+    const uint32_t local = (blockIdx.x >> 3) * 4 + wave, i = (blockIdx.x & 7u) * chunk + local;
+    if (local >= chunk || i >= count) return;   // whole wave
+    const uint32_t tile = order[i], tiles_x = (L.W + 7) / 8;
+    const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
+    Counts cn;
+    if (px < L.W && py < L.H) an_pixel_config(L, px, py, cn);
     flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
 }
 
@@ -801,9 +849,19 @@ __global__ void k_zero_band(uint32_t *p)
 hipError_t launch_march_aniso(const MarchLaunch &L, hipStream_t st)
 {
     const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
-    // one ray per lane to its end (the default: neighbouring rays of a tile
-    // sample neighbouring points, so the wave's gathers share cache lines), or
-    // (schedule 1) persistent waves with ray refill and the hit-list finish
+    // one ray per lane to its end (neighbouring rays of a tile sample
+    // neighbouring points, so the wave's gathers share cache lines): over the
+    // active tiles after the tile pre-pass (configured march, schedule 0), or
+    // over all tiles (reference mode; or no tile buffer); or (schedule 1)
+    // persistent waves with ray refill and the hit-list finish
+    if (L.s.mode == PM_MODE_CONFIG && L.s.schedule == 0 && L.tile_buf) {
+        hipError_t e = launch_tile_prepass(L, st);
+        if (e != hipSuccess) return e;
+        // 8 ceil(ceil(T / 8) / 4) workgroups: enough waves for every XCD's share
+        const uint32_t per_xcd = ((tiles + 7) / 8 + 3) / 4;
+        hipLaunchKernelGGL(k_march_aniso_active, dim3(8 * per_xcd), dim3(256), 0, st, L);
+        return hipGetLastError();
+    }
     if (L.s.mode == PM_MODE_REFERENCE || L.s.schedule != 1 || !L.hit_rec) {
         hipLaunchKernelGGL(k_march_aniso, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
         return hipGetLastError();

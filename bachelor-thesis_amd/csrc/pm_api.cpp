@@ -690,8 +690,9 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
         L.hit_rec = (uint4 *)c->d_hits.p;
     }
     if (s->anisotropic) L.an = aniso_params(f, s);
-    // tile order and miss pre-pass for the persistent schedule
-    const bool persistent = !s->anisotropic && (s->learned || s->schedule == 0);
+    // tile pre-pass (active-tile list, bulk miss writes) for the persistent
+    // schedule and for the anisotropic configured march
+    const bool persistent = s->schedule == 0 || (!s->anisotropic && s->learned);
     if (s->mode == PM_MODE_CONFIG && persistent) {
         uint64_t tiles = (uint64_t)((W + 7) / 8) * ((H + 7) / 8);
         if (na) {

@@ -126,6 +126,10 @@ size_t scan_tmp_size(uint32_t len);
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint32_t len, uint32_t *tmp, hipStream_t st);
 hipError_t launch_grid_build(const GridBuildArgs &a, hipStream_t st);
 
+// The tile pre-pass of the configured march (L.tile_buf set): flags of the 8x8
+// tiles one of whose rays enters the padded AABB, their scan, the list of
+// active tiles (order[T] = their count) and the misses of the other tiles.
+hipError_t launch_tile_prepass(const MarchLaunch &L, hipStream_t st);
 hipError_t launch_march(const MarchLaunch &L, hipStream_t st);
 // The anisotropic march (configured or reference mode), one kernel.
 hipError_t launch_march_aniso(const MarchLaunch &L, hipStream_t st);

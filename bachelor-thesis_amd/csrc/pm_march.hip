@@ -1127,6 +1127,18 @@ hipError_t launch_miss_tiles(const MarchLaunch &L, uint32_t T, hipStream_t st)
 // The configured isotropic march of one view: schedule 1, or the tile
 // pre-pass, the persistent march and the finish of the tiles its waves left
 // (L.tile_buf and L.hit_rec set).
+hipError_t launch_tile_prepass(const MarchLaunch &L, hipStream_t st)
+{
+    const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
+    uint32_t *flag = L.tile_buf, *pos = L.tile_buf + (tiles + 1);
+    hipLaunchKernelGGL(k_tile_flags, dim3((tiles + 1 + 3) / 4), dim3(256), 0, st, L, tiles);
+    hipError_t e = launch_scan_u32(flag, pos, tiles + 1, L.tile_buf + 3 * (tiles + 1), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tile_compact, dim3((tiles + 1 + 255) / 256), dim3(256), 0, st, L.tile_buf, tiles);
+    hipLaunchKernelGGL(k_miss_tiles, dim3((tiles + 3) / 4), dim3(256), 0, st, L, tiles);
+    return hipGetLastError();
+}
+
 static hipError_t launch_march_config(const MarchLaunch &L, hipStream_t st)
 {
     const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
@@ -1135,12 +1147,8 @@ static hipError_t launch_march_config(const MarchLaunch &L, hipStream_t st)
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, L.band_ctr, 8);
-    uint32_t *flag = L.tile_buf, *pos = L.tile_buf + (tiles + 1);
-    hipLaunchKernelGGL(k_tile_flags, dim3((tiles + 1 + 3) / 4), dim3(256), 0, st, L, tiles);
-    hipError_t e = launch_scan_u32(flag, pos, tiles + 1, L.tile_buf + 3 * (tiles + 1), st);
+    hipError_t e = launch_tile_prepass(L, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tile_compact, dim3((tiles + 1 + 255) / 256), dim3(256), 0, st, L.tile_buf, tiles);
-    hipLaunchKernelGGL(k_miss_tiles, dim3((tiles + 3) / 4), dim3(256), 0, st, L, tiles);
     // enough resident waves to fill the chip; extra waves find no tile and exit
     uint32_t blocks = L.persist_blocks ? L.persist_blocks : 1024;
     const uint32_t need = (tiles + 3) / 4;

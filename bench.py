@@ -193,11 +193,11 @@ def traffic_for(path: str, workload: str):
 
 
 def alg_bytes_aniso(st: dict) -> float:
-    """Anisotropic march: 12 B per candidate of its walks (two passes over the
-    2h stencil for WPCA, one over the h stencil for the density: the candidate
-    counter counts all three), 8 B of row bounds per row (25 + 25 + 9 rows per
-    sample) and 36 B of output per ray."""
-    return 12.0 * st["cand"] + 8.0 * 59 * st["samples"] + 36.0 * st["rays"]
+    """Anisotropic march: 12 B per candidate of its two walks (the 2h
+    search-grid stencil for WPCA, the trimmed h march-grid stencil for the
+    density: the candidate counter counts both), 8 B of row bounds per row
+    (25 + 25 rows per sample) and 36 B of output per ray."""
+    return 12.0 * st["cand"] + 8.0 * 50 * st["samples"] + 36.0 * st["rays"]
 
 
 def cpu_baseline_aniso(scene, cam, settings, budget_s: float):

@@ -1034,6 +1034,15 @@ uint32_t or_aniso_C_f(const or_grid *g, const or_settings *s, const float q[3], 
     return an_cov_f(g, &a, q, q, c, &cand);
 }
 
+uint32_t or_aniso_C_d(const or_grid *g, const or_settings *s, const double q[3], double c[6])
+{
+    an_par_d a;
+    an_par_make_d(g, s, &a);
+    uint32_t cand = 0;
+    const float qf[3] = { (float)q[0], (float)q[1], (float)q[2] };
+    return an_cov_d(g, &a, qf, q, c, &cand);
+}
+
 void or_aniso_G_from_C_f(const or_grid *g, const or_settings *s, const float c[6], uint32_t n, float G[9],
                          float *detG)
 {

@@ -169,6 +169,7 @@ uint32_t or_aniso_G_d(const or_grid *g, const or_settings *s, const double q[3],
  * c21, c22); returns the number of extended neighbours.  G and det G from C
  * and that number (capped at 8192). */
 uint32_t or_aniso_C_f(const or_grid *g, const or_settings *s, const float q[3], float c[6]);
+uint32_t or_aniso_C_d(const or_grid *g, const or_settings *s, const double q[3], double c[6]);
 void or_aniso_G_from_C_f(const or_grid *g, const or_settings *s, const float c[6], uint32_t n, float G[9],
                          float *detG);
 /* Cyclic Jacobi eigen-decomposition of the symmetric 3x3 matrix given by its

@@ -93,6 +93,64 @@ def _G_from_C(G, s, c, n):
     return g, d.value
 
 
+def _two_pass_cov(nb, q, hx):
+    """RayMarcher.cpp:138-169 in FP64 with numpy: weights 1 - (|x - q| / hx)^3,
+    the weighted mean m, then C = sum w (x - m)(x - m)^T / sum w."""
+    nb = nb.astype(np.float64)
+    d = np.linalg.norm(nb - q.astype(np.float64), axis=1)
+    w = 1.0 - (d / hx) ** 3
+    m = (w[:, None] * nb).sum(0) / w.sum()
+    x = nb - m
+    return (w[:, None, None] * x[:, :, None] * x[:, None, :]).sum(0) / w.sum()
+
+
+@pytest.mark.parametrize("scene", ["lattice", "sheet", "line"])
+def test_one_pass_covariance_equals_two_pass(scene):
+    """WPCA's covariance about the weighted mean, taken in one walk as
+    sum w u u^T / W - mu mu^T with u = x - (first neighbour), against the
+    reference's two-pass form restated in FP64 over the brute-force extended
+    neighbours: the FP64 oracle to 1e-12 of the largest entry, the FP32 one to
+    2e-5 -- at points in the bulk, on the surface and outside, where the mean
+    is far from q and a few neighbours lie close together."""
+    rng = np.random.default_rng(11)
+    if scene == "lattice":
+        xyz = scenes.lattice_cube(3).xyz
+    elif scene == "sheet":
+        i = np.arange(-12, 13) * 0.05
+        xyz = (np.stack(np.meshgrid(i, i, [0.0], indexing="ij"), -1).reshape(-1, 3) + 0.5)
+        xyz = (xyz + rng.normal(0, 0.002, xyz.shape)).astype(np.float32)
+    else:
+        t = np.arange(-40, 41) * 0.01
+        xyz = (0.5 + np.stack([t, 0.3 * t, -0.2 * t], -1) + rng.normal(0, 0.001, (t.size, 3))).astype(np.float32)
+    G = O.Grid(xyz, H)
+    s = _settings()
+    hx = 2.0 * H
+    L = O.lib()
+    fp = np.ctypeslib.ndpointer
+    L.or_aniso_C_d.restype = C.c_uint32
+    L.or_aniso_C_d.argtypes = [C.POINTER(O.GridStruct), C.POINTER(O.Settings), fp(np.float64), fp(np.float64)]
+    lo, hi = xyz.min(0) - 0.15, xyz.max(0) + 0.15
+    qs = np.concatenate([xyz[rng.integers(0, len(xyz), 60)] + rng.normal(0, 0.03, (60, 3)),
+                         rng.uniform(lo, hi, (60, 3))]).astype(np.float32)
+    checked = 0
+    for q in qs:
+        nb = xyz[_brute_ext(xyz, q, np.float32(hx))]
+        c64 = np.zeros(6)
+        n = L.or_aniso_C_d(C.byref(G.g), C.byref(s), q.astype(np.float64), c64)
+        assert n == len(nb)
+        if n < 2:
+            continue
+        ref = _two_pass_cov(nb, q, hx)
+        want = ref[[0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]]
+        scale = np.abs(ref).max()
+        assert np.abs(c64 - want).max() <= 1e-12 * scale, (scene, q)
+        c32, n32 = _cov_samples(G, s, [q])[0]
+        assert n32 == n
+        assert np.abs(c32.astype(np.float64) - want).max() <= 2e-5 * scale, (scene, q)
+        checked += 1
+    assert checked > 40
+
+
 @pytest.mark.skipif(not REF_EIGEN.exists(), reason="oracle/_ref not built (the reference tree is absent)")
 def test_G_matches_reference_eigen_solver():
     """G and det G from the Jacobi step against Eigen::SelfAdjointEigenSolver::
