@@ -360,8 +360,57 @@ def bench_autoplay(args):
     print(json.dumps(out))
 
 
+def sprite_pixel_tests(xyz, cam, R: float) -> int:
+    """Pixel centres k_depth_sprites tests in one pass: per particle the pixels
+    of its projected quad's bounding box, clamped to the image (particles with
+    a corner behind the eye draw nothing) -- the kernel's own loop bounds,
+    evaluated in FP32 in the kernel's operation order."""
+    f = np.float32
+    V = np.asarray(cam.view, f).reshape(16)
+    P = np.asarray(cam.proj, f).reshape(16)
+    W, H = cam.width, cam.height
+    q = np.asarray(xyz, f)
+    a = np.array([V[0], V[4], V[8]], f)
+    b = np.array([V[1], V[5], V[9]], f)
+    lo_x = np.full(len(q), f(3.0e38)); hi_x = np.full(len(q), f(-3.0e38))
+    lo_y = lo_x.copy(); hi_y = hi_x.copy()
+    ok = np.ones(len(q), bool)
+    R = f(R)
+
+    def xform(M, x, y, z):
+        return [(M[r] * x + M[4 + r] * y) + (M[8 + r] * z + M[12 + r] * f(1.0)) for r in range(4)]
+
+    for k in range(4):
+        sx = f(1.0) if k & 1 else f(-1.0)
+        sy = f(1.0) if k & 2 else f(-1.0)
+        c = [q[:, i] + R * (sx * a[i] + sy * b[i]) for i in range(3)]
+        v = xform(V, *c)
+        cl = xform(P, v[0], v[1], v[2])
+        ok &= cl[3] > 0
+        w3 = np.where(cl[3] > 0, cl[3], f(1.0))
+        fx = (cl[0] / w3 + f(1.0)) * f(0.5) * f(W)
+        fy = (cl[1] / w3 + f(1.0)) * f(0.5) * f(H)
+        lo_x = np.minimum(lo_x, fx); hi_x = np.maximum(hi_x, fx)
+        lo_y = np.minimum(lo_y, fy); hi_y = np.maximum(hi_y, fy)
+    x0 = np.maximum(np.ceil(lo_x - f(0.5)), f(0.0)); x1 = np.minimum(np.floor(hi_x - f(0.5)), f(W - 1))
+    y0 = np.maximum(np.ceil(lo_y - f(0.5)), f(0.0)); y1 = np.minimum(np.floor(hi_y - f(0.5)), f(H - 1))
+    n = np.where(ok & (x0 <= x1) & (y0 <= y1), (x1 - x0 + 1).astype(np.int64) * (y1 - y0 + 1).astype(np.int64), 0)
+    return int(n.sum())
+
+
+# FP32 operations per pixel test of k_depth_sprites before its disc test (the
+# pixel's ray, its intersection with the quad's plane and the UV disc test);
+# a drawn fragment adds about 90 more (two more matrix products, the cosine
+# polynomial, the projection) that this count leaves out.
+SPRITE_FLOP_PER_TEST = 58
+FP32_PEAK_TFLOPS = 157.3
+
+
 def bench_sprites(args):
-    """The depth-sprite pass of the 1M dam break at 1080p."""
+    """The depth-sprite pass of the 1M dam break at 1080p into device memory:
+    the steps are enqueued back to back on the context stream and timed from
+    the first launch to the end of the last pass."""
+    import torch
     import pmarch
     from pmarch import scenes
 
@@ -369,30 +418,43 @@ def bench_sprites(args):
     cam = sc.cameras[0]
     m = pmarch.Marcher(0)
     m.set_frame(0, sc.xyz, sc.h)
+    d = torch.empty(cam.height * cam.width, dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
     for _ in range(args.warmup):
-        d = m.depth_sprites(0, cam, sc.h)
+        m.depth_sprites_device(0, cam, sc.h, d.data_ptr())
+        m.wait()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        d = m.depth_sprites(0, cam, sc.h)
+        m.depth_sprites_device(0, cam, sc.h, d.data_ptr())
+    m.wait()
     wall = time.perf_counter() - t0
+    ms = wall / args.steps * 1e3
+    tests = sprite_pixel_tests(sc.xyz, cam, sc.h)
+    tflops = SPRITE_FLOP_PER_TEST * tests / (ms * 1e-3) / 1e12
+    img = d.cpu().numpy()
     out = {
-        "metric": "depth sprites Mparticles/s @1080p/1M particles (DepthRenderPass), host image out",
+        "metric": "depth sprites Mparticles/s @1080p/1M particles (DepthRenderPass), image in device memory",
         "value": round(args.steps * sc.n / wall / 1e6, 3),
         "unit": "Mparticles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "depth sprites (radius h, camera-facing quads, depth test LESS) of the "
-                               "1,024,000-particle dam break, 1920x1080 view, image copied to the host",
+                               "1,024,000-particle dam break, 1920x1080 view, image in device memory",
                    "particles": sc.n, "width": cam.width, "height": cam.height, "path": "sprites"},
-        "roofline": None,
-        "pixels_drawn": int((d < 1.0).sum()),
+        "roofline": {"bound": "valu", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "flops": f"{SPRITE_FLOP_PER_TEST} per pixel test x {tests} tests per pass (drawn fragments' "
+                              "extra work not counted)"},
+        "pixel_tests_per_pass": tests,
+        "pixels_drawn": int((img < 1.0).sum()),
+        "timing": "host clock over back-to-back passes on the context stream (fill + sprites kernels)",
     }
     m.close()
     print(json.dumps(out))
@@ -456,6 +518,7 @@ def bench_smooth(args):
     kms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     B = 8.0 * W * H                    # one read and one write of the image
     achieved = B / (kms * 1e-3) / 1e9
+    tflops = 2.0 * (2 * N + 1) ** 2 * W * H / (kms * 1e-3) / 1e12
     out = {
         "metric": "depth smoothing Mpixels/s @1080p (GaussRenderPass, 17x17 window)",
         "value": round(args.steps * W * H / wall / 1e6, 3),
@@ -472,8 +535,10 @@ def bench_smooth(args):
         "config": {"workload": "screen-space depth smoothing (GaussRenderPass: normalised Gaussian, N = 8, "
                                "clamp-to-edge) of a 1920x1080 depth image in device memory",
                    "width": W, "height": H, "N": N, "path": "smooth"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for("smooth", args.workload)},
+        "roofline": {"bound": "valu", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic_for("smooth", args.workload),
+                     "flops": f"2 x {(2 * N + 1) ** 2} per pixel (one fused multiply-add per window tap)",
+                     "hbm_gbps": round(achieved, 2), "hbm_frac": round(achieved / HBM_PEAK_GBS, 4)},
         "kernel_ms": round(kms, 5),
         "alg_bytes_per_launch": B,
         "copy_gbps": round(m.stream_copy_gbps(1 << 28, 10), 1),
