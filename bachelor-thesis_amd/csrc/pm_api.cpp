@@ -561,8 +561,9 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
         if (!(s->k_n > 0.0f && s->k_r > 0.0f && s->k_s > 0.0f) || !std::isfinite(s->k_n) || !std::isfinite(s->k_r) ||
             !std::isfinite(s->k_s))
             return fail(c, PM_ERR_ARG, "anisotropic: k_n, k_r and k_s must be finite and > 0");
-        if (s->ext_mult != 0.0f && !(s->ext_mult >= 1.0f && s->ext_mult <= 16.0f))
-            return fail(c, PM_ERR_ARG, "anisotropic: ext_mult must be 0 (= 2) or in [1, 16]");
+        // the extended walk covers the cells within 2h of the sample
+        if (s->ext_mult != 0.0f && !(s->ext_mult >= 1.0f && s->ext_mult <= 2.0f))
+            return fail(c, PM_ERR_ARG, "anisotropic: ext_mult must be 0 (= 2) or in [1, 2]");
         if (s->learned) return fail(c, PM_ERR_ARG, "anisotropic: no learned step");
     }
     if (s->mode == PM_MODE_CONFIG) {
@@ -1189,8 +1190,8 @@ static int aniso_query_settings(pm_ctx *c, const pm_settings *s)
     if (!(t.k_n > 0.0f && t.k_r > 0.0f && t.k_s > 0.0f) || !std::isfinite(t.k_n) || !std::isfinite(t.k_r) ||
         !std::isfinite(t.k_s))
         return fail(c, PM_ERR_ARG, "anisotropic: k_n, k_r and k_s must be finite and > 0");
-    if (t.ext_mult != 0.0f && !(t.ext_mult >= 1.0f && t.ext_mult <= 16.0f))
-        return fail(c, PM_ERR_ARG, "anisotropic: ext_mult must be 0 (= 2) or in [1, 16]");
+    if (t.ext_mult != 0.0f && !(t.ext_mult >= 1.0f && t.ext_mult <= 2.0f))
+        return fail(c, PM_ERR_ARG, "anisotropic: ext_mult must be 0 (= 2) or in [1, 2]");
     return PM_OK;
 }
 

@@ -93,7 +93,8 @@ typedef struct pm_settings {
     float   k_s;           /* k_s: scale of the clamped eigenvalues (> 0)             */
     int32_t n_eps;         /* N_eps                                                   */
     float   ext_mult;      /* ParticleRadiusExt / ParticleRadius (Dataset.cpp:169-177),
-                              >= 1; 0 = 2 (config.yml particleRadiusMultiplier)       */
+                              in [1, 2] (the extended walk covers 2h); 0 = 2
+                              (config.yml particleRadiusMultiplier)                   */
     int32_t instrument;    /* configured isotropic march, schedule 0: 1 = also count,
                               per density evaluation, the candidates of its own fine
                               stencil and of its 27 search cells (pm_stats.cand_stencil,

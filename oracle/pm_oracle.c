@@ -1440,7 +1440,7 @@ static int run_march(const or_grid *g, const or_hash *hs, const or_settings *s, 
      * sub-boxes with no particle within h, where that density is exactly 0 */
     const float biso = s->anisotropic ? OR_ZERO_ISO : s->iso;
     if (s->skip_mode == 4 && (!g->bound || g->bound_iso != biso)) return -2;
-    if (s->anisotropic && !(s->ext_mult >= 1.0f)) return -1;
+    if (s->anisotropic && !(s->ext_mult >= 1.0f && s->ext_mult <= 2.0f)) return -1;
     if (row_step == 0) row_step = 1;
     march_job j;
     memset(&j, 0, sizeof(j));
@@ -1582,7 +1582,7 @@ int or_march_f64(const or_grid *g, const or_settings *s, const float invpv[16], 
                  double *pos, double *nrm, double *tout)
 {
     if (!g || !s || s->mode != OR_MODE_CONFIG || !(s->step > 0.0f) || W == 0 || H == 0) return -1;
-    if (s->anisotropic && !(s->ext_mult >= 1.0f)) return -1;
+    if (s->anisotropic && !(s->ext_mult >= 1.0f && s->ext_mult <= 2.0f)) return -1;
     if (row_step == 0) row_step = 1;
     march64_job j;
     memset(&j, 0, sizeof(j));

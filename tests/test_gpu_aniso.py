@@ -155,7 +155,7 @@ def test_aniso_settings_rejected(marcher):
     sc = scenes.lattice_cube(0)
     marcher.set_frame(0, sc.xyz, sc.h)
     cam = sc.cameras[0]
-    for bad in (dict(learned=1), dict(k_s=0.0), dict(k_r=-1.0), dict(ext_mult=0.5)):
+    for bad in (dict(learned=1), dict(k_s=0.0), dict(k_r=-1.0), dict(ext_mult=0.5), dict(ext_mult=2.5)):
         st = scenes.aniso_settings(sc, **bad)
         with pytest.raises(Exception):
             marcher.march(0, st, cam.inv_pv, cam.eye, 64, 48)
