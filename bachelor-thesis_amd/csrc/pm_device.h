@@ -570,102 +570,6 @@ __device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy
     return out;
 }
 
-// The six central-difference densities of pm_cdiff6 over a list.
-__device__ __forceinline__ void pm_list_cdiff6(const GridDev &g, const float *buf, uint32_t n, const float x[3],
-                                               float eps, bool comp, float v[6])
-{
-    const float xp[3] = { x[0] + eps, x[1] + eps, x[2] + eps }, xm[3] = { x[0] - eps, x[1] - eps, x[2] - eps };
-    Acc6 a;
-#pragma unroll
-    for (int k = 0; k < 6; k++) { a.s[k] = 0.0f; a.c[k] = 0.0f; }
-    for (uint32_t j = 0; j < n; j++)
-        pm_acc6(g, buf[(3 * j) * 64], buf[(3 * j + 1) * 64], buf[(3 * j + 2) * 64], x, xp, xm, comp, a);
-#pragma unroll
-    for (int k = 0; k < 6; k++) v[k] = comp ? a.s[k] + a.c[k] : a.s[k];
-}
-
-// ---- per-hit candidate lists ----
-// Every particle within h (1 + 2^-10) of the box [lo, hi], in storage order --
-// the canonical summation order -- as (x, y, z) triplets in the lane's slots
-// (entry j at buf[(3 j + c) 64]).  A sample anywhere in the box has every
-// particle within h of it in the list, so a density summed over the list (each
-// entry tested with the sample's own r^2 expression) equals the sample's walk
-// bit for bit.  n: particles found (the list holds the first CAPL); false when
-// the box's cells leave the march grid.
-template <int CAPL>
-__device__ __forceinline__ bool pm_list_build(const GridDev &g, const float lo[3], const float hi[3], float *buf,
-                                              uint32_t &n, uint32_t &cand)
-{
-    BoxWalk w;
-    n = 0;
-    if (!pm_box_begin(g, lo, hi, w)) return false;
-    const float he = g.h * 1.0009765625f, R2 = he * he;
-    uint32_t nb, ne;
-    pm_box_row(g, w, 0, 0, nb, ne);
-    const int nr = w.ny * w.nz;
-    int jy = 0, jz = 0;
-    _Pragma("unroll 1") for (int r = 0; __ballot(r < nr) != 0; r++) {
-        const uint32_t b = nb, e = ne;
-        cand += e - b;
-        if (++jy == w.ny) { jy = 0; jz++; }
-        pm_box_row(g, w, jy, jz, nb, ne);                 // next row in flight
-        const int m = (int)(e - b);
-        const float4 *const p0 = g.fsorted + b;
-        for (int i = 0; i < m; i += 4) {
-            float4 v[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const float dx = fmaxf(fmaxf(lo[0] - v[u].x, v[u].x - hi[0]), 0.0f);
-                const float dy = fmaxf(fmaxf(lo[1] - v[u].y, v[u].y - hi[1]), 0.0f);
-                const float dz = fmaxf(fmaxf(lo[2] - v[u].z, v[u].z - hi[2]), 0.0f);
-                // each candidate goes to the next free slot and stays only if it
-                // passes (n moves past it); a full list takes no more stores
-                // (n then only counts)
-                if (n < (uint32_t)CAPL) {
-                    buf[(3 * n) * 64] = v[u].x;
-                    buf[(3 * n + 1) * 64] = v[u].y;
-                    buf[(3 * n + 2) * 64] = v[u].z;
-                }
-                n += (i + u < m && (dx * dx + dy * dy) + dz * dz < R2) ? 1u : 0u;
-            }
-        }
-    }
-    return true;
-}
-
-// pm_density32 / pm_density32_lds over a list: the same sum, bit for bit.
-__device__ __forceinline__ float pm_list_density32(const GridDev &g, const float *buf, uint32_t n, float qx, float qy,
-                                                   float qz, bool comp)
-{
-    float s = 0.0f, c = 0.0f;
-    for (uint32_t j = 0; j < n; j++) {
-        const float rx = buf[(3 * j) * 64] - qx, ry = buf[(3 * j + 1) * 64] - qy, rz = buf[(3 * j + 2) * 64] - qz;
-        const float r2 = (rx * rx + ry * ry) + rz * rz;
-        if (r2 < g.h2) {
-            const float w = pm_W_in(r2, g.inv_h, g.sig);
-            if (comp) pm_neu_add(s, c, w);
-            else s += w;
-        }
-    }
-    return comp ? s + c : s;
-}
-
-// pm_density64 over a list.
-__device__ __forceinline__ double pm_list_density64(const GridDev &g, const float *buf, uint32_t n, double qx,
-                                                    double qy, double qz)
-{
-    double acc = 0.0;
-    for (uint32_t j = 0; j < n; j++) {
-        const double rx = (double)buf[(3 * j) * 64] - qx, ry = (double)buf[(3 * j + 1) * 64] - qy;
-        const double rz = (double)buf[(3 * j + 2) * 64] - qz;
-        const double r2 = (rx * rx + ry * ry) + rz * rz;
-        if (r2 < g.h2_d) acc += pm_W_in_d(r2, g.hinv_d, g.sig_d);
-    }
-    return acc;
-}
-
 // FP64 density; the stencil is chosen from the FP32-rounded point.
 __device__ __forceinline__ double pm_density64(const GridDev &g, double qx, double qy, double qz)
 {
