@@ -90,6 +90,7 @@ struct pm_ctx {
     DevBuf d_pos, d_nrm, d_t, d_depth, d_counters, d_q, d_out, d_band, d_tiles;
     DevBuf d_hits;                     // per-pixel hit records of one march
     DevBuf d_gauss;                    // the smoothing tables for N = 0..31, written once
+    DevBuf d_sbig;                     // the sprite pass's large-footprint list (count + indices)
     uint64_t npx = 0;
     bool have_t = false;
     // pinned staging for asynchronous hand-off
@@ -220,6 +221,7 @@ void pm_destroy(pm_ctx *c)
     for (auto &kv : c->frames) { kv.second->release(); delete kv.second; }
     c->d_pos.release(); c->d_nrm.release(); c->d_t.release(); c->d_depth.release();
     c->d_counters.release(); c->d_q.release(); c->d_out.release(); c->d_band.release(); c->d_tiles.release();
+    c->d_sbig.release();
     c->d_gauss.release();
     c->d_hits.release();
     c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release(); c->mlp_W1.release(); c->mlp_W2.release();
@@ -1359,12 +1361,18 @@ static int sprites_launch(pm_ctx *c, int32_t frame, const float view[16], const 
     S.H = H;
     HIPCHK(c, hipSetDevice(c->device));
     const uint64_t npx = (uint64_t)W * H;
+    // room to list every particle with a large footprint (a wave then shares its
+    // pixels); a sprite pass still in flight may use the old list, so it ends
+    // before the list grows
+    const size_t big_bytes = sizeof(uint32_t) * (f->info.n + 1);
+    if (c->d_sbig.cap < big_bytes && c->sprites_pending) HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, c->d_sbig.ensure(big_bytes));
     if (d_depth) {
         if (!device_range(c->device, d_depth, 4 * npx))
             return fail(c, PM_ERR_ARG, "pm_depth_sprites_device: the image must be device memory of GPU %d, W*H floats",
                         c->device);
         HIPCHK(c, launch_depth_sprites(S, (const float4 *)f->sorted.p, (uint32_t)f->info.n, (uint32_t *)d_depth,
-                                       c->stream));
+                                       (uint32_t *)c->d_sbig.p, (uint32_t)f->info.n, c->stream));
         HIPCHK(c, hipEventRecord(c->ev_sprites, c->stream));
         c->sprites_pending = true;
         c->sprites_frame = frame;
@@ -1372,7 +1380,7 @@ static int sprites_launch(pm_ctx *c, int32_t frame, const float view[16], const 
     }
     HIPCHK(c, c->d_out.ensure(4 * npx));
     HIPCHK(c, launch_depth_sprites(S, (const float4 *)f->sorted.p, (uint32_t)f->info.n, (uint32_t *)c->d_out.p,
-                                   c->stream));
+                                   (uint32_t *)c->d_sbig.p, (uint32_t)f->info.n, c->stream));
     HIPCHK(c, hipMemcpyAsync(depth, c->d_out.p, 4 * npx, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PM_OK;

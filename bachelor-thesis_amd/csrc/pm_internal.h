@@ -154,7 +154,10 @@ struct SpriteLaunch {
     float radius;
     uint32_t W, H;
 };
-hipError_t launch_depth_sprites(const SpriteLaunch &S, const float4 *pts, uint32_t n, uint32_t *depth, hipStream_t st);
+// big: the large-footprint list (1 + cap u32: count, then indices), or null to
+// draw every sprite with its own thread.
+hipError_t launch_depth_sprites(const SpriteLaunch &S, const float4 *pts, uint32_t n, uint32_t *depth, uint32_t *big,
+                                uint32_t cap, hipStream_t st);
 hipError_t launch_depth_smooth(const float *in, float *out, uint32_t W, uint32_t H, int N, const float *K,
                                hipStream_t st);
 hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz, uint8_t *out, hipStream_t st);

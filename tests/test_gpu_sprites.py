@@ -5,6 +5,8 @@ skips only atomics that cannot lower it), for the reference's perspective and
 an oblique projection, and the reference march
 started from the GPU's sprite depths (PerPixel_Isotropic's depth-buffer start,
 RayMarcher.cpp:256-344) equals the oracle's march started from its own."""
+import time
+
 import numpy as np
 import pytest
 
@@ -52,6 +54,30 @@ def test_depth_sprites_dam_and_edges(marcher):
     for cam in (sc.cameras[0], make_camera(sc.xyz.mean(0), sc.xyz.mean(0) + [0.0, 0.0, 1.0], 203, 77)):
         got = marcher.depth_sprites(1, cam, sc.h)
         assert np.array_equal(got, O.depth_sprites(sc.xyz, cam, sc.h))
+
+
+def test_depth_sprites_large_footprints(marcher):
+    """A camera inside the lattice cube at 640x360: most quads cover more than
+    1024 pixels (about 1,800 on average), so most sprites are listed and drawn
+    by a whole wave, the rest by their own thread; the image equals the
+    oracle's.  Then the 1 M dam break from inside at 1920x1080 (timed only)."""
+    sc = scenes.lattice_cube(0)
+    c = sc.xyz.mean(0)
+    cam = make_camera(c, c + np.array([0.3, 0.2, 1.0]), 640, 360)
+    print("depth sprites: camera inside the lattice cube, 640x360", flush=True)
+    marcher.set_frame(2, sc.xyz, sc.h)
+    got = marcher.depth_sprites(2, cam, sc.h)
+    assert np.array_equal(got, O.depth_sprites(sc.xyz, cam, sc.h))
+    assert (got < 1.0).mean() > 0.9
+    dam = scenes.dam_break(0)
+    inside = make_camera(dam.xyz.mean(0), dam.xyz.mean(0) + np.array([0.0, 0.0, 1.0]), 1920, 1080)
+    marcher.set_frame(2, dam.xyz, dam.h)
+    marcher.depth_sprites(2, inside, dam.h)
+    t0 = time.perf_counter()
+    img = marcher.depth_sprites(2, inside, dam.h)
+    print(f"1 M dam break, camera inside, 1920x1080: {1e3 * (time.perf_counter() - t0):.2f} ms "
+          f"(host image out), {(img < 1.0).mean():.3f} of the pixels drawn", flush=True)
+    assert (img < 1.0).mean() > 0.9
 
 
 def test_reference_march_from_sprites(marcher):
