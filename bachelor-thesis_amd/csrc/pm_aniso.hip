@@ -587,8 +587,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
 }
 
-// The configured march over the active tiles only (the tile pre-pass has
-// written every pixel of the other tiles as a miss), one tile per wave: every
+// The march over the active tiles only (the tile pre-pass has written every
+// pixel of the other tiles as a miss; in reference mode the active tiles are
+// those with a drawn sprite depth), one tile per wave: every
 // wave that starts has rays to march, and the heavy tiles are not dispatched
 // between thousands of empty ones.  Workgroups go to the 8 XCDs round-robin
 // (workgroup b to XCD b mod 8), so XCD x takes the x-th eighth of the active
@@ -605,7 +606,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     const uint32_t tile = order[i], tiles_x = (L.W + 7) / 8;
     const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
     Counts cn;
-    if (px < L.W && py < L.H) an_pixel_config(L, px, py, cn);
+    if (px < L.W && py < L.H) {
+        if (L.s.mode == PM_MODE_REFERENCE) an_pixel_reference(L, px, py, cn);
+        else an_pixel_config(L, px, py, cn);
+    }
     flush_counters(L.counters, cn.rays, cn.in_aabb, cn.hits, cn.samples, cn.cand, cn.march);
 }
 
@@ -851,10 +855,10 @@ hipError_t launch_march_aniso(const MarchLaunch &L, hipStream_t st)
     const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
     // one ray per lane to its end (neighbouring rays of a tile sample
     // neighbouring points, so the wave's gathers share cache lines): over the
-    // active tiles after the tile pre-pass (configured march, schedule 0), or
-    // over all tiles (reference mode; or no tile buffer); or (schedule 1)
-    // persistent waves with ray refill and the hit-list finish
-    if (L.s.mode == PM_MODE_CONFIG && L.s.schedule == 0 && L.tile_buf) {
+    // active tiles after the tile pre-pass (schedule 0), or over all tiles (no
+    // tile buffer); or (schedule 1, configured march) persistent waves with
+    // ray refill and the hit-list finish
+    if (L.s.schedule == 0 && L.tile_buf) {
         hipError_t e = launch_tile_prepass(L, st);
         if (e != hipSuccess) return e;
         // 8 ceil(ceil(T / 8) / 4) workgroups: enough waves for every XCD's share

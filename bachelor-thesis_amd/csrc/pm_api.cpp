@@ -694,9 +694,9 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
     }
     if (s->anisotropic) L.an = aniso_params(f, s);
     // tile pre-pass (active-tile list, bulk miss writes) for the persistent
-    // schedule and for the anisotropic configured march
+    // schedule, the anisotropic march and the reference march
     const bool persistent = s->schedule == 0 || (!s->anisotropic && s->learned);
-    if (s->mode == PM_MODE_CONFIG && persistent) {
+    if (persistent) {
         uint64_t tiles = (uint64_t)((W + 7) / 8) * ((H + 7) / 8);
         if (na) {
             L.tile_buf = na->tiles;

@@ -325,7 +325,8 @@ __device__ __forceinline__ uint32_t band_lo(uint32_t T, uint32_t b) { return (ui
 
 // Tile pre-pass of the persistent schedules.  tile_buf = flag[T+1] | pos[T+1] |
 // order[T+1] | scan scratch: flag = 1 for a tile one of whose rays enters the
-// padded AABB (the ray set-up and slab test of ray_setup, bit for bit), pos
+// padded AABB (the ray set-up and slab test of ray_setup, bit for bit) -- in
+// reference mode, one of whose pixels has a depth other than 1 -- pos
 // its exclusive scan, order the active tiles in row order (order[T] = their
 // count).  The persistent waves take only active tiles; k_miss_tiles writes
 // the misses of the others.
@@ -338,7 +339,9 @@ __global__ __launch_bounds__(256) void k_tile_flags(MarchLaunch L, uint32_t T)
     if (t < T) {
         const uint32_t tiles_x = (L.W + 7) / 8;
         const uint32_t px = (t % tiles_x) * 8 + (j & 7u), py = (t / tiles_x) * 8 + (j >> 3);
-        if (px < L.W && py < L.H) {
+        if (px < L.W && py < L.H && L.s.mode == PM_MODE_REFERENCE) {
+            enters = L.depth[py * L.W + px] != 1.0f;   // the reference march starts where a sprite was drawn
+        } else if (px < L.W && py < L.H) {
             const float cx = (float)px * (2.0f / (float)L.W) - 1.0f;
             const float cy = (float)py * (2.0f / (float)L.H) - 1.0f;
             float wx, wy, wz, d[3], tn, tf;
@@ -862,13 +865,15 @@ __global__ void k_zero_u32(uint32_t *p, int n)
 // PerPixel_Isotropic (RayMarcher.cpp:256-344): start on the depth-buffer
 // surface, fixed step vector, MaxSteps samples, first sample with rho >= iso,
 // normal = normalize(sum gradW) (or central differences on request).
-__global__ __launch_bounds__(256) void k_march_reference(MarchLaunch L)
+struct RefCounts {
+    uint32_t rays = 0, in = 0, hits = 0, samples = 0, ms = 0;
+    unsigned long long cand = 0;
+};
+
+__device__ __forceinline__ void ref_pixel(const MarchLaunch &L, uint32_t px, uint32_t py, bool active, RefCounts &rc)
 {
-    uint32_t px, py;
-    if (!pm_tile(L.W, L.H, px, py)) return;
     const GridDev &g = L.g;
     const pm_settings &s = L.s;
-    const bool active = px < L.W && py < L.H;
     const uint32_t idx = py * L.W + px;
     const bool comp = s.compensated != 0;
     uint32_t c_rays = 0, c_in = 0, c_hits = 0, c_samples = 0, c_ms = 0;
@@ -939,7 +944,36 @@ __global__ __launch_bounds__(256) void k_march_reference(MarchLaunch L)
         L.nrm[idx] = N;
         if (L.t) L.t[idx] = tout;
     }
-    flush_counters(L.counters, c_rays, c_in, c_hits, c_samples, c_cand, c_ms);
+    rc.rays += c_rays; rc.in += c_in; rc.hits += c_hits; rc.samples += c_samples; rc.ms += c_ms;
+    rc.cand += c_cand;
+}
+
+__global__ __launch_bounds__(256) void k_march_reference(MarchLaunch L)
+{
+    uint32_t px, py;
+    if (!pm_tile(L.W, L.H, px, py)) return;
+    RefCounts rc;
+    ref_pixel(L, px, py, px < L.W && py < L.H, rc);
+    flush_counters(L.counters, rc.rays, rc.in, rc.hits, rc.samples, rc.cand, rc.ms);
+}
+
+// The reference march over the active tiles of the tile pre-pass only (tiles
+// with a pixel whose depth is not 1; the pre-pass wrote the others' misses),
+// one tile per wave, the active list spread over the 8 XCDs as
+// k_march_aniso_active spreads it.
+__global__ __launch_bounds__(256) void k_march_reference_active(MarchLaunch L)
+{
+    const uint32_t T = ((L.W + 7) / 8) * ((L.H + 7) / 8);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t *order = L.tile_buf + 2 * (T + 1);
+    const uint32_t count = order[T], chunk = (count + 7) / 8;
+    const uint32_t local = (blockIdx.x >> 3) * 4 + wave, i = (blockIdx.x & 7u) * chunk + local;
+    if (local >= chunk || i >= count) return;   // whole wave
+    const uint32_t tile = order[i], tiles_x = (L.W + 7) / 8;
+    const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u), py = (tile / tiles_x) * 8 + (lane >> 3);
+    RefCounts rc;
+    ref_pixel(L, px, py, px < L.W && py < L.H, rc);
+    flush_counters(L.counters, rc.rays, rc.in, rc.hits, rc.samples, rc.cand, rc.ms);
 }
 
 // Batched schedule: persistent waves take chunks of tiles (each chunk belongs to
@@ -1172,6 +1206,13 @@ hipError_t launch_march(const MarchLaunch &L, hipStream_t st)
     if (L.s.anisotropic) return launch_march_aniso(L, st);
     if (L.s.mode == PM_MODE_REFERENCE) {
         const uint32_t tiles = ((L.W + 7) / 8) * ((L.H + 7) / 8);
+        if (L.tile_buf && L.s.schedule == 0) {
+            hipError_t e = launch_tile_prepass(L, st);
+            if (e != hipSuccess) return e;
+            const uint32_t per_xcd = ((tiles + 7) / 8 + 3) / 4;
+            hipLaunchKernelGGL(k_march_reference_active, dim3(8 * per_xcd), dim3(256), 0, st, L);
+            return hipGetLastError();
+        }
         hipLaunchKernelGGL(k_march_reference, dim3((tiles + 3) / 4), dim3(256), 0, st, L);
         return hipGetLastError();
     }

@@ -283,6 +283,87 @@ def bench_aniso(args):
     print(json.dumps(out))
 
 
+def bench_reference(args):
+    """The reference's own isotropic march -- the facade's path -- on the 1M dam
+    break at 1080p: rays start on the GPU sprite pass's depths
+    (PerPixel_Isotropic, RayMarcher.cpp:256-344), MaxSteps 128 of the
+    reference's step (0.009 at h = 0.1, scaled to h), the DensityGrid box skip,
+    normalize(sum gradW), plain FP32 sums."""
+    import pmarch
+    from pmarch import scenes
+
+    scene = scenes.dam_break(0)
+    cam = scene.cameras[0]
+    W, H = cam.width, cam.height
+    m = pmarch.Marcher(0)
+    fi = m.set_frame(0, scene.xyz, scene.h)
+    depth = m.depth_sprites(0, cam, scene.h)
+    st = dict(mode=1, max_steps=128, step=float(np.float32(0.009 * scene.h / 0.1)), iso=scene.iso,
+              normal_mode=1, skip_mode=2, compensated=0)
+    for _ in range(args.warmup):
+        m.march_async(0, st, cam.inv_pv, cam.eye, W, H, depth=depth, outputs=False)
+        m.wait()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.march_async(0, st, cam.inv_pv, cam.eye, W, H, depth=depth, outputs=False)
+        m.wait()
+        kms.append(m.stats()["kernel_ms"])
+    wall = time.perf_counter() - t0
+    stats = m.stats()
+    kms = float(np.mean(kms))
+    B = alg_bytes(stats, "cand") + 4.0 * W * H         # + the depth image
+    out = {
+        "metric": "Mrays/s @1080p/1M particles, plain vs learned-step march, at fixed depth error",
+        "value": round(args.steps * W * H / wall / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "reference-mode march (PerPixel_Isotropic: start on the sprite depths, MaxSteps "
+                               "128, step 0.009 h / 0.1, DensityGrid box skip, normalize(sum gradW), plain FP32 "
+                               "sums) of the 1,024,000-particle dam break, 1920x1080 view",
+                   "particles": scene.n, "width": W, "height": H, "h": scene.h, "step": st["step"],
+                   "iso": scene.iso, "path": "reference"},
+        "roofline": {"bound": "valu", "achieved": round(B / (kms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes": "12 cand (walk candidates) + 72 samples + 32 rays + 4 depth per ray"},
+        "kernel_ms": round(kms, 4),
+        "march_counters": {k: stats[k] for k in ("rays", "rays_in_aabb", "hits", "samples", "cand")},
+        "grid_build_ms": round(float(fi.build_ms), 3),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+
+        threads = max(1, min(16, os.cpu_count() or 2) - 1)
+        G = O.Grid(scene.xyz, scene.h)
+        ost = O.make_settings(**st)
+        step = 1
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, _, _, cnt = O.march(G, ost, cam.inv_pv, cam.eye, W, H, depth=depth, row_step=step, threads=threads,
+                                   want_t=False)
+            runs.append(time.perf_counter() - t0)
+            if sum(runs) > args.cpu_budget:
+                break
+        dt = float(np.median(runs))
+        out["cpu_baseline"] = {"value": round(cnt["rays"] / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the same {W}x{H} view ({cnt['rays']} rays) from the same depths, "
+                                         f"median of {len(runs)} runs of {dt:.2f} s, {threads} threads on "
+                                         f"{os.cpu_count()} cpus ({cpu_model()})"}
+    m.close()
+    print(json.dumps(out))
+
+
 def bench_autoplay(args):
     """16 frames of the 1M dam break, each marched once at 1080p while the next
     one streams in, against the serial order."""
@@ -638,7 +719,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["dam_break_1080p", "clustered_4k", "batch", "aniso_dam_1080p",
-                                           "depth_smooth_1080p", "autoplay_dam_1080p", "sprites_dam_1080p"],
+                                           "depth_smooth_1080p", "autoplay_dam_1080p", "sprites_dam_1080p",
+                                           "reference_dam_1080p"],
                     default="dam_break_1080p")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -662,6 +744,9 @@ def main():
         return
     if args.workload == "sprites_dam_1080p":
         bench_sprites(args)
+        return
+    if args.workload == "reference_dam_1080p":
+        bench_reference(args)
         return
 
     import pmarch
