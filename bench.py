@@ -133,18 +133,29 @@ def image_errors(img, ref, ext: float, step: int) -> dict:
                                      and (ang.max() if ang.size else 0) <= 0.5)}
 
 
-def fp64_errors(scene, cam, img, step: int) -> dict:
+def fp64_errors(scene, cam, img, step: int, settings=None, xyz=None) -> dict:
     """The oracle leg's check of the GPU image: the FP64 oracle march (every
     operation in double, all bisection steps in double) on every step-th row of
     the same view, against the GPU's rows."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
-    G = O.Grid(scene.xyz, scene.h)
-    st = O.make_settings(**{k: v for k, v in scene.settings().items() if k in dict(O.Settings._fields_)})
+    xyz = scene.xyz if xyz is None else xyz
+    settings = scene.settings() if settings is None else settings
+    G = O.Grid(xyz, scene.h)
+    st = O.make_settings(**{k: v for k, v in settings.items() if k in dict(O.Settings._fields_)})
     ref = O.march64(G, st, cam.inv_pv, cam.eye, cam.width, cam.height, row_step=step)
-    ext = float((scene.xyz.max(0) - scene.xyz.min(0)).max())
+    ext = float((xyz.max(0) - xyz.min(0)).max())
     return image_errors(img, ref, ext, step)
+
+
+def merge_errors(errs) -> dict:
+    """The worst of several images' errors."""
+    return {"images": len(errs), "rows": sum(e["rows"] for e in errs),
+            "mask_agree": min(e["mask_agree"] for e in errs),
+            "depth_max_over_extent": max(e["depth_max_over_extent"] for e in errs),
+            "normal_max_deg": max(e["normal_max_deg"] for e in errs),
+            "within_tolerance": all(e["within_tolerance"] for e in errs)}
 
 
 def cpu_baseline(scene, cam, settings, budget_s: float, xyz=None):
@@ -279,6 +290,8 @@ def bench_aniso(args):
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_aniso(scene, cam, st_in, args.cpu_budget)
+        img = m.march(0, st_in, cam.inv_pv, cam.eye, cam.width, cam.height)[:3]
+        out["error_vs_fp64"] = fp64_errors(scene, cam, img, 16, settings=st_in)
     m.close()
     print(json.dumps(out))
 
@@ -709,6 +722,12 @@ def bench_batch(args):
     if not args.no_cpu:
         c0 = cams[0]
         out["cpu_baseline"] = cpu_baseline(base, c0, base.settings(), args.cpu_budget)
+        # the oracle leg's check: four view-frames spread over cameras and
+        # frames against the FP64 march of their frame, every 32nd row
+        errs = []
+        for i in (0, 341, 682, 1023):
+            errs.append(fp64_errors(base, cams[i // 16], m.batch_download(i), 32, xyz=frames[i % 16]))
+        out["error_vs_fp64"] = merge_errors(errs)
     m.close()
     print(json.dumps(out))
 
