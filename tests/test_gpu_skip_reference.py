@@ -75,3 +75,28 @@ def test_reference_mode_pool_skip(marcher, pool):
         opos, onrm, ot, ocnt = O.march(G, O.make_settings(**st), c.inv_pv, c.eye, c.width, c.height, depth=depth)
         assert np.array_equal(pos, opos) and np.array_equal(t, ot)
         assert stats["samples"] == ocnt["samples"]
+
+
+@pytest.mark.parametrize("size", [(203, 77), (64, 8), (300, 9)])
+def test_reference_mode_ragged_active_tiles(marcher, size):
+    """The reference march runs over the tiles with a drawn depth (the tile
+    pre-pass writes the others' misses): ragged image sizes, tiles partly
+    outside the image and tiles with no depth at all, against the oracle;
+    every pixel is counted once, the started rays are the pixels whose depth
+    is not 1."""
+    W, H = size
+    sc = scenes.lattice_cube(2, width=W, height=H)
+    c = sc.cameras[1]
+    marcher.set_frame(12, sc.xyz, sc.h)
+    G = O.Grid(sc.xyz, sc.h)
+    depth = depth_in_front(c, G.bmin, G.bmax, 0.05)
+    assert 0 < (depth != 1.0).sum() < W * H
+    st = dict(mode=1, max_steps=128, step=0.009, iso=sc.iso, normal_mode=1, skip_mode=2, compensated=0)
+    pos, nrm, t, stats = marcher.march(12, st, c.inv_pv, c.eye, W, H, depth=depth)
+    opos, onrm, ot, ocnt = O.march(G, O.make_settings(**st), c.inv_pv, c.eye, W, H, depth=depth)
+    assert np.array_equal(pos, opos) and np.array_equal(t, ot)
+    assert np.array_equal(np.isnan(nrm), np.isnan(onrm))
+    ok = ~np.isnan(onrm)
+    assert np.array_equal(nrm[ok], onrm[ok])
+    assert stats["rays"] == W * H and stats["rays_in_aabb"] == int((depth != 1.0).sum())
+    assert stats["samples"] == ocnt["samples"] and stats["hits"] == ocnt["hits"]
