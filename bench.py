@@ -10,11 +10,15 @@ central-difference normals, compensated FP32 density sums, sub-box bound skip
 (exact: the image is that of the march without it).  Particles and grid are
 resident in HBM before the timed region; outputs stay in device memory.
 
-Both forms are timed: the plain march and the learned step proposal (network
-from pmarch/weights/step_mlp.npz).  `value` is the faster form's Mrays/s among
-those whose image is within the tolerances of the plain image (mask >= 99.9 %,
-99.9th percentile depth error <= 1e-4 x extent and normal error <= 0.5 deg);
-both are reported.  Prints one JSON line.
+Three forms are timed, with identical images (every pixel is checked): the
+march with a fixed step from the AABB entry (the configuration as written), the
+same with the exact sub-box bound skip, and the learned step proposal on top of
+the bound skip (network from pmarch/weights/step_mlp.npz; its jumps are
+certified, so its image is the plain one).  `value` is the fastest form's
+Mrays/s; all three are reported, with the roofline of the value form from an
+instrumented launch of the same view, the PCIe-inclusive rate, the bound-bit
+build time, the CPU baseline and the FP64 errors of the image.  Prints one
+JSON line.
 
 --workload clustered_4k: the 4,000,000-particle clustered pool at 3840x2160,
 same march and forms.
@@ -40,8 +44,11 @@ one play of the 16 frames.
 
 --workload sprites_dam_1080p: the depth-sprite pass (DepthRenderPass: every
 particle of the 1M dam break as a sphere sprite of radius h, depth test LESS)
-for the 1080p view, host image out; one step = one pass, wall clock including
-the image's copy to the host (the kernel alone is in the rocprof summary).
+for the 1080p view into device memory; one step = one pass.
+
+--workload reference_dam_1080p: the reference's own isotropic march (the
+facade's path) from the GPU sprite pass's depths: MaxSteps 128, DensityGrid
+box skip, normalize(sum gradW), plain sums.
 
 --workload depth_smooth_1080p: the screen-space depth smoothing
 (GaussRenderPass, N = 8: a 17x17 window) of a 1920x1080 depth image resident
