@@ -73,13 +73,17 @@ typedef struct pm_settings {
     int32_t normal_mode;   /* PM_NORMAL_*                                             */
     int32_t skip_mode;     /* PM_SKIP_*                                               */
     int32_t compensated;   /* 1 = Neumaier-compensated FP32 density sums              */
-    int32_t learned;       /* 1 = learned step proposal (needs pm_set_mlp)            */
-    int32_t schedule;      /* kernel schedule, same images and counters: 0 = persistent
-                              waves over the tiles a ray of which enters the AABB, with
-                              ray refill, lane groups and per-wave finishing (the
-                              product); 1 = one 8x8 tile per wave, one ray per lane to
-                              its end, direct sums (the plain schedule the persistent
-                              one is checked against)                                */
+    int32_t learned;       /* 1 = learned step proposal (needs pm_mlp_set)            */
+    int32_t schedule;      /* kernel schedule, same images and counters: 0 = over the
+                              8x8 tiles a ray of which enters the AABB (reference
+                              mode: with a drawn depth), the others answered as misses
+                              at once -- the isotropic configured march with persistent
+                              waves, ray refill, lane groups and per-wave finishing,
+                              the anisotropic and reference marches one tile per wave
+                              (the product); 1 = every tile, one ray per lane to its
+                              end, direct sums (the anisotropic configured march:
+                              persistent waves with ray refill) -- the schedules the
+                              product is checked against                           */
     float   learned_alpha; /* learned: fraction of the predicted distance skipped (0 = 0.7) */
     int32_t learned_min_skip; /* learned: shorter proposals are not taken (0 = 3, >= 2) */
     int32_t learned_max_skip; /* learned: longest jump in steps (0 = 8)               */
