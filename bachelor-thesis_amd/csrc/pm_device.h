@@ -322,22 +322,28 @@ __device__ __forceinline__ float pm_density32_lds(const GridDev &g, float qx, fl
                 pm_fine_row(g, fw, dy, dz, tz2, b, e);
                 cand += e - b;
                 if (__ballot(e > b) == 0) continue;         // no lane has candidates in this row
-                if (__ballot(n + (int)(e - b) > CAP) != 0) {   // some lane might overflow:
+                if (__ballot(n + (int)(e - b) >= CAP) != 0) {  // some lane might fill the buffer:
                     pm_drain<CAP>(g, buf, n, s, c);           // all drain together
                     drained += (uint32_t)n;
                     n = 0;
                 }
-                if ((int)(e - b) <= CAP) {
-                    const float4 *p = g.fsorted + b;
-                    const float4 *const pe = g.fsorted + e;
-                    for (; p + UNROLL <= pe; p += UNROLL) {
+                if ((int)(e - b) < CAP) {
+                    // groups of UNROLL loads, the last group's unused slots re-reading
+                    // the group's first candidate; slot n stays free for their stores
+                    const int m = (int)(e - b);
+                    const float4 *const p0 = g.fsorted + b;
+                    for (int i = 0; i < m; i += UNROLL) {
                         float4 v[UNROLL];
 #pragma unroll
-                        for (int u = 0; u < UNROLL; u++) v[u] = p[u];
+                        for (int u = 0; u < UNROLL; u++) v[u] = p0[i + u < m ? i + u : i];
 #pragma unroll
-                        for (int u = 0; u < UNROLL; u++) pm_test_append(g, v[u], qx, qy, qz, buf, n);
+                        for (int u = 0; u < UNROLL; u++) {
+                            const float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
+                            const float r2 = (rx * rx + ry * ry) + rz * rz;
+                            buf[n * 64] = r2;
+                            n += (i + u < m && r2 < g.h2) ? 1 : 0;
+                        }
                     }
-                    for (; p < pe; p++) pm_test_append(g, *p, qx, qy, qz, buf, n);
                 } else {
                     // a row longer than the buffer (dense clusters hold hundreds of
                     // particles per row): in chunks that fit, loaded UNROLL at a time
