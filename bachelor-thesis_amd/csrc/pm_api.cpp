@@ -46,10 +46,11 @@ struct Frame {
     DevBuf sorted, cell_start, cell_of, counts, cursor, scan_tmp, xyz, bounds_partial, bounds, scattered;
     DevBuf empty27, dg_box, dg_cons, dg_count, coarse, bound;
     DevBuf fsorted, fcell_start;    // march grid (cells of h/2)
-    bool bound_ok = false;      // bound holds the sub-box bits for bound_iso
+    bool bound_ok = false;      // bound holds the sub-box bits for bound_iso and bound_boxes
     hipEvent_t bound_ev[2] = { nullptr, nullptr };   // around the last bound build
     bool dg_ok = false;         // the DensityGrid flags are built
     float bound_iso = 0.0f;
+    int32_t bound_boxes = 0;    // pm_settings.bound_boxes of those bits
     int32_t dg_dims[3] = { 0, 0, 0 };
     CoarseDev cg{};
     void release()
@@ -75,6 +76,7 @@ struct pm_ctx {
     bool outputs_on_device = false;       // the last march wrote the caller's device memory
     bool bound_used = false;              // a march has used the bound skip ...
     float bound_iso_last = 0.0f;          // ... with this iso (its bits' key)
+    int32_t bound_boxes_last = 0;         // ... and these sub-boxes
     hipEvent_t ev_launch = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
     // a depth-sprite pass into the caller's device memory, queued on `stream`
     // and not yet waited for (pm_depth_sprites_device)
@@ -406,7 +408,7 @@ int pm_set_frame(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, float h
     return build_frame(c, frame, (const float *)f->xyz.p, n, h, c->stream);
 }
 
-static int ensure_bound(pm_ctx *c, Frame *f, float iso, hipStream_t st);
+static int ensure_bound(pm_ctx *c, Frame *f, float iso, int32_t boxes, hipStream_t st);
 
 int pm_set_frame_stream(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, float h)
 {
@@ -428,8 +430,8 @@ int pm_set_frame_stream(pm_ctx *c, int32_t frame, const float *xyz, uint64_t n, 
     // Autoplay marches every frame with the same settings: the skip's bits for
     // the iso of the last bound-skip march are built here too, beside the march
     // in flight, instead of at the start of this frame's march.  A march with
-    // another iso rebuilds them (the bits are keyed by iso).
-    rc = ensure_bound(c, f, c->bound_iso_last, c->stream_build);
+    // another iso or sub-box size rebuilds them (the bits are keyed by both).
+    rc = ensure_bound(c, f, c->bound_iso_last, c->bound_boxes_last, c->stream_build);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream_build));
     return PM_OK;
@@ -518,18 +520,19 @@ static int ensure_densitygrid(pm_ctx *c, Frame *f)
 
 // Sub-box density bounds of a frame for one iso value, built on the context
 // stream when first needed (a batch holding the old bits is invalidated).
-static int ensure_bound(pm_ctx *c, Frame *f, float iso, hipStream_t st)
+static int ensure_bound(pm_ctx *c, Frame *f, float iso, int32_t boxes, hipStream_t st)
 {
-    if (f->bound_ok && f->bound_iso == iso) return PM_OK;
-    HIPCHK(c, f->bound.ensure(sizeof(uint64_t) * f->info.ncells));
+    if (f->bound_ok && f->bound_iso == iso && f->bound_boxes == boxes) return PM_OK;
+    HIPCHK(c, f->bound.ensure(sizeof(uint64_t) * 8 * f->info.ncells));
     f->g.bound = (const uint64_t *)f->bound.p;
     for (hipEvent_t &e : f->bound_ev)
         if (!e) HIPCHK(c, hipEventCreate(&e));
     HIPCHK(c, hipEventRecord(f->bound_ev[0], st));
-    HIPCHK(c, launch_bound_grid(f->g, iso, (uint64_t *)f->bound.p, st));
+    HIPCHK(c, launch_bound_grid(f->g, iso, boxes != 0, (uint64_t *)f->bound.p, st));
     HIPCHK(c, hipEventRecord(f->bound_ev[1], st));
     f->bound_ok = true;
     f->bound_iso = iso;
+    f->bound_boxes = boxes;
     c->frame_epoch++;
     return PM_OK;
 }
@@ -543,6 +546,7 @@ static int validate_settings(pm_ctx *c, const pm_settings *s, const Frame *f)
     if (s->bisect_iters < 0 || s->bisect_iters > 40) return fail(c, PM_ERR_ARG, "bisect_iters out of [0,40]");
     if (s->normal_mode != PM_NORMAL_CDIFF && s->normal_mode != PM_NORMAL_GRADW) return fail(c, PM_ERR_ARG, "unknown normal_mode");
     if (s->skip_mode < PM_SKIP_NONE || s->skip_mode > PM_SKIP_BOUND) return fail(c, PM_ERR_ARG, "unknown skip_mode %d", s->skip_mode);
+    if (s->bound_boxes != 0 && s->bound_boxes != 1) return fail(c, PM_ERR_ARG, "bound_boxes must be 0 or 1");
     if (s->mode == PM_MODE_CONFIG && (s->skip_mode == PM_SKIP_DG_BOX || s->skip_mode == PM_SKIP_DG_CONS))
         return fail(c, PM_ERR_ARG, "the DensityGrid skip belongs to the reference march");
     if (s->mode == PM_MODE_REFERENCE && (s->skip_mode == PM_SKIP_EMPTY || s->skip_mode == PM_SKIP_BOUND))
@@ -641,14 +645,16 @@ static int march_launch(pm_ctx *c, int32_t frame, const pm_settings *s, const fl
         // the bound bits for PM_ZERO_ISO mark those sub-boxes
         const float biso = s->anisotropic ? PM_ZERO_ISO : s->iso;
         if (na) {
-            if (!(f->bound_ok && f->bound_iso == biso))
+            if (!(f->bound_ok && f->bound_iso == biso && f->bound_boxes == s->bound_boxes))
                 return fail(c, PM_ERR_STATE, "pm_march_device_noalloc: frame %d has no bound bits for iso %g "
-                            "(build them first: pm_get_bound_grid or one pm_march_async_device)", frame, (double)biso);
+                            "and bound_boxes %d (build them first: pm_get_bound_grid or one pm_march_async_device)",
+                            frame, (double)biso, s->bound_boxes);
         } else {
-            rc = ensure_bound(c, f, biso, c->stream);
+            rc = ensure_bound(c, f, biso, s->bound_boxes, c->stream);
             if (rc) return rc;
         }
         c->bound_iso_last = biso;
+        c->bound_boxes_last = s->bound_boxes;
         c->bound_used = true;
     }
     if (s->skip_mode == PM_SKIP_DG_BOX || s->skip_mode == PM_SKIP_DG_CONS) {
@@ -912,17 +918,18 @@ int pm_get_march_grid(pm_ctx *c, int32_t frame, uint32_t *cell_start, float *sor
     return PM_OK;
 }
 
-int pm_get_bound_grid(pm_ctx *c, int32_t frame, float iso, uint64_t *bits)
+int pm_get_bound_grid(pm_ctx *c, int32_t frame, float iso, int32_t boxes, uint64_t *bits)
 {
     if (!c || !bits) return PM_ERR_ARG;
+    if (boxes != 0 && boxes != 1) return fail(c, PM_ERR_ARG, "pm_get_bound_grid: boxes must be 0 or 1");
     if (c->pending) return fail(c, PM_ERR_STATE, "pm_get_bound_grid: a march is in flight");
     if (!std::isfinite(iso)) return fail(c, PM_ERR_ARG, "iso must be finite");
     Frame *f = get_frame(c, frame);
     if (!f) return fail(c, PM_ERR_FRAME, "frame %d not set", frame);
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_bound(c, f, iso, c->stream);
+    int rc = ensure_bound(c, f, iso, boxes, c->stream);
     if (rc) return rc;
-    HIPCHK(c, copy_sync(c, bits, f->bound.p, sizeof(uint64_t) * f->info.ncells, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, bits, f->bound.p, sizeof(uint64_t) * 8 * f->info.ncells, hipMemcpyDeviceToHost));
     return PM_OK;
 }
 
@@ -955,7 +962,7 @@ int pm_batch_set(pm_ctx *c, const pm_settings *s, const pm_view *views, uint32_t
         int rc = validate_settings(c, s, f);
         if (rc) return rc;
         if (s->skip_mode == PM_SKIP_BOUND) {
-            rc = ensure_bound(c, f, s->iso, c->stream);
+            rc = ensure_bound(c, f, s->iso, s->bound_boxes, c->stream);
             if (rc) return rc;
         }
         c->b_off[v] = npx;

@@ -31,7 +31,7 @@ struct GridDev {
     double h_d, h2_d, hinv_d, sig_d;
     float bmin[3], bmax[3];   // padded AABB
     const uint8_t *empty27;   // per cell: 1 if the cell and its 26 neighbours are empty
-    const uint64_t *bound;    // per cell: bit l set when sub-box l (4x4x4) stays below iso
+    const uint64_t *bound;    // per cell 8 words: the 512 eighth-box bits (set: the box stays below iso)
     // reference DensityGrid over the padded AABB (Dataset.cpp:94-164)
     const uint8_t *dg_flag;   // flag per node for the selected skip rule
     int32_t dg_nx, dg_ny, dg_nz;

@@ -249,80 +249,168 @@ hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz
 }
 
 // Density bound per sub-box.  Every cell of the dense grid is split into
-// 4x4x4 sub-boxes; sub-box l = sx + 4 sy + 16 sz holds the points whose
-// u = inv_h * x lies in [key + s/4, key + (s+1)/4) on every axis.  For the box
-// B widened by delta (which covers the rounding of u and of the march's sample
-// positions), U = sum over the cell's 27-cell stencil of W(dist(x_j, B)) bounds
-// the density at every point of the sub-box, since W falls with r and the
-// stencil holds every particle within h.  Bit l of bound[c] is set when
-// U < iso (1 - 2^-9); the margin covers the rounding of both sums.  U sums the
+// 8x8x8 sub-boxes (eighth boxes); sub-box (fx, fy, fz) holds the points whose
+// u = inv_h * x lies in [key + f/8, key + (f+1)/8) on every axis, and quarter
+// box s = f / 2 (4x4x4 per cell) the eight sub-boxes with the same s.  For a
+// box B widened by delta (which covers the rounding of u and of the march's
+// sample positions), U = sum over the cell's 27-cell stencil of W(dist(x_j, B))
+// bounds the density at every point of B, since W falls with r and the
+// stencil holds every particle within h.  A sub-box's bit is set when U of its
+// quarter box, or else its own U, is below iso (1 - 2^-9); the margin covers
+// the rounding of both sums.  Cells whose centre row (dz = dy = 0) alone takes
+// every quarter box's U past the limit lie inside the fluid and get no bit (a
+// clear bit only means the sample is evaluated), which spares their eight
+// sub-box sums.  A cell's 512 bits are 64 bytes: byte
+// l = sx + 4 sy + 16 sz belongs to quarter box s and its bit
+// (fx & 1) + 2 (fy & 1) + 4 (fz & 1) to the sub-box inside it, so a byte of
+// 0xff marks a quarter box the march can cross in one jump.  Each U sums the
 // centre row (dz = dy = 0) first, then the other eight in (dz, dy) order,
-// particles in storage order.  Border
-// cells (no stencil, zero density) and cells with an empty 27-neighbourhood get
-// every bit.  One wave per cell: lane l evaluates sub-box l, all lanes read
-// the same particle, and the walk stops once every lane has reached the limit.
+// particles in storage order.  Border cells (no stencil, zero density) and
+// cells with an empty 27-neighbourhood get every bit.  One wave per cell: lane
+// l takes quarter box l, all lanes read the same particle; the quarter boxes'
+// sums come first (one W per particle), the eight sub-box sums only for the
+// quarter boxes left at or above the limit, and each walk stops once every box
+// it sums has reached the limit.
+// Squared distance along one axis from x to [lo, hi] (0 inside).
+__device__ __forceinline__ float bound_e2(float lo, float hi, float x)
+{
+    const float e = fmaxf(fmaxf(lo - x, x - hi), 0.0f);
+    return e * e;
+}
+
 // W at the distance from particle p to the box [l, h] (0 outside the support).
 __device__ __forceinline__ float bound_term(const GridDev &g, float4 p, float lx, float hx, float ly, float hy, float lz,
                                             float hz)
 {
-    const float ex = fmaxf(fmaxf(lx - p.x, p.x - hx), 0.0f);
-    const float ey = fmaxf(fmaxf(ly - p.y, p.y - hy), 0.0f);
-    const float ez = fmaxf(fmaxf(lz - p.z, p.z - hz), 0.0f);
-    const float d2 = (ex * ex + ey * ey) + ez * ez;
+    const float d2 = (bound_e2(lx, hx, p.x) + bound_e2(ly, hy, p.y)) + bound_e2(lz, hz, p.z);
     return d2 < g.h2 ? pm_W_in(d2, g.inv_h, g.sig) : 0.0f;
 }
 
-// The bits of cell c (one wave; returns the mask for lane 0 to store).
-__device__ __forceinline__ uint64_t bound_cell(const GridDev &g, float iso, uint32_t c, uint32_t lane)
+// The eight sub-box bounds of one quarter box, one particle at a time.
+struct BoundEighths {
+    float lo[3][2], hi[3][2];   // per axis, the two sub-boxes' widened ends
+    float U[8];
+
+    __device__ void init(const GridDev &g, const int32_t k[3], const int32_t s[3], float delta)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                lo[a][b] = ((float)k[a] + 0.125f * (float)(2 * s[a] + b)) * g.h - delta;
+                hi[a][b] = ((float)k[a] + 0.125f * (float)(2 * s[a] + b + 1)) * g.h + delta;
+            }
+#pragma unroll
+        for (int f = 0; f < 8; f++) U[f] = 0.0f;
+    }
+    __device__ void add(const GridDev &g, float4 p)
+    {
+        const float ex0 = bound_e2(lo[0][0], hi[0][0], p.x), ex1 = bound_e2(lo[0][1], hi[0][1], p.x);
+        const float ey0 = bound_e2(lo[1][0], hi[1][0], p.y), ey1 = bound_e2(lo[1][1], hi[1][1], p.y);
+        const float ez0 = bound_e2(lo[2][0], hi[2][0], p.z), ez1 = bound_e2(lo[2][1], hi[2][1], p.z);
+        // no sub-box within h: all eight terms are 0
+        if (!((fminf(ex0, ex1) + fminf(ey0, ey1)) + fminf(ez0, ez1) < g.h2)) return;
+#pragma unroll
+        for (int f = 0; f < 8; f++) {
+            const float d2 = ((f & 1 ? ex1 : ex0) + (f & 2 ? ey1 : ey0)) + (f & 4 ? ez1 : ez0);
+            U[f] += d2 < g.h2 ? pm_W_in(d2, g.inv_h, g.sig) : 0.0f;
+        }
+    }
+    __device__ bool below(float lim) const
+    {
+        bool b = false;
+#pragma unroll
+        for (int f = 0; f < 8; f++) b = b || U[f] < lim;
+        return b;
+    }
+};
+
+// Row r of the bound walk's order: the centre row, then (dz, dy) in order.
+__device__ __forceinline__ uint32_t bound_row(const GridDev &g, int32_t x, int32_t y, int32_t z, int r)
+{
+    const int dz = r == 0 ? 0 : (r - 1 + (r > 4)) / 3 - 1, dy = r == 0 ? 0 : (r - 1 + (r > 4)) % 3 - 1;
+    return (uint32_t)x - 1u + (uint32_t)g.nx * ((uint32_t)(y + dy) + (uint32_t)g.ny * (uint32_t)(z + dz));
+}
+
+// The byte of quarter box `lane` of cell c (one wave; each lane stores its own).
+// quarter_only: every byte 0xff or 0 (the quarter boxes' bounds alone).
+__device__ __forceinline__ uint32_t bound_cell(const GridDev &g, float iso, bool quarter_only, uint32_t c, uint32_t lane)
 {
     const uint32_t nx = (uint32_t)g.nx, ny = (uint32_t)g.ny;
     const int32_t x = (int32_t)(c % nx), y = (int32_t)((c / nx) % ny), z = (int32_t)(c / (nx * ny));
-    if (x < 1 || y < 1 || z < 1 || x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2 || g.empty27[c]) return ~0ull;
-    const int32_t kx = x + g.ox, ky = y + g.oy, kz = z + g.oz;
-    int32_t km = abs(kx) > abs(ky) ? abs(kx) : abs(ky);
-    km = (km > abs(kz) ? km : abs(kz)) + 1;
+    if (x < 1 || y < 1 || z < 1 || x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2 || g.empty27[c]) return 0xffu;
+    const int32_t k[3] = { x + g.ox, y + g.oy, z + g.oz };
+    int32_t km = abs(k[0]) > abs(k[1]) ? abs(k[0]) : abs(k[1]);
+    km = (km > abs(k[2]) ? km : abs(k[2])) + 1;
     const float delta = g.h * (0x1p-10f + (float)km * 0x1p-18f);
-    const float sx = (float)(lane & 3u), sy = (float)((lane >> 2) & 3u), sz = (float)(lane >> 4);
-    const float lx = ((float)kx + 0.25f * sx) * g.h - delta, hx = ((float)kx + 0.25f * (sx + 1.0f)) * g.h + delta;
-    const float ly = ((float)ky + 0.25f * sy) * g.h - delta, hy = ((float)ky + 0.25f * (sy + 1.0f)) * g.h + delta;
-    const float lz = ((float)kz + 0.25f * sz) * g.h - delta, hz = ((float)kz + 0.25f * (sz + 1.0f)) * g.h + delta;
+    const int32_t sq[3] = { (int32_t)(lane & 3u), (int32_t)((lane >> 2) & 3u), (int32_t)(lane >> 4) };
     const float lim = iso * (1.0f - 0x1p-9f);
-    float U = 0.0f;
-    // the centre row first: inside the fluid it alone takes every sub-box
-    // past the limit, and the walk stops there
+    // the quarter box's bound; inside the fluid the centre row alone takes
+    // every quarter box past the limit
+    const float lx = ((float)k[0] + 0.25f * (float)sq[0]) * g.h - delta, hx = ((float)k[0] + 0.25f * (float)(sq[0] + 1)) * g.h + delta;
+    const float ly = ((float)k[1] + 0.25f * (float)sq[1]) * g.h - delta, hy = ((float)k[1] + 0.25f * (float)(sq[1] + 1)) * g.h + delta;
+    const float lz = ((float)k[2] + 0.25f * (float)sq[2]) * g.h - delta, hz = ((float)k[2] + 0.25f * (float)(sq[2] + 1)) * g.h + delta;
+    float Uq = 0.0f;
     for (int r = 0; r < 9; r++) {
-            const int dz = r == 0 ? 0 : (r - 1 + (r > 4)) / 3 - 1, dy = r == 0 ? 0 : (r - 1 + (r > 4)) % 3 - 1;
-            const uint32_t row = (uint32_t)x - 1u + nx * ((uint32_t)(y + dy) + ny * (uint32_t)(z + dz));
-            const uint32_t b = g.cell_start[row], e = g.cell_start[row + 3];
-            // all lanes read the same particles (broadcast); 8 loads in flight
-            uint32_t j = b;
-            for (; j + 8 <= e; j += 8) {
-                float4 p[8];
+        const uint32_t row = bound_row(g, x, y, z, r);
+        const uint32_t b = g.cell_start[row], e = g.cell_start[row + 3];
+        // all lanes read the same particles (broadcast); 8 loads in flight
+        uint32_t j = b;
+        for (; j + 8 <= e; j += 8) {
+            float4 p[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) p[u] = g.sorted[j + u];
+            for (int u = 0; u < 8; u++) p[u] = g.sorted[j + u];
 #pragma unroll
-                for (int u = 0; u < 8; u++) U += bound_term(g, p[u], lx, hx, ly, hy, lz, hz);
-            }
-            for (; j < e; j++) U += bound_term(g, g.sorted[j], lx, hx, ly, hy, lz, hz);
-            if (__ballot(U < lim) == 0) return 0ull;
+            for (int u = 0; u < 8; u++) Uq += bound_term(g, p[u], lx, hx, ly, hy, lz, hz);
         }
-    return __ballot(U < lim);
+        for (; j < e; j++) Uq += bound_term(g, g.sorted[j], lx, hx, ly, hy, lz, hz);
+        if (__ballot(Uq < lim) == 0) {
+            if (r == 0) return 0u;                  // inside the fluid
+            break;
+        }
+    }
+    const bool whole = Uq < lim;
+    if (quarter_only) return whole ? 0xffu : 0u;
+    if (__ballot(!whole) == 0) return 0xffu;
+    // the eight sub-boxes of the quarter boxes left at or above the limit
+    BoundEighths B;
+    B.init(g, k, sq, delta);
+    for (int r = 0; r < 9; r++) {
+        if (!whole) {
+            const uint32_t row = bound_row(g, x, y, z, r);
+            const uint32_t b = g.cell_start[row], e = g.cell_start[row + 3];
+            uint32_t j = b;
+            for (; j + 4 <= e; j += 4) {
+                float4 p[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) p[u] = g.sorted[j + u];
+#pragma unroll
+                for (int u = 0; u < 4; u++) B.add(g, p[u]);
+            }
+            for (; j < e; j++) B.add(g, g.sorted[j]);
+        }
+        if (__ballot(!whole && B.below(lim)) == 0) break;
+    }
+    if (whole) return 0xffu;
+    uint32_t m = 0;
+#pragma unroll
+    for (int f = 0; f < 8; f++) m |= (B.U[f] < lim ? 1u : 0u) << f;
+    return m;
 }
 
-// One wave per cell.
-__global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, uint64_t *__restrict__ out)
+// One wave per cell; bits[64 c + l] is the byte of quarter box l.
+__global__ __launch_bounds__(256) void k_bound_grid(GridDev g, float iso, int quarter_only, uint8_t *__restrict__ out)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= (uint32_t)g.nx * (uint32_t)g.ny * (uint32_t)g.nz) return;   // whole wave
-    const uint64_t m = bound_cell(g, iso, c, lane);
-    if (lane == 0) out[c] = m;
+    out[64 * (uint64_t)c + lane] = (uint8_t)bound_cell(g, iso, quarter_only != 0, c, lane);
 }
 
-hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, hipStream_t st)
+hipError_t launch_bound_grid(const GridDev &g, float iso, bool quarter_only, uint64_t *out, hipStream_t st)
 {
     uint32_t n = (uint32_t)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(k_bound_grid, dim3((n + 3) / 4), dim3(256), 0, st, g, iso, out);
+    hipLaunchKernelGGL(k_bound_grid, dim3((n + 3) / 4), dim3(256), 0, st, g, iso, quarter_only ? 1 : 0, (uint8_t *)out);
     return hipGetLastError();
 }
 

@@ -91,7 +91,7 @@ struct BatchLaunch {
     uint32_t nchunks;
     uint32_t *chunk_ctr;
 };
-hipError_t launch_bound_grid(const GridDev &g, float iso, uint64_t *out, hipStream_t st);
+hipError_t launch_bound_grid(const GridDev &g, float iso, bool quarter_only, uint64_t *out, hipStream_t st);
 hipError_t launch_march_batch(const BatchLaunch &B, uint32_t blocks, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long *p, uint32_t n, hipStream_t st);

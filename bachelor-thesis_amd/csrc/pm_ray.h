@@ -257,11 +257,13 @@ __device__ __forceinline__ int pm_skip_empty(const GridDev &g, float o0, float o
     return kf > (float)r.k ? (int)kf : r.k + 1;
 }
 
-// Bound skip: when the sample's sub-box (4x4x4 per cell) has its bit set in
-// g.bound, no point of the sub-box reaches iso, so the march resumes at the
-// first sample past the sub-box exit; every sample it passes lies in the
-// sub-box (within the widening the bound was computed for) and would have been
-// below iso.  Returns the next sample index (r.k when there is nothing to skip).
+// Bound skip: when the sample's eighth box (8x8x8 per cell) has its bit set
+// in g.bound, no point of the box reaches iso, so the march resumes at the
+// first sample past the box exit; when all eight eighth boxes of its quarter
+// box (4x4x4 per cell) are set, past the quarter box's exit.  Every sample it
+// passes lies in a set box (within the widening the bound was computed for)
+// and would have been below iso.  Returns the next sample index (r.k when
+// there is nothing to skip).
 __device__ __forceinline__ float pm_box_exit(float lo, float hi, float o, float d, float id, float texit)
 {
     if (d == 0.0f) return texit;
@@ -278,14 +280,26 @@ __device__ __forceinline__ int pm_skip_bound(const GridDev &g, float o0, float o
     const int32_t k2 = qz >= 0.0f ? (int32_t)u2 : (int32_t)u2 - 1;
     const int32_t cx = k0 - g.ox, cy = k1 - g.oy, cz = k2 - g.oz;
     if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) return r.k;
-    const uint64_t m = g.bound[(uint32_t)cx + (uint32_t)g.nx * ((uint32_t)cy + (uint32_t)g.ny * (uint32_t)cz)];
-    if (m == 0ull) return r.k;
-    const int s0 = min(3, (int)((u0 - (float)k0) * 4.0f));
-    const int s1 = min(3, (int)((u1 - (float)k1) * 4.0f));
-    const int s2 = min(3, (int)((u2 - (float)k2) * 4.0f));
-    if (!((m >> (s0 + 4 * s1 + 16 * s2)) & 1ull)) return r.k;
-    const float l0 = (float)k0 + 0.25f * (float)s0, l1 = (float)k1 + 0.25f * (float)s1, l2 = (float)k2 + 0.25f * (float)s2;
-    const float span = 0.25f;
+    const int f0 = min(7, (int)((u0 - (float)k0) * 8.0f));
+    const int f1 = min(7, (int)((u1 - (float)k1) * 8.0f));
+    const int f2 = min(7, (int)((u2 - (float)k2) * 8.0f));
+    const uint32_t l = (uint32_t)((f0 >> 1) + 4 * (f1 >> 1) + 16 * (f2 >> 1));
+    const uint64_t cell = (uint64_t)cx + (uint64_t)g.nx * ((uint64_t)cy + (uint64_t)g.ny * (uint64_t)cz);
+    const uint32_t byte = (uint32_t)(g.bound[8 * cell + (l >> 3)] >> (8 * (l & 7u))) & 0xffu;
+    if (byte == 0u) return r.k;
+    float l0, l1, l2, span;
+    if (byte == 0xffu) {   // the whole quarter box
+        l0 = (float)k0 + 0.25f * (float)(f0 >> 1);
+        l1 = (float)k1 + 0.25f * (float)(f1 >> 1);
+        l2 = (float)k2 + 0.25f * (float)(f2 >> 1);
+        span = 0.25f;
+    } else {
+        if (!((byte >> ((f0 & 1) + 2 * (f1 & 1) + 4 * (f2 & 1))) & 1u)) return r.k;
+        l0 = (float)k0 + 0.125f * (float)f0;
+        l1 = (float)k1 + 0.125f * (float)f1;
+        l2 = (float)k2 + 0.125f * (float)f2;
+        span = 0.125f;
+    }
     float texit = INFINITY;
     texit = pm_box_exit(l0 * g.h, (l0 + span) * g.h, o0, r.d0, r.id0, texit);
     texit = pm_box_exit(l1 * g.h, (l1 + span) * g.h, o1, r.d1, r.id1, texit);

@@ -33,7 +33,7 @@ class PmSettings(C.Structure):
         ("learned_min_skip", C.c_int32), ("learned_max_skip", C.c_int32),
         ("anisotropic", C.c_int32), ("k_n", C.c_float), ("k_r", C.c_float), ("k_s", C.c_float),
         ("n_eps", C.c_int32), ("ext_mult", C.c_float), ("instrument", C.c_int32), ("learned_gate", C.c_float),
-        ("reserved", C.c_int32 * 1),
+        ("bound_boxes", C.c_int32),
     ]
 
 
@@ -121,7 +121,7 @@ def load():
     L.pm_trace_append.argtypes = [vp, i32, C.POINTER(PmSettings), vp, vp, u32, u32, u32, u32, u32, u64, C.POINTER(u64)]
     L.pm_dataset_get.argtypes = [vp, u64, u64, vp, vp, vp, vp]
     L.pm_mlp_train.argtypes = [vp, C.c_int, u32, f32, f32, u64, vp]
-    L.pm_get_bound_grid.argtypes = [vp, i32, f32, vp]
+    L.pm_get_bound_grid.argtypes = [vp, i32, f32, i32, vp]
     L.pm_get_march_grid.argtypes = [vp, i32, vp, vp]
     L.pm_batch_set.argtypes = [vp, C.POINTER(PmSettings), vp, u32, C.c_int]
     L.pm_batch_run.argtypes = [vp]
@@ -328,10 +328,12 @@ class Marcher:
         self._check(self.L.pm_get_march_grid(self.ctx, frame, _ptr(cs), _ptr(srt)), "pm_get_march_grid")
         return fi, cs, srt
 
-    def bound_grid(self, frame: int, iso: float) -> np.ndarray:
+    def bound_grid(self, frame: int, iso: float, boxes: int = 0) -> np.ndarray:
+        """The frame's sub-box bound bits for iso and bound_boxes: (ncells, 8) words
+        of 512 bits per cell."""
         fi = self.frame_info(frame)
-        bits = np.zeros(fi.ncells, np.uint64)
-        self._check(self.L.pm_get_bound_grid(self.ctx, frame, float(iso), _ptr(bits)), "pm_get_bound_grid")
+        bits = np.zeros((fi.ncells, 8), np.uint64)
+        self._check(self.L.pm_get_bound_grid(self.ctx, frame, float(iso), int(boxes), _ptr(bits)), "pm_get_bound_grid")
         return bits
 
     def batch_set(self, settings: dict, views, use_graph: bool = True):

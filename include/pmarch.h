@@ -50,9 +50,10 @@ extern "C" {
 #define PM_SKIP_EMPTY      1  /* configured mode: skip samples whose 27 cells are empty */
 #define PM_SKIP_DG_BOX     2  /* reference mode: DensityGrid, box flag                 */
 #define PM_SKIP_DG_CONS    3  /* reference mode: DensityGrid, conservative flag        */
-#define PM_SKIP_BOUND      4  /* configured mode: skip 4x4x4 sub-boxes of a cell whose
+#define PM_SKIP_BOUND      4  /* configured mode: skip 8x8x8 sub-boxes of a cell whose
                                  density bound (sum of W at each particle's distance to
-                                 the sub-box) stays below iso; the image is that of the
+                                 the sub-box) stays below iso, a 4x4x4 box at once when
+                                 all eight of its sub-boxes do; the image is that of the
                                  march without the skip.  With the anisotropic density:
                                  sub-boxes with no particle within h (the bound for
                                  iso PM_ZERO_ISO), where that density is exactly 0     */
@@ -105,7 +106,10 @@ typedef struct pm_settings {
                               cand27; same images, slower)                          */
     float   learned_gate;  /* learned: a sample asks for a proposal only when its
                               density is below learned_gate x iso (0 = 0.25)       */
-    int32_t reserved[1];
+    int32_t bound_boxes;   /* PM_SKIP_BOUND: 0 = sub-boxes of 8x8x8 per cell (the
+                              default; fewer march samples), 1 = 4x4x4 only (a build
+                              about 5x cheaper, for a frame marched once, as in
+                              streaming); the images are the same                   */
 } pm_settings;
 
 /* Per-launch counters, written by the march kernels themselves. */
@@ -195,10 +199,13 @@ int pm_get_march_grid(pm_ctx *ctx, int32_t frame, uint32_t *cell_start, float *s
  * to read dg_dims. */
 int pm_get_skip_grids(pm_ctx *ctx, int32_t frame, uint8_t *empty27, int32_t dg_dims[3], uint8_t *flag_box,
                       uint8_t *flag_cons, uint32_t *count_box);
-/* The frame's sub-box bound bits for iso (PM_SKIP_BOUND): bits[ncells], bit
- * l = sx + 4 sy + 16 sz of a cell set when its sub-box stays below iso.  Builds
- * them if the frame has none for this iso. */
-int pm_get_bound_grid(pm_ctx *ctx, int32_t frame, float iso, uint64_t *bits);
+/* The frame's sub-box bound bits for iso and bound_boxes `boxes` (PM_SKIP_BOUND):
+ * bits[8 ncells], 512 bits per cell.  Sub-box (fx, fy, fz), f in 0..7 per axis,
+ * is bit 8 l + (fx & 1) + 2 (fy & 1) + 4 (fz & 1) of the cell's words with
+ * l = fx/2 + 4 (fy/2) + 16 (fz/2) (bit b in word b / 64), set when the sub-box
+ * stays below iso (boxes = 1: each byte 0xff or 0, the 4x4x4 boxes' bounds).
+ * Builds them if the frame has none for this iso and boxes. */
+int pm_get_bound_grid(pm_ctx *ctx, int32_t frame, float iso, int32_t boxes, uint64_t *bits);
 
 /* RayMarcher::Prepare + Start (RayMarcher.cpp:76-112).  inv_pv is
  * Camera3D::GetInvProjectionView (column-major 16 floats), cam is

@@ -42,6 +42,7 @@ class Settings(C.Structure):
         ("k_s", C.c_float),
         ("n_eps", C.c_int32),
         ("ext_mult", C.c_float),
+        ("bound_boxes", C.c_int32),
     ]
 
 
@@ -79,6 +80,7 @@ class GridStruct(C.Structure):
         ("empty27", C.POINTER(C.c_uint8)),
         ("bound", C.POINTER(C.c_uint64)),
         ("bound_iso", C.c_float),
+        ("bound_boxes", C.c_int32),
         ("inv_hf", C.c_float),
         ("forigin", C.c_int32 * 3),
         ("fdims", C.c_int32 * 3),
@@ -116,7 +118,7 @@ def lib():
         L.or_grid_struct_size.restype = C.c_uint64
         L.or_grid_build.argtypes = [fp(np.float32, flags="C"), C.c_uint64, C.c_float, C.POINTER(GridStruct)]
         L.or_grid_free.argtypes = [C.POINTER(GridStruct)]
-        L.or_bound_build.argtypes = [C.POINTER(GridStruct), C.c_float]
+        L.or_bound_build.argtypes = [C.POINTER(GridStruct), C.c_float, C.c_int32]
         L.or_neighbors_fine.restype = C.c_uint32
         L.or_neighbors_fine.argtypes = [C.POINTER(GridStruct), fp(np.float32, flags="C"), fp(np.float32, flags="C"),
                                         C.c_uint32, C.POINTER(C.c_uint32)]
@@ -260,13 +262,14 @@ class Grid:
                 np.ctypeslib.as_array(self.g.dg_flag_box, shape=(n,)).copy(),
                 np.ctypeslib.as_array(self.g.dg_flag_cons, shape=(n,)).copy())
 
-    def bound(self, iso: float) -> np.ndarray:
-        """Sub-box bound bits for iso (builds them when the grid holds none for iso)."""
+    def bound(self, iso: float, boxes: int = 0) -> np.ndarray:
+        """Sub-box bound bits for iso and bound_boxes, (ncells, 8) words: 512 bits
+        per cell (builds them when the grid holds none for iso and boxes)."""
         iso = float(np.float32(iso))
-        if not self.g.bound or float(np.float32(self.g.bound_iso)) != iso:
-            if lib().or_bound_build(C.byref(self.g), iso) != 0:
+        if not self.g.bound or float(np.float32(self.g.bound_iso)) != iso or self.g.bound_boxes != boxes:
+            if lib().or_bound_build(C.byref(self.g), iso, int(boxes)) != 0:
                 raise RuntimeError("or_bound_build failed")
-        return np.ctypeslib.as_array(self.g.bound, shape=(self.g.ncells,)).copy()
+        return np.ctypeslib.as_array(self.g.bound, shape=(self.g.ncells * 8,)).reshape(-1, 8).copy()
 
     def empty27(self) -> np.ndarray:
         return np.ctypeslib.as_array(self.g.empty27, shape=(self.g.ncells,)).copy()
@@ -458,7 +461,7 @@ def make_settings(**kw) -> Settings:
     s = Settings()
     defaults = dict(mode=0, max_steps=0, step=0.0, iso=0.0, bisect_iters=8, final_fp64=1,
                     normal_eps=0.0, normal_mode=0, skip_mode=0, compensated=1,
-                    anisotropic=0, k_n=0.5, k_r=2.0, k_s=2000.0, n_eps=1, ext_mult=2.0)
+                    anisotropic=0, k_n=0.5, k_r=2.0, k_s=2000.0, n_eps=1, ext_mult=2.0, bound_boxes=0)
     defaults.update(kw)
     for k, v in defaults.items():
         setattr(s, k, v)
@@ -481,7 +484,7 @@ def march(grid: Grid, settings: Settings, inv_pv, cam, W: int, H: int, depth=Non
         dptr = depth.ctypes.data
     if settings.skip_mode == 4:
         # the anisotropic density skips the sub-boxes with no particle within h
-        grid.bound(ZERO_ISO if settings.anisotropic else settings.iso)
+        grid.bound(ZERO_ISO if settings.anisotropic else settings.iso, settings.bound_boxes)
     cnt = Counters()
     rc = lib().or_march(C.byref(grid.g), C.byref(settings), inv_pv, cam, W, H, dptr, row_step, threads,
                         pos.reshape(-1), nrm.reshape(-1), t.ctypes.data if want_t else None, C.byref(cnt))

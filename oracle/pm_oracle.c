@@ -20,6 +20,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <unistd.h>
 #include <stdatomic.h>
 #include <stdlib.h>
 #include <string.h>
@@ -691,93 +692,181 @@ static int64_t skip_empty(const or_grid *g, const float o[3], const float d[3], 
 }
 
 /* Sub-box density bound (configured-mode skip, no counterpart in the
- * reference).  Cell c is split into 4x4x4 sub-boxes; sub-box (sx,sy,sz) holds
- * the points whose u = inv_h * x lies in [key + s/4, key + (s+1)/4) on every
- * axis.  With B that box widened by delta = h (2^-10 + (max|key| + 1) 2^-18),
+ * reference).  Cell c is split into 8x8x8 sub-boxes; sub-box (fx,fy,fz) holds
+ * the points whose u = inv_h * x lies in [key + f/8, key + (f+1)/8) on every
+ * axis, and quarter box s = f/2 (4x4x4 per cell) the eight with the same s.
+ * With B a box widened by delta = h (2^-10 + (max|key| + 1) 2^-18),
  * U = sum over the cell's 27-cell stencil of W(dist(x_j, B)) bounds the density
  * of every point inside (W falls with r; the stencil holds every particle
- * within h).  The bit is set when U < iso (1 - 2^-9).  Border cells and cells
- * with an empty 27-neighbourhood get every bit.  U sums the centre row
+ * within h).  A sub-box's bit is set when U of its quarter box, or else its own
+ * U, is below iso (1 - 2^-9) -- except in cells whose centre row (dz = dy = 0)
+ * alone takes every quarter box's U past that limit: those lie inside the
+ * fluid and get no bit (a clear bit only means the sample is evaluated).  A
+ * cell holds 8 words: sub-box (fx,fy,fz) is bit
+ * 8 l + (fx&1) + 2 (fy&1) + 4 (fz&1) (bit b of word b / 64) with
+ * l = fx/2 + 4 (fy/2) + 16 (fz/2) its quarter box.  Border cells and cells with
+ * an empty 27-neighbourhood get every bit.  Each U sums the centre row
  * (dz = dy = 0) first, then the other eight rows in (dz, dy) order, particles
- * in storage order, as the GPU build does (it stops a cell's walk once every
- * sub-box is past the limit; adding terms >= 0 never lowers U again). */
-int or_bound_build(or_grid *g, float iso)
+ * in storage order, as the GPU build does (it stops a walk once every box it
+ * sums is past the limit; adding terms >= 0 never lowers U again).  Cells are
+ * split over threads by z-slab; each cell is computed by one thread, so the
+ * bits do not depend on the thread count. */
+typedef struct {
+    or_grid *g;
+    float lim;
+    int quarter_only;
+    int32_t next_z;
+    pthread_mutex_t lock;
+} bound_job;
+
+/* U of the box lo..hi (already widened) over the first nrows rows of cell
+ * (x,y,z)'s stencil, its walk stopped at the first row end with U >= lim. */
+static float bound_sum(const or_grid *g, int32_t x, int32_t y, int32_t z, const float lo[3], const float hi[3],
+                       float lim, int nrows)
 {
+    static const int8_t order[9][2] = { { 0, 0 }, { -1, -1 }, { -1, 0 }, { -1, 1 }, { 0, -1 },
+                                        { 0, 1 }, { 1, -1 }, { 1, 0 }, { 1, 1 } };
+    const float h = g->h, h2 = h * h, sig = or_sigma_f(h);
+    float U = 0.0f;
+    for (int r = 0; r < nrows && U < lim; r++) {
+        const int dz = order[r][0], dy = order[r][1];
+        uint64_t row = cell_lin(g, x - 1, y + dy, z + dz);
+        for (uint32_t j = g->cell_start[row]; j < g->cell_start[row + 3]; j++) {
+            const float *p = g->sorted + 4 * (uint64_t)j;
+            float e[3];
+            for (int a = 0; a < 3; a++) {
+                float v = lo[a] - p[a], w = p[a] - hi[a];
+                float m = v > w ? v : w;
+                e[a] = m > 0.0f ? m : 0.0f;
+            }
+            float d2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+            U += W_f(d2, h2, g->inv_h, sig);
+        }
+    }
+    return U;
+}
+
+static void bound_cell_bits(or_grid *g, float lim, int quarter_only, int32_t x, int32_t y, int32_t z)
+{
+    const uint64_t c = cell_lin(g, x, y, z);
+    uint64_t *out = g->bound + 8 * c;
+    if (x < 1 || y < 1 || z < 1 || x > g->dims[0] - 2 || y > g->dims[1] - 2 || z > g->dims[2] - 2 || g->empty27[c]) {
+        for (int w = 0; w < 8; w++) out[w] = ~0ull;
+        return;
+    }
+    const float h = g->h;
+    int32_t k[3] = { x + g->origin[0], y + g->origin[1], z + g->origin[2] };
+    int32_t km = 0;
+    for (int a = 0; a < 3; a++) km = abs(k[a]) > km ? abs(k[a]) : km;
+    const float delta = h * (0x1p-10f + (float)(km + 1) * 0x1p-18f);
+    for (int w = 0; w < 8; w++) out[w] = 0;
+    int inside = 1;
+    for (int l = 0; l < 64 && inside; l++) {
+        const int sq[3] = { l & 3, (l >> 2) & 3, l >> 4 };
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = ((float)k[a] + 0.25f * (float)sq[a]) * h - delta;
+            hi[a] = ((float)k[a] + 0.25f * (float)(sq[a] + 1)) * h + delta;
+        }
+        if (bound_sum(g, x, y, z, lo, hi, lim, 1) < lim) inside = 0;
+    }
+    if (inside) return;                          /* the centre row alone: every quarter box past the limit */
+    for (int l = 0; l < 64; l++) {
+        const int sq[3] = { l & 3, (l >> 2) & 3, l >> 4 };
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = ((float)k[a] + 0.25f * (float)sq[a]) * h - delta;
+            hi[a] = ((float)k[a] + 0.25f * (float)(sq[a] + 1)) * h + delta;
+        }
+        unsigned byte = 0;
+        if (bound_sum(g, x, y, z, lo, hi, lim, 9) < lim) {
+            byte = 0xffu;                       /* the quarter box's bound covers its eight */
+        } else if (!quarter_only) {
+            for (int f = 0; f < 8; f++) {
+                const int fi[3] = { 2 * sq[0] + (f & 1), 2 * sq[1] + ((f >> 1) & 1), 2 * sq[2] + (f >> 2) };
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = ((float)k[a] + 0.125f * (float)fi[a]) * h - delta;
+                    hi[a] = ((float)k[a] + 0.125f * (float)(fi[a] + 1)) * h + delta;
+                }
+                if (bound_sum(g, x, y, z, lo, hi, lim, 9) < lim) byte |= 1u << f;
+            }
+        }
+        out[l >> 3] |= (uint64_t)byte << (8 * (l & 7));
+    }
+}
+
+static void *bound_worker(void *arg)
+{
+    bound_job *j = (bound_job *)arg;
+    for (;;) {
+        pthread_mutex_lock(&j->lock);
+        int32_t z = j->next_z++;
+        pthread_mutex_unlock(&j->lock);
+        if (z >= j->g->dims[2]) return NULL;
+        for (int32_t y = 0; y < j->g->dims[1]; y++)
+            for (int32_t x = 0; x < j->g->dims[0]; x++) bound_cell_bits(j->g, j->lim, j->quarter_only, x, y, z);
+    }
+}
+
+int or_bound_build(or_grid *g, float iso, int32_t boxes)
+{
+    if (boxes != 0 && boxes != 1) return -1;
     free(g->bound);
-    g->bound = (uint64_t *)calloc(g->ncells ? g->ncells : 1, sizeof(uint64_t));
+    g->bound = (uint64_t *)calloc((g->ncells ? g->ncells : 1) * 8, sizeof(uint64_t));
     if (!g->bound) return -1;
     g->bound_iso = iso;
-    const float lim = iso * (1.0f - 0x1p-9f), h = g->h, h2 = h * h, sig = or_sigma_f(h);
-    for (int32_t z = 0; z < g->dims[2]; z++)
-        for (int32_t y = 0; y < g->dims[1]; y++)
-            for (int32_t x = 0; x < g->dims[0]; x++) {
-                uint64_t c = cell_lin(g, x, y, z);
-                if (x < 1 || y < 1 || z < 1 || x > g->dims[0] - 2 || y > g->dims[1] - 2 || z > g->dims[2] - 2 ||
-                    g->empty27[c]) {
-                    g->bound[c] = ~0ull;
-                    continue;
-                }
-                int32_t k[3] = { x + g->origin[0], y + g->origin[1], z + g->origin[2] };
-                int32_t km = 0;
-                for (int a = 0; a < 3; a++) km = abs(k[a]) > km ? abs(k[a]) : km;
-                float delta = h * (0x1p-10f + (float)(km + 1) * 0x1p-18f);
-                uint64_t bits = 0;
-                for (int l = 0; l < 64; l++) {
-                    float sub[3] = { (float)(l & 3), (float)((l >> 2) & 3), (float)(l >> 4) }, lo[3], hi[3];
-                    for (int a = 0; a < 3; a++) {
-                        lo[a] = ((float)k[a] + 0.25f * sub[a]) * h - delta;
-                        hi[a] = ((float)k[a] + 0.25f * (sub[a] + 1.0f)) * h + delta;
-                    }
-                    static const int8_t order[9][2] = { { 0, 0 }, { -1, -1 }, { -1, 0 }, { -1, 1 }, { 0, -1 },
-                                                        { 0, 1 }, { 1, -1 }, { 1, 0 }, { 1, 1 } };
-                    float U = 0.0f;
-                    for (int r = 0; r < 9 && U < lim; r++) {
-                            const int dz = order[r][0], dy = order[r][1];
-                            uint64_t row = cell_lin(g, x - 1, y + dy, z + dz);
-                            for (uint32_t j = g->cell_start[row]; j < g->cell_start[row + 3]; j++) {
-                                const float *p = g->sorted + 4 * (uint64_t)j;
-                                float e[3];
-                                for (int a = 0; a < 3; a++) {
-                                    float v = lo[a] - p[a], w = p[a] - hi[a];
-                                    float m = v > w ? v : w;
-                                    e[a] = m > 0.0f ? m : 0.0f;
-                                }
-                                float d2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
-                                U += W_f(d2, h2, g->inv_h, sig);
-                            }
-                        }
-                    if (U < lim) bits |= 1ull << l;
-                }
-                g->bound[c] = bits;
-            }
+    g->bound_boxes = boxes;
+    bound_job j;
+    j.g = g;
+    j.lim = iso * (1.0f - 0x1p-9f);
+    j.quarter_only = boxes == 1;
+    j.next_z = 0;
+    pthread_mutex_init(&j.lock, NULL);
+    long nc = sysconf(_SC_NPROCESSORS_ONLN);
+    int nthreads = nc > 1 ? (int)(nc - 1 < 16 ? nc - 1 : 16) : 1;
+    pthread_t th[16];
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, bound_worker, &j);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&j.lock);
     return 0;
 }
 
-/* Bound skip of the configured march: the sample's sub-box has its bit set, so
- * the march resumes at the first sample past the sub-box exit (the exit index
- * lowered by 2^-8 + |k| 2^-18 steps against rounding; slab distances use the
- * reciprocal direction and step, as the GPU computes them).  Returns the next sample
- * index (k when there is nothing to skip). */
+/* Bound skip of the configured march: the sample's sub-box (eighth box) has
+ * its bit set, so the march resumes at the first sample past the box exit --
+ * past the exit of its quarter box when all eight eighth boxes of that are set
+ * (the exit index lowered by 2^-8 + |k| 2^-18 steps against rounding; slab
+ * distances use the reciprocal direction and step, as the GPU computes them).
+ * Returns the next sample index (k when there is nothing to skip). */
 static int64_t skip_bound(const or_grid *g, const float o[3], const float d[3], const float x[3], float tn,
                           float step, int64_t k)
 {
-    int32_t key[3], c[3], sub[3];
+    int32_t key[3], c[3], fi[3];
     for (int a = 0; a < 3; a++) {
         float u = g->inv_h * x[a];
         key[a] = x[a] >= 0.0f ? (int32_t)u : (int32_t)u - 1;
         c[a] = key[a] - g->origin[a];
         if (c[a] < 0 || c[a] >= g->dims[a]) return k;
-        int s4 = (int)((u - (float)key[a]) * 4.0f);
-        sub[a] = s4 < 3 ? s4 : 3;
+        int s8 = (int)((u - (float)key[a]) * 8.0f);
+        fi[a] = s8 < 7 ? s8 : 7;
     }
-    uint64_t m = g->bound[cell_lin(g, c[0], c[1], c[2])];
-    if (!((m >> (sub[0] + 4 * sub[1] + 16 * sub[2])) & 1ull)) return k;
+    const int l = (fi[0] >> 1) + 4 * (fi[1] >> 1) + 16 * (fi[2] >> 1);
+    const uint64_t *cb = g->bound + 8 * cell_lin(g, c[0], c[1], c[2]);
+    const unsigned byte = (unsigned)(cb[l >> 3] >> (8 * (l & 7))) & 0xffu;
+    float lo[3], span;
+    if (byte == 0xffu) {
+        for (int a = 0; a < 3; a++) lo[a] = (float)key[a] + 0.25f * (float)(fi[a] >> 1);
+        span = 0.25f;
+    } else {
+        if (!((byte >> ((fi[0] & 1) + 2 * (fi[1] & 1) + 4 * (fi[2] & 1))) & 1u)) return k;
+        for (int a = 0; a < 3; a++) lo[a] = (float)key[a] + 0.125f * (float)fi[a];
+        span = 0.125f;
+    }
     float texit = INFINITY;
     for (int a = 0; a < 3; a++) {
         if (d[a] == 0.0f) continue;
-        float lo = ((float)key[a] + 0.25f * (float)sub[a]) * g->h, hi = ((float)key[a] + 0.25f * (float)(sub[a] + 1)) * g->h;
+        float blo = lo[a] * g->h, bhi = (lo[a] + span) * g->h;
         float id = 1.0f / d[a];
-        float ta = ((d[a] > 0.0f ? hi : lo) - o[a]) * id;
+        float ta = ((d[a] > 0.0f ? bhi : blo) - o[a]) * id;
         if (ta < texit) texit = ta;
     }
     float kq = (texit - tn) * (1.0f / step);
@@ -1439,7 +1528,7 @@ static int run_march(const or_grid *g, const or_hash *hs, const or_settings *s, 
     /* or_bound_build first: for iso, or (anisotropic density) for OR_ZERO_ISO --
      * sub-boxes with no particle within h, where that density is exactly 0 */
     const float biso = s->anisotropic ? OR_ZERO_ISO : s->iso;
-    if (s->skip_mode == 4 && (!g->bound || g->bound_iso != biso)) return -2;
+    if (s->skip_mode == 4 && (!g->bound || g->bound_iso != biso || g->bound_boxes != s->bound_boxes)) return -2;
     if (s->anisotropic && !(s->ext_mult >= 1.0f && s->ext_mult <= 2.0f)) return -1;
     if (row_step == 0) row_step = 1;
     march_job j;

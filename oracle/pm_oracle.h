@@ -44,6 +44,7 @@ typedef struct or_settings {
     float   k_n, k_r, k_s; /* VisualizationSettings k_n, k_r, k_s (RayMarcher.h:22-24) */
     int32_t n_eps;         /* VisualizationSettings N_eps                          */
     float   ext_mult;      /* ParticleRadiusExt / ParticleRadius (>= 1)            */
+    int32_t bound_boxes;   /* skip_mode 4: 0 = 8x8x8 sub-boxes, 1 = 4x4x4 only     */
 } or_settings;
 
 enum { OR_MODE_CONFIG = 0, OR_MODE_REFERENCE = 1 };
@@ -88,10 +89,13 @@ typedef struct or_grid {
     /* 1 where the cell and its 26 neighbours hold no particle: every point of
      * the cell has zero density (empty-space skip of the configured march)   */
     uint8_t  *empty27;
-    /* sub-box density bounds for one iso (or_bound_build): bit l = sx + 4 sy +
-     * 16 sz of bound[c] set when sub-box l of cell c stays below iso         */
+    /* sub-box density bounds for one iso (or_bound_build): 8 words per cell,
+     * bit 8 l + (fx&1) + 2 (fy&1) + 4 (fz&1) set when sub-box (fx,fy,fz) of the
+     * cell's 8x8x8 stays below iso, l = fx/2 + 4 (fy/2) + 16 (fz/2); with
+     * bound_boxes 1 every byte is 0xff or 0 (the 4x4x4 boxes' bounds alone)   */
     uint64_t *bound;
     float     bound_iso;
+    int32_t   bound_boxes;
     /* march grid: cells of h/2 (the same key rule with inv_hf = 2/h), four
      * empty cells of margin on each side, particles fine-cell-major (x
      * fastest) and by input index inside a cell.  Every density sum walks the
@@ -118,7 +122,7 @@ void or_grid_free(or_grid *g);
 uint64_t or_grid_struct_size(void);
 
 /* Sub-box density bounds for iso (skip_mode 4); replaces any earlier bits. */
-int  or_bound_build(or_grid *g, float iso);
+int  or_bound_build(or_grid *g, float iso, int32_t boxes);
 
 /* ---- neighbour search ---- */
 uint32_t or_neighbors_brute(const float *xyz, uint64_t n, float h, const float q[3],

@@ -16,31 +16,38 @@ def _oracle_settings(sc, **kw):
     return O.make_settings(**{k: v for k, v in sc.settings(**kw).items() if k in KEYS})
 
 
+@pytest.mark.parametrize("boxes", [0, 1])
 @pytest.mark.parametrize("name", ["lattice", "dam_small", "clustered"])
-def test_bound_bits_match_oracle(marcher, name):
+def test_bound_bits_match_oracle(marcher, name, boxes):
     sc = {"lattice": lambda: scenes.lattice_cube(2),
           "dam_small": lambda: scenes.dam_break(4, nx=30, ny=60, nz=30),
           "clustered": lambda: scenes.clustered_pool(3, n_total=30000, r=0.008, width=96, height=64)}[name]()
-    print(f"bound bits {name} n={sc.n}", flush=True)
+    print(f"bound bits {name} n={sc.n} boxes={boxes}", flush=True)
     marcher.set_frame(40, sc.xyz, sc.h)
-    gpu = marcher.bound_grid(40, sc.iso)
-    ref = O.Grid(sc.xyz, sc.h).bound(sc.iso)
+    gpu = marcher.bound_grid(40, sc.iso, boxes)
+    ref = O.Grid(sc.xyz, sc.h).bound(sc.iso, boxes)
     assert gpu.shape == ref.shape
-    assert np.array_equal(gpu, ref), f"{np.sum(gpu != ref)} cells differ"
+    assert np.array_equal(gpu, ref), f"{np.sum(gpu != ref)} words differ"
+    if boxes == 1:
+        b = gpu.view(np.uint8)
+        assert np.all((b == 0) | (b == 0xFF))
 
 
+@pytest.mark.parametrize("boxes", [0, 1])
 @pytest.mark.parametrize("name", ["lattice", "clustered"])
-def test_bound_march_matches_oracle(marcher, name):
+def test_bound_march_matches_oracle(marcher, name, boxes):
     sc = {"lattice": lambda: scenes.lattice_cube(1, width=200, height=150),
           "clustered": lambda: scenes.clustered_pool(2, n_total=40000, r=0.008, width=200, height=150)}[name]()
     c = sc.cameras[0]
-    print(f"bound march {name} n={sc.n} {c.width}x{c.height}", flush=True)
+    print(f"bound march {name} n={sc.n} {c.width}x{c.height} boxes={boxes}", flush=True)
     marcher.set_frame(41, sc.xyz, sc.h)
-    pos, nrm, t, st = marcher.march(41, sc.settings(skip_mode=4), c.inv_pv, c.eye, c.width, c.height)
+    pos, nrm, t, st = marcher.march(41, sc.settings(skip_mode=4, bound_boxes=boxes), c.inv_pv, c.eye, c.width,
+                                    c.height)
     G = O.Grid(sc.xyz, sc.h)
     p0, n0, t0, c0 = O.march(G, _oracle_settings(sc, skip_mode=0), c.inv_pv, c.eye, c.width, c.height)
     assert np.array_equal(pos, p0) and np.array_equal(nrm, n0) and np.array_equal(t, t0)
-    p4, n4, t4, c4 = O.march(G, _oracle_settings(sc, skip_mode=4), c.inv_pv, c.eye, c.width, c.height)
+    p4, n4, t4, c4 = O.march(G, _oracle_settings(sc, skip_mode=4, bound_boxes=boxes), c.inv_pv, c.eye, c.width,
+                             c.height)
     for k in ("rays", "rays_in_aabb", "hits", "samples", "march_cand", "march_samples"):
         assert st[k] == c4[k], (k, st[k], c4[k])
     print("march samples", c0["march_samples"], "->", st["march_samples"], flush=True)

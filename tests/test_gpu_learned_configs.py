@@ -41,24 +41,32 @@ def network(marcher):
     return params
 
 
+@pytest.mark.parametrize("boxes", [0, 1])
 @pytest.mark.parametrize("name", ["dam_break_1080p", "clustered_pool_4k"])
-def test_learned_configured_scenes(marcher, network, name):
+def test_learned_configured_scenes(marcher, network, name, boxes):
+    """With the 4x4x4 bound (boxes = 1) the network proposes and certified jumps
+    are taken; with 8x8x8 (boxes = 0) the bound itself passes almost every
+    low-density sample, so few proposals remain.  The image is the plain
+    march's either way."""
     if name == "dam_break_1080p":
         sc, step = scenes.dam_break(0), 27
     else:
         sc, step = scenes.clustered_pool(0), 72
     c = sc.cameras[0]
-    print(f"learned {name} n={sc.n} {c.width}x{c.height}", flush=True)
+    print(f"learned {name} n={sc.n} {c.width}x{c.height} bound_boxes={boxes}", flush=True)
     marcher.set_frame(70, sc.xyz, sc.h)
     marcher.mlp_set(network)
-    plain = marcher.march(70, sc.settings(skip_mode=4), c.inv_pv, c.eye, c.width, c.height)
-    lrn = marcher.march(70, sc.settings(skip_mode=4, learned=1), c.inv_pv, c.eye, c.width, c.height)
+    plain = marcher.march(70, sc.settings(skip_mode=4, bound_boxes=boxes), c.inv_pv, c.eye, c.width, c.height)
+    lrn = marcher.march(70, sc.settings(skip_mode=4, bound_boxes=boxes, learned=1), c.inv_pv, c.eye, c.width,
+                        c.height)
     sp, sl = plain[3], lrn[3]
     print("plain", sp, "\nlearned", sl, flush=True)
     for a, b in zip(plain[:3], lrn[:3]):
         assert np.array_equal(a, b)
-    assert sl["mlp_evals"] > 1000 and sl["cert_skipped"] > 1000
-    assert sl["march_samples"] < sp["march_samples"]
+    assert sl["march_samples"] <= sp["march_samples"]
+    if boxes == 1:
+        assert sl["mlp_evals"] > 1000 and sl["cert_skipped"] > 1000
+        assert sl["march_samples"] < sp["march_samples"]
     op, on, ot, _ = O.march(O.Grid(sc.xyz, sc.h), _oracle_settings(sc), c.inv_pv, c.eye, c.width, c.height,
                             row_step=step)
     _rows_equal(lrn[:3], (op, on, ot), step)
