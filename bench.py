@@ -287,7 +287,7 @@ def bench_aniso(args):
                    "step": scene.step, "iso": 1.0, "path": "anisotropic"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for("anisotropic", args.workload),
-                     "limiter": "VALU issue of the per-lane WPCA and density walks (lane utilisation 43 %); "
+                     "limiter": "VALU issue of the per-lane WPCA and density walks (lane utilisation 42 %); "
                                 "achieved/frac are the walks' algorithmic bytes, read mostly from L1/L2"},
         "kernel_ms": round(kms, 4),
         "alg_bytes_per_launch": B,
@@ -850,8 +850,8 @@ def main():
                      "frac_of_copy": round(B / kt / 1e9 / copy, 4),
                      "hbm_gbps": round(traffic / kt / 1e9, 2) if traffic else None,
                      "hbm_frac": round(traffic / kt / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-                     "limiter": "VALU issue of divergent per-lane stencil walks (SIMD VALU busy ~75 %, lane "
-                                "utilisation 44 %); achieved/frac are the algorithmic bytes of the HBM "
+                     "limiter": "VALU issue of divergent per-lane stencil walks (SIMD VALU busy ~79 %, lane "
+                                "utilisation 47 %); achieved/frac are the algorithmic bytes of the HBM "
                                 "roofline, read mostly from L1/L2 -- the HBM traffic itself is hbm_gbps"},
         "kernel_ms": round(f["kernel_ms"], 4),
         "alg_bytes_per_launch": B,

@@ -369,9 +369,9 @@ int pm_mlp_train(pm_ctx *ctx, int epochs, uint32_t batch, float lr, float w_over
  * pre-pass's lists, counters -- in one static device array per GPU
  * (pm_noalloc_scratch_bytes(), about 34 MB, held until the process ends).
  * Limits: W * H <= PM_NOALLOC_MAX_PIXELS and W, H <= 16384 (PM_ERR_SIZE); the
- * frame must be set and, with PM_SKIP_BOUND, its bound bits for s->iso built
- * (one pm_march_async_device or pm_get_bound_grid does it; PM_ERR_STATE
- * otherwise); the images are device memory of the context's GPU.  Nothing is
+ * frame must be set and, with PM_SKIP_BOUND, its bound bits for s->iso and
+ * s->bound_boxes built (one pm_march_async_device or pm_get_bound_grid does
+ * it; PM_ERR_STATE otherwise); the images are device memory of the context's GPU.  Nothing is
  * queued when a check fails.  Completes like pm_march_async_device (pm_query,
  * pm_wait, pm_get_stats).  Every call on one GPU shares that GPU's array:
  * marches that overlap on one GPU (two contexts, two threads) race. */
