@@ -61,8 +61,9 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
     const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
 
     if (r.phase == PH_MARCH) {
-        // at most 8 skips per step: a lane crossing much skippable space then
-        // idles for a step instead of holding the whole wave in this loop
+        // at most 16 skips per step: a lane crossing much skippable space then
+        // idles for a step instead of holding the whole wave in this loop (a
+        // skip costs a few dozen instructions, a step a density walk)
         for (int hop = 0;; hop++) {
             float t = r.tn + (float)r.k * s.step;
             if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) {
@@ -78,7 +79,7 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
             if (kn == r.k) break;
             r.k = kn;
             r.lrn = s.learned;
-            if (hop == 7) return false;
+            if (hop == 15) return false;
         }
     } else {  // PH_BISECT
         float tm = (r.lo + r.hi) * 0.5f;

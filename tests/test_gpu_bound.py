@@ -77,3 +77,44 @@ def test_bound_rejected_in_reference_mode(marcher):
     with pytest.raises(pmarch.PmError):
         marcher.march(43, sc.settings(mode=1, skip_mode=4, max_steps=100), c.inv_pv, c.eye, 32, 24,
                       depth=np.ones((24, 32), np.float32))
+
+
+def test_bound_boxes_invalid_rejected(marcher):
+    """bound_boxes other than 0 or 1 is refused before any launch, by the march
+    and by pm_get_bound_grid."""
+    import pmarch
+    sc = scenes.lattice_cube(0, n_side=8, width=32, height=24)
+    marcher.set_frame(44, sc.xyz, sc.h)
+    c = sc.cameras[0]
+    for bad in (2, -1):
+        with pytest.raises(pmarch.PmError):
+            marcher.march(44, sc.settings(skip_mode=4, bound_boxes=bad), c.inv_pv, c.eye, 32, 24)
+        with pytest.raises(pmarch.PmError):
+            marcher.bound_grid(44, sc.iso, bad)
+    # a valid march still runs afterwards
+    pos = marcher.march(44, sc.settings(skip_mode=4, bound_boxes=1), c.inv_pv, c.eye, 32, 24)[0]
+    assert pos.shape == (24, 32, 4)
+
+
+def test_bound_boxes_switch_rebuilds(marcher):
+    """The bits are keyed by iso and bound_boxes: marching one frame with 8x8x8,
+    then 4x4x4, then 8x8x8 boxes gives the oracle's counters for each, and the
+    same image every time."""
+    sc = scenes.clustered_pool(2, n_total=20000, r=0.008, width=96, height=64)
+    c = sc.cameras[0]
+    marcher.set_frame(45, sc.xyz, sc.h)
+    G = O.Grid(sc.xyz, sc.h)
+    ref = {}
+    for boxes in (0, 1):
+        ref[boxes] = O.march(G, _oracle_settings(sc, skip_mode=4, bound_boxes=boxes), c.inv_pv, c.eye, c.width,
+                             c.height)
+    first = None
+    for boxes in (0, 1, 0):
+        out = marcher.march(45, sc.settings(skip_mode=4, bound_boxes=boxes), c.inv_pv, c.eye, c.width, c.height)
+        for k in ("samples", "march_samples", "hits"):
+            assert out[3][k] == ref[boxes][3][k], (boxes, k)
+        if first is None:
+            first = out
+        for a, b in zip(first[:3], out[:3]):
+            assert np.array_equal(a, b)
+    assert ref[1][3]["march_samples"] > ref[0][3]["march_samples"]

@@ -3,8 +3,8 @@ callers that cannot allocate device memory: the images it writes into the
 caller's device memory, with the library's static scratch, equal
 pm_march_async_device's bit for bit, on two calls in a row (the second a
 smaller image, so the scratch holds the first call's records), for the plain
-and the learned march; a size beyond the scratch, and bound bits not built yet,
-are refused before anything is queued."""
+and the learned march; a size beyond the scratch, and bound bits not built yet
+(for the march's iso and bound_boxes), are refused before anything is queued."""
 import ctypes as C
 
 import numpy as np
@@ -91,6 +91,12 @@ def test_noalloc_refuses(marcher, hip):
         assert rc == PM_ERR_SIZE
         # the bound skip with bits that were never built for this iso
         s = settings_from(sc.settings(skip_mode=4, iso=sc.iso * 1.5))
+        rc = L.pm_march_device_noalloc(marcher.ctx, 61, C.byref(s), inv.ctypes.data, eye.ctypes.data, 64, 48,
+                                       d, d, None)
+        assert rc == PM_ERR_STATE
+        # bits built for this iso with 8x8x8 sub-boxes, asked for with 4x4x4 ones
+        marcher.bound_grid(61, sc.iso, 0)
+        s = settings_from(sc.settings(skip_mode=4, bound_boxes=1))
         rc = L.pm_march_device_noalloc(marcher.ctx, 61, C.byref(s), inv.ctypes.data, eye.ctypes.data, 64, 48,
                                        d, d, None)
         assert rc == PM_ERR_STATE
