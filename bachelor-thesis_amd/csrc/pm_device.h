@@ -214,15 +214,23 @@ __device__ __forceinline__ float pm_density32(const GridDev &g, float qx, float 
             uint32_t b, e;
             pm_fine_row(g, fw, dy, dz, tz2, b, e);
             cand += e - b;
-            for (uint32_t i = b; i < e; i++) {
-                float4 p = g.fsorted[i];
-                float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-                float r2 = (rx * rx + ry * ry) + rz * rz;
-                if (r2 < g.h2) {
-                    float w = pm_W_in(r2, g.inv_h, g.sig);
-                    if (compensated) pm_neu_add(s, c, w);
-                    else s += w;
-                    nsup++;
+            // groups of 4 loads, the last group's unused slots re-reading its first candidate
+            const int m = (int)(e - b);
+            const float4 *const p0 = g.fsorted + b;
+            for (int i = 0; i < m; i += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
+                    float r2 = (rx * rx + ry * ry) + rz * rz;
+                    if (i + u < m && r2 < g.h2) {
+                        float w = pm_W_in(r2, g.inv_h, g.sig);
+                        if (compensated) pm_neu_add(s, c, w);
+                        else s += w;
+                        nsup++;
+                    }
                 }
             }
         }

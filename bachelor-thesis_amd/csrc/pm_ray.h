@@ -66,11 +66,18 @@ __device__ __forceinline__ void pm_sum_gradw(const GridDev &g, float qx, float q
             uint32_t b, e;
             pm_fine_row(g, fw, dy, dz, pm_fine_tz2(fw, dz), b, e);
             cand += e - b;
-            for (uint32_t i = b; i < e; i++) {
-                float4 p = g.fsorted[i];
+            const int m = (int)(e - b);
+            const float4 *const p0 = g.fsorted + b;
+            for (int i = 0; i < m; i += 4) {
+              float4 v[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                const float4 p = v[u];
                 float r0 = p.x - qx, r1 = p.y - qy, r2c = p.z - qz;
                 float rn = (r0 * r0 + r1 * r1) + r2c * r2c;
-                if (rn < g.h2) {
+                if (i + u < m && rn < g.h2) {
                     float rl = sqrtf(rn);
                     float q = rl * g.inv_h;
                     float inv = 1.0f / sqrtf(rn);
@@ -88,6 +95,7 @@ __device__ __forceinline__ void pm_sum_gradw(const GridDev &g, float qx, float q
                     }
                     gx += w0; gy += w1; gz += w2;
                 }
+              }
             }
         }
 }
