@@ -256,11 +256,14 @@ hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz
 // sample positions), U = sum over the cell's 27-cell stencil of W(dist(x_j, B))
 // bounds the density at every point of B, since W falls with r and the
 // stencil holds every particle within h.  A sub-box's bit is set when U of its
-// quarter box, or else its own U, is below iso (1 - 2^-9); the margin covers
-// the rounding of both sums.  Cells whose centre row (dz = dy = 0) alone takes
-// every quarter box's U past the limit lie inside the fluid and get no bit (a
-// clear bit only means the sample is evaluated), which spares their eight
-// sub-box sums.  A cell's 512 bits are 64 bytes: byte
+// quarter box, or else its own U, is below the limit iso (1 - 2^-9); the margin
+// covers the rounding of both sums.  Two rules spare sub-box sums that would
+// find no bit: a cell whose centre row (dz = dy = 0) alone takes every quarter
+// box's U past the limit lies inside the fluid and gets no bit, and neither
+// does a quarter box whose U reaches 4 x the limit (a clear bit only means the
+// sample is evaluated) -- except for the zero-density bound (iso = PM_ZERO_ISO),
+// where a quarter box's U only says that some particle lies within h of it,
+// not near which of its sub-boxes.  A cell's 512 bits are 64 bytes: byte
 // l = sx + 4 sy + 16 sz belongs to quarter box s and its bit
 // (fx & 1) + 2 (fy & 1) + 4 (fz & 1) to the sub-box inside it, so a byte of
 // 0xff marks a quarter box the march can cross in one jump.  Each U sums the
@@ -269,8 +272,8 @@ hipError_t launch_empty27(const uint32_t *cs, int32_t nx, int32_t ny, int32_t nz
 // cells with an empty 27-neighbourhood get every bit.  One wave per cell: lane
 // l takes quarter box l, all lanes read the same particle; the quarter boxes'
 // sums come first (one W per particle), the eight sub-box sums only for the
-// quarter boxes left at or above the limit, and each walk stops once every box
-// it sums has reached the limit.
+// quarter boxes between the limit and 4 x the limit, and each walk stops once
+// every box it sums has reached the value that decides it.
 // Squared distance along one axis from x to [lo, hi] (0 inside).
 __device__ __forceinline__ float bound_e2(float lo, float hi, float x)
 {
@@ -344,7 +347,7 @@ __device__ __forceinline__ uint32_t bound_cell(const GridDev &g, float iso, bool
     km = (km > abs(k[2]) ? km : abs(k[2])) + 1;
     const float delta = g.h * (0x1p-10f + (float)km * 0x1p-18f);
     const int32_t sq[3] = { (int32_t)(lane & 3u), (int32_t)((lane >> 2) & 3u), (int32_t)(lane >> 4) };
-    const float lim = iso * (1.0f - 0x1p-9f);
+    const float lim = iso * (1.0f - 0x1p-9f), lim4 = iso == PM_ZERO_ISO ? INFINITY : 4.0f * lim;
     // the quarter box's bound; inside the fluid the centre row alone takes
     // every quarter box past the limit
     const float lx = ((float)k[0] + 0.25f * (float)sq[0]) * g.h - delta, hx = ((float)k[0] + 0.25f * (float)(sq[0] + 1)) * g.h + delta;
@@ -364,19 +367,18 @@ __device__ __forceinline__ uint32_t bound_cell(const GridDev &g, float iso, bool
             for (int u = 0; u < 8; u++) Uq += bound_term(g, p[u], lx, hx, ly, hy, lz, hz);
         }
         for (; j < e; j++) Uq += bound_term(g, g.sorted[j], lx, hx, ly, hy, lz, hz);
-        if (__ballot(Uq < lim) == 0) {
-            if (r == 0) return 0u;                  // inside the fluid
-            break;
-        }
+        if (r == 0 && __ballot(Uq < lim) == 0) return 0u;       // inside the fluid
+        if (__ballot(Uq < (quarter_only ? lim : lim4)) == 0) break;
     }
     const bool whole = Uq < lim;
     if (quarter_only) return whole ? 0xffu : 0u;
-    if (__ballot(!whole) == 0) return 0xffu;
+    const bool eighths = !whole && Uq < lim4;
+    if (__ballot(eighths) == 0) return whole ? 0xffu : 0u;
     // the eight sub-boxes of the quarter boxes left at or above the limit
     BoundEighths B;
     B.init(g, k, sq, delta);
     for (int r = 0; r < 9; r++) {
-        if (!whole) {
+        if (eighths) {
             const uint32_t row = bound_row(g, x, y, z, r);
             const uint32_t b = g.cell_start[row], e = g.cell_start[row + 3];
             uint32_t j = b;
@@ -389,9 +391,10 @@ __device__ __forceinline__ uint32_t bound_cell(const GridDev &g, float iso, bool
             }
             for (; j < e; j++) B.add(g, g.sorted[j]);
         }
-        if (__ballot(!whole && B.below(lim)) == 0) break;
+        if (__ballot(eighths && B.below(lim)) == 0) break;
     }
     if (whole) return 0xffu;
+    if (!eighths) return 0u;
     uint32_t m = 0;
 #pragma unroll
     for (int f = 0; f < 8; f++) m |= (B.U[f] < lim ? 1u : 0u) << f;

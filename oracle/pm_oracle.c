@@ -699,10 +699,13 @@ static int64_t skip_empty(const or_grid *g, const float o[3], const float d[3], 
  * U = sum over the cell's 27-cell stencil of W(dist(x_j, B)) bounds the density
  * of every point inside (W falls with r; the stencil holds every particle
  * within h).  A sub-box's bit is set when U of its quarter box, or else its own
- * U, is below iso (1 - 2^-9) -- except in cells whose centre row (dz = dy = 0)
- * alone takes every quarter box's U past that limit: those lie inside the
- * fluid and get no bit (a clear bit only means the sample is evaluated).  A
- * cell holds 8 words: sub-box (fx,fy,fz) is bit
+ * U, is below the limit iso (1 - 2^-9) -- except in cells whose centre row
+ * (dz = dy = 0) alone takes every quarter box's U past that limit (inside the
+ * fluid: no bit) and in quarter boxes whose U reaches 4 x the limit (no bit; a
+ * clear bit only means the sample is evaluated; not for the zero-density bound,
+ * iso = OR_ZERO_ISO, where a quarter box's U only says that some particle lies
+ * within h of it).  A cell holds 8 words:
+ * sub-box (fx,fy,fz) is bit
  * 8 l + (fx&1) + 2 (fy&1) + 4 (fz&1) (bit b of word b / 64) with
  * l = fx/2 + 4 (fy/2) + 16 (fz/2) its quarter box.  Border cells and cells with
  * an empty 27-neighbourhood get every bit.  Each U sums the centre row
@@ -779,9 +782,11 @@ static void bound_cell_bits(or_grid *g, float lim, int quarter_only, int32_t x, 
             hi[a] = ((float)k[a] + 0.25f * (float)(sq[a] + 1)) * h + delta;
         }
         unsigned byte = 0;
-        if (bound_sum(g, x, y, z, lo, hi, lim, 9) < lim) {
+        const float lim4 = g->bound_iso == OR_ZERO_ISO ? INFINITY : 4.0f * lim;
+        const float Uq = bound_sum(g, x, y, z, lo, hi, quarter_only ? lim : lim4, 9);
+        if (Uq < lim) {
             byte = 0xffu;                       /* the quarter box's bound covers its eight */
-        } else if (!quarter_only) {
+        } else if (!quarter_only && Uq < lim4) {
             for (int f = 0; f < 8; f++) {
                 const int fi[3] = { 2 * sq[0] + (f & 1), 2 * sq[1] + ((f >> 1) & 1), 2 * sq[2] + (f >> 2) };
                 for (int a = 0; a < 3; a++) {
