@@ -202,9 +202,10 @@ int pm_get_skip_grids(pm_ctx *ctx, int32_t frame, uint8_t *empty27, int32_t dg_d
 /* The frame's sub-box bound bits for iso and bound_boxes `boxes` (PM_SKIP_BOUND):
  * bits[8 ncells], 512 bits per cell.  Sub-box (fx, fy, fz), f in 0..7 per axis,
  * is bit 8 l + (fx & 1) + 2 (fy & 1) + 4 (fz & 1) of the cell's words with
- * l = fx/2 + 4 (fy/2) + 16 (fz/2) (bit b in word b / 64), set when the sub-box
- * stays below iso (boxes = 1: each byte 0xff or 0, the 4x4x4 boxes' bounds).
- * Builds them if the frame has none for this iso and boxes. */
+ * l = fx/2 + 4 (fy/2) + 16 (fz/2) (bit b in word b / 64); a set bit means the
+ * density bound proves the sub-box below iso, a clear bit proves nothing
+ * (boxes = 1: each byte 0xff or 0, the 4x4x4 boxes' bounds alone).  Builds
+ * them if the frame has none for this iso and boxes. */
 int pm_get_bound_grid(pm_ctx *ctx, int32_t frame, float iso, int32_t boxes, uint64_t *bits);
 
 /* RayMarcher::Prepare + Start (RayMarcher.cpp:76-112).  inv_pv is
