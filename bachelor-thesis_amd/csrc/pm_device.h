@@ -498,13 +498,11 @@ __device__ __forceinline__ void pm_acc6(const GridDev &g, float px, float py, fl
 }
 
 // The six central-difference densities of q = (x[0], x[1], x[2]) in one walk.
-// buf: the lane's compaction slots (lane-interleaved, CAP floats; entries of
-// three floats), or nullptr to add every kept candidate at once.
-template <int CAP>
-__device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy, float qz, float eps, float *buf,
-                                         bool comp)
+// A kept candidate is added to the six sums at once: about nine in ten pass the
+// union test, so compacting them first (through LDS, as the single-sample
+// walk does) costs more than it saves.
+__device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy, float qz, float eps, bool comp)
 {
-    constexpr int CAPE = CAP / 3;
     Cdiff6 out;
     const float x[3] = { qx, qy, qz };
     const float xp[3] = { qx + eps, qy + eps, qz + eps }, xm[3] = { qx - eps, qy - eps, qz - eps };
@@ -521,7 +519,6 @@ __device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy
 #pragma unroll
     for (int k = 0; k < 6; k++) { a.s[k] = 0.0f; a.c[k] = 0.0f; }
     uint32_t cand = 0;
-    int n = 0;
     int jy = 0, jz = 0;
     uint32_t nb, ne;
     pm_box_row(g, w, 0, 0, nb, ne);
@@ -535,49 +532,17 @@ __device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy
         if (!any) continue;
         const int m = (int)(e - b);
         const float4 *const p0 = g.fsorted + b;
-        if (buf == nullptr) {
-            for (int i = 0; i < m; i++) {
-                const float4 p = p0[i];
-                const float rx = p.x - qx, ry = p.y - qy, rz = p.z - qz;
-                if ((rx * rx + ry * ry) + rz * rz < R2) pm_acc6(g, p.x, p.y, p.z, x, xp, xm, comp, a);
-            }
-            continue;
-        }
-        if (__ballot(n + m >= CAPE) != 0) {              // drain all lanes together
-            for (int j = 0; j < n; j++)
-                pm_acc6(g, buf[(3 * j) * 64], buf[(3 * j + 1) * 64], buf[(3 * j + 2) * 64], x, xp, xm, comp, a);
-            n = 0;
-        }
-        if (m < CAPE) {
-            for (int i = 0; i < m; i += 4) {
-                float4 v[4];
+        for (int i = 0; i < m; i += 4) {
+            float4 v[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+            for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
-                    buf[(3 * n) * 64] = v[u].x;
-                    buf[(3 * n + 1) * 64] = v[u].y;
-                    buf[(3 * n + 2) * 64] = v[u].z;
-                    n += (i + u < m && (rx * rx + ry * ry) + rz * rz < R2) ? 1 : 0;
-                }
-            }
-        } else {                                          // a row longer than the buffer
-            for (int i = 0; i < m; i += 4) {
-                float4 v[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
-                    if (i + u < m && (rx * rx + ry * ry) + rz * rz < R2) pm_acc6(g, v[u].x, v[u].y, v[u].z, x, xp, xm, comp, a);
-                }
+            for (int u = 0; u < 4; u++) {
+                const float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
+                if (i + u < m && (rx * rx + ry * ry) + rz * rz < R2) pm_acc6(g, v[u].x, v[u].y, v[u].z, x, xp, xm, comp, a);
             }
         }
     }
-    if (buf != nullptr)
-        for (int j = 0; j < n; j++)
-            pm_acc6(g, buf[(3 * j) * 64], buf[(3 * j + 1) * 64], buf[(3 * j + 2) * 64], x, xp, xm, comp, a);
     out.cand = cand;
 #pragma unroll
     for (int k = 0; k < 6; k++) out.v[k] = comp ? a.s[k] + a.c[k] : a.s[k];

@@ -52,8 +52,9 @@ __device__ __forceinline__ void start_bracket(const MarchLaunch &L, RayState &r,
 
 // First half of one step: the sample point of the lane's next FP32 density
 // evaluation, after the skips of the march phase; false when the lane takes
-// no sample this step (the ray ended, or it used up its skips).  Past an
+// no sample this step (the ray ended, or it used up its HOPS skips).  Past an
 // exactly skipped stretch the learned march may ask for proposals again.
+template <int HOPS>
 __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, float &qx, float &qy, float &qz)
 {
     const pm_settings &s = L.s;
@@ -61,9 +62,9 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
     const float o0 = L.cam[0], o1 = L.cam[1], o2 = L.cam[2];
 
     if (r.phase == PH_MARCH) {
-        // at most 16 skips per step: a lane crossing much skippable space then
-        // idles for a step instead of holding the whole wave in this loop (a
-        // skip costs a few dozen instructions, a step a density walk)
+        // at most HOPS skips per step: a lane crossing much skippable space
+        // then idles for a step instead of holding the whole wave in this loop
+        // (a skip costs a few dozen instructions, a step a density walk)
         for (int hop = 0;; hop++) {
             float t = r.tn + (float)r.k * s.step;
             if (t > r.tf || (s.max_steps > 0 && r.k >= s.max_steps)) {
@@ -79,7 +80,7 @@ __device__ __forceinline__ bool ray_prepare(const MarchLaunch &L, RayState &r, f
             if (kn == r.k) break;
             r.k = kn;
             r.lrn = s.learned;
-            if (hop == 15) return false;
+            if (hop == HOPS - 1) return false;
         }
     } else {  // PH_BISECT
         float tm = (r.lo + r.hi) * 0.5f;
@@ -138,12 +139,14 @@ __device__ __forceinline__ void ray_finish(const MarchLaunch &L, RayState &r, fl
 }
 
 // One density evaluation of this ray, whatever its phase, and the transition
-// it implies.  Dens(q, cand) is the FP32 density.
-template <bool INSTR, class Dens>
+// it implies.  Dens(q, cand) is the FP32 density.  HOPS: skips per step before
+// a lane idles -- 16 in the single-view march, whose end is spent on few long
+// rays with idle lanes beside them; 8 in the batch, whose lanes stay busy.
+template <bool INSTR, int HOPS, class Dens>
 __device__ __forceinline__ void ray_step(const MarchLaunch &L, RayState &r, Dens dens, Counts &cn)
 {
     float qx, qy, qz;
-    if (!ray_prepare(L, r, qx, qy, qz)) return;
+    if (!ray_prepare<HOPS>(L, r, qx, qy, qz)) return;
     uint32_t cand = 0;
     float rho = dens(qx, qy, qz, cand);
     ray_finish<INSTR>(L, r, rho, cand, qx, qy, qz, cn);
@@ -684,7 +687,7 @@ __device__ __forceinline__ void finish_hit(const MarchLaunch &L, uint32_t px, ui
         const bool comp = s.compensated != 0;
         const float e = s.normal_eps;
         float gv[3];
-        const Cdiff6 o = pm_cdiff6<kLdsCap>(g, xs[0], xs[1], xs[2], e, buf, comp);
+        const Cdiff6 o = pm_cdiff6(g, xs[0], xs[1], xs[2], e, comp);
         if (o.cand != 0xffffffffu) {
             cn.samples += 6;
             cn.cand += o.cand;
@@ -830,8 +833,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         if (exhausted && group_ok) regroup(r, G, region);
         if (G == 1) {
             if (r.phase == PH_MARCH || r.phase == PH_BISECT) {
-                if (use_lds) ray_step<INSTR>(L, r, DensLds<CAP, 4>{ &L.g, buf }, cn);
-                else ray_step<INSTR>(L, r, DensPlain{ &L.g, false }, cn);
+                if (use_lds) ray_step<INSTR, 16>(L, r, DensLds<CAP, 4>{ &L.g, buf }, cn);
+                else ray_step<INSTR, 16>(L, r, DensPlain{ &L.g, false }, cn);
             }
         } else {
             const int j = (int)(lane & (uint32_t)(G - 1));
@@ -1051,8 +1054,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         if (group_ok && draining) regroup(r, G, region);
         if (G == 1) {
             if (r.phase == PH_MARCH || r.phase == PH_BISECT) {
-                if (use_lds) ray_step<false>(*Lp, r, DensLds<kLdsCap, 4>{ &Lp->g, buf }, cn);
-                else ray_step<false>(*Lp, r, DensPlain{ &Lp->g, false }, cn);
+                if (use_lds) ray_step<false, 8>(*Lp, r, DensLds<kLdsCap, 4>{ &Lp->g, buf }, cn);
+                else ray_step<false, 8>(*Lp, r, DensPlain{ &Lp->g, false }, cn);
             }
         } else {
             const int j = (int)(lane & (uint32_t)(G - 1));
