@@ -498,9 +498,8 @@ __device__ __forceinline__ void pm_acc6(const GridDev &g, float px, float py, fl
 }
 
 // The six central-difference densities of q = (x[0], x[1], x[2]) in one walk.
-// A kept candidate is added to the six sums at once: about nine in ten pass the
-// union test, so compacting them first (through LDS, as the single-sample
-// walk does) costs more than it saves.
+// A kept candidate is added to the six sums at once (about nine in ten pass the
+// union test).
 __device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy, float qz, float eps, bool comp)
 {
     Cdiff6 out;
@@ -543,6 +542,85 @@ __device__ __forceinline__ Cdiff6 pm_cdiff6(const GridDev &g, float qx, float qy
             }
         }
     }
+    out.cand = cand;
+#pragma unroll
+    for (int k = 0; k < 6; k++) out.v[k] = comp ? a.s[k] + a.c[k] : a.s[k];
+    return out;
+}
+
+// The same six densities with the kept candidates compacted first through the
+// lane's LDS slots (buf: lane-interleaved, CAP floats; entries of three floats)
+// and added from there: better where every lane of the wave finishes a hit at
+// once (the finish kernel), worse inside the march kernel's own finishing.
+template <int CAP>
+__device__ __forceinline__ Cdiff6 pm_cdiff6_lds(const GridDev &g, float qx, float qy, float qz, float eps, float *buf,
+                                             bool comp)
+{
+    constexpr int CAPE = CAP / 3;
+    Cdiff6 out;
+    const float x[3] = { qx, qy, qz };
+    const float xp[3] = { qx + eps, qy + eps, qz + eps }, xm[3] = { qx - eps, qy - eps, qz - eps };
+    BoxWalk w;
+    if (!pm_box_begin(g, xm, xp, w)) {
+        out.cand = 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < 6; k++) out.v[k] = 0.0f;
+        return out;
+    }
+    const float he = (g.h + eps) * 1.0009765625f;
+    const float R2 = he * he;
+    Acc6 a;
+#pragma unroll
+    for (int k = 0; k < 6; k++) { a.s[k] = 0.0f; a.c[k] = 0.0f; }
+    uint32_t cand = 0;
+    int n = 0;
+    int jy = 0, jz = 0;
+    uint32_t nb, ne;
+    pm_box_row(g, w, 0, 0, nb, ne);
+    const int nr = w.ny * w.nz;
+    _Pragma("unroll 1") for (int r = 0; __ballot(r < nr) != 0; r++) {
+        const uint32_t b = nb, e = ne;
+        cand += e - b;
+        const bool any = __ballot(e > b) != 0;
+        if (++jy == w.ny) { jy = 0; jz++; }
+        pm_box_row(g, w, jy, jz, nb, ne);                 // next row in flight
+        if (!any) continue;
+        const int m = (int)(e - b);
+        const float4 *const p0 = g.fsorted + b;
+        if (__ballot(n + m >= CAPE) != 0) {              // drain all lanes together
+            for (int j = 0; j < n; j++)
+                pm_acc6(g, buf[(3 * j) * 64], buf[(3 * j + 1) * 64], buf[(3 * j + 2) * 64], x, xp, xm, comp, a);
+            n = 0;
+        }
+        if (m < CAPE) {
+            for (int i = 0; i < m; i += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
+                    buf[(3 * n) * 64] = v[u].x;
+                    buf[(3 * n + 1) * 64] = v[u].y;
+                    buf[(3 * n + 2) * 64] = v[u].z;
+                    n += (i + u < m && (rx * rx + ry * ry) + rz * rz < R2) ? 1 : 0;
+                }
+            }
+        } else {                                          // a row longer than the buffer
+            for (int i = 0; i < m; i += 4) {
+                float4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = p0[i + u < m ? i + u : i];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float rx = v[u].x - qx, ry = v[u].y - qy, rz = v[u].z - qz;
+                    if (i + u < m && (rx * rx + ry * ry) + rz * rz < R2) pm_acc6(g, v[u].x, v[u].y, v[u].z, x, xp, xm, comp, a);
+                }
+            }
+        }
+    }
+    for (int j = 0; j < n; j++)
+        pm_acc6(g, buf[(3 * j) * 64], buf[(3 * j + 1) * 64], buf[(3 * j + 2) * 64], x, xp, xm, comp, a);
     out.cand = cand;
 #pragma unroll
     for (int k = 0; k < 6; k++) out.v[k] = comp ? a.s[k] + a.c[k] : a.s[k];
